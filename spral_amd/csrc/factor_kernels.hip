@@ -1,0 +1,1067 @@
+// Numeric factorization kernels for gfx950 (CDNA4, wave64).
+//
+// One tree level is processed per launch sequence; a launch covers every
+// front of the level.  Workgroups find their front by binary search in a
+// per-level prefix array of workgroup counts (wg_off).
+//
+//  k_assemble   owner-computes extend-add: a workgroup owns 32 columns of one
+//               parent front, zeroes them, scatters A, then adds each child's
+//               trailing matrix (children in order => deterministic sums)
+//               through an LDS-staged child->parent row map.
+//  k_diag       one workgroup per front: factor the w<=64 column diagonal
+//               block with complete pivoting (1x1/2x2) in LDS
+//               (pivot rules of src/ssids/cpu/kernels/block_ldlt.hxx:213-414).
+//  k_apply      L = A Lbb^{-T} D^{-1} for the rows below the block, keeping a
+//               backup and the column max |l| (a posteriori test,
+//               src/ssids/cpu/kernels/ldlt_app.cxx:304-414).
+//  k_update     restore failed columns and apply the accepted pivots to the
+//               trailing lower matrix (panel and contribution block) with
+//               v_mfma_f64_16x16x4_f64 on 64x64 tiles.
+//  k_swap_*     move failed columns behind the unprocessed candidates.
+//  k_tpp        threshold partial pivoting on the failed columns
+//               (src/ssids/cpu/kernels/ldlt_tpp.cxx:166-270).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "dev.hpp"
+
+namespace ssids_hip {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline int find_node(const int *wg_off, int nlvl, int b) {
+  int lo = 0, hi = nlvl;  // wg_off[lo] <= b < wg_off[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (wg_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// element (i,j), i>=j, of the front (panel for j<n, contribution block else)
+__device__ inline double *front_at(const NodeDev &N, int i, int j) {
+  if (j < N.n) return N.L + (size_t)j * N.ldl + i;
+  return N.C + (size_t)(j - N.n) * N.ldc + (i - N.n);
+}
+
+// parent position of trailing index k of child c (monotone in k)
+__device__ inline int child_pos(const NodeDev &c, int doff, int pnd, int k) {
+  if (k < c.n) return doff + (k - c.nelim);
+  return pnd + c.rmap[k - c.n];
+}
+
+__device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
+  if (k < c.n) return c.L + (size_t)k * c.ldl + kr;
+  return c.C + (size_t)(k - c.n) * c.ldc + (kr - c.n);
+}
+
+// ---------------------------------------------------------------------------
+// Assembly
+// ---------------------------------------------------------------------------
+constexpr int kAsmCols = 32;
+constexpr int kAsmMap = 8192;  // LDS entries for the staged child row map
+
+__global__ __launch_bounds__(kThreads) void k_assemble(
+    NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
+    const int *child_list, const double *aval, const int64_t *asrc,
+    const int *arow, const double *scaling) {
+  __shared__ int smap[kAsmMap];
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  const int s = lvl_slots[li];
+  NodeDev &P = nodes[s];
+  const int m = P.m, n = P.n, nd = P.nd;
+  const int J0 = (blockIdx.x - wg_off[li]) * kAsmCols;
+  if (J0 >= m) return;
+  const int J1 = min(J0 + kAsmCols, m);
+  const int tid = threadIdx.x;
+
+  // fully-summed variable list for owned columns
+  for (int j = J0 + tid; j < min(J1, n); j += kThreads) {
+    if (j >= nd) {
+      P.perm[j] = P.rows[j - nd];
+      P.perm0[j] = P.rows[j - nd];
+    } else {
+      int doff = 0;
+      for (int ci = P.cbeg; ci < P.cend; ++ci) {
+        const NodeDev &c = nodes[child_list[ci]];
+        int nf = c.n - c.nelim;
+        if (j >= doff && j < doff + nf) {
+          P.perm[j] = c.perm[c.nelim + (j - doff)];
+          P.perm0[j] = P.perm[j];
+          break;
+        }
+        doff += nf;
+      }
+    }
+  }
+  // zero the owned columns (lower part)
+  for (int j = J0; j < J1; ++j) {
+    if (j < n) {
+      double *col = P.L + (size_t)j * P.ldl;
+      for (int i = j + tid; i < m; i += kThreads) col[i] = 0.0;
+    } else {
+      double *col = P.C + (size_t)(j - n) * P.ldc - n;
+      for (int i = j + tid; i < m; i += kThreads) col[i] = 0.0;
+    }
+  }
+  __syncthreads();
+  // A entries of owned fully-summed own columns
+  {
+    int c0 = max(J0, nd) - nd, c1 = min(J1, n) - nd;
+    if (c1 > c0) {
+      int64_t e0 = P.acol[c0], e1 = P.acol[c1];
+      for (int64_t e = e0 + tid; e < e1; e += kThreads) {
+        int r = arow[e];
+        // column of entry e: search acol
+        int lo = c0, hi = c1;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (P.acol[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        int c = lo;
+        double v = aval[asrc[e]];
+        if (scaling) v *= scaling[P.rows[r]] * scaling[P.rows[c]];
+        P.L[(size_t)(c + nd) * P.ldl + (r + nd)] = v;
+      }
+    }
+  }
+  __syncthreads();
+  // children
+  int doff = 0;
+  for (int ci = P.cbeg; ci < P.cend; ++ci) {
+    const NodeDev &c = nodes[child_list[ci]];
+    const int kb0 = c.nelim, ke0 = c.m;
+    const int cdoff = doff;
+    doff += c.n - c.nelim;
+    if (ke0 <= kb0) continue;
+    // first trailing index with pos >= J0, and >= J1
+    int lo = kb0, hi = ke0;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (child_pos(c, cdoff, nd, mid) < J0) lo = mid + 1;
+      else hi = mid;
+    }
+    const int kb = lo;
+    hi = ke0;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (child_pos(c, cdoff, nd, mid) < J1) lo = mid + 1;
+      else hi = mid;
+    }
+    const int ke = lo;
+    if (ke <= kb) continue;
+    const int nmap = ke0 - kb;
+    const bool staged = nmap <= kAsmMap;
+    if (staged) {
+      for (int t = tid; t < nmap; t += kThreads) smap[t] = child_pos(c, cdoff, nd, kb + t);
+      __syncthreads();
+    }
+    for (int k = kb; k < ke; ++k) {
+      const int pc = staged ? smap[k - kb] : child_pos(c, cdoff, nd, k);
+      const double *src = child_src(c, k, k);  // column k from row k
+      double *dst = (pc < n) ? P.L + (size_t)pc * P.ldl
+                             : P.C + (size_t)(pc - n) * P.ldc - n;
+      for (int kr = k + tid; kr < ke0; kr += kThreads) {
+        int pr = staged ? smap[kr - kb] : child_pos(c, cdoff, nd, kr);
+        dst[pr] += src[kr - k];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block factorization (complete pivoting inside the block)
+// ---------------------------------------------------------------------------
+struct MaxLoc {
+  double v;
+  int r, c;
+};
+
+__device__ inline bool better(const MaxLoc &a, const MaxLoc &b) {
+  // larger value first; ties: smaller column, then smaller row
+  if (a.v != b.v) return a.v > b.v;
+  if (a.c != b.c) return a.c < b.c;
+  return a.r < b.r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lvl_slots,
+                                                   int step, FactorParams fp) {
+  __shared__ double a[kBlk][kBlk + 1];
+  __shared__ double dinv[2 * kBlk];
+  __shared__ int lp[kBlk];
+  __shared__ MaxLoc red[kThreads / 64];
+  __shared__ int sh_p, sh_w, sh_piv1, sh_piv2, sh_size, sh_stop;
+  __shared__ double sh_d[4];
+  const int s = lvl_slots[blockIdx.x];
+  NodeDev &N = nodes[s];
+  if (step >= N.nsteps) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int p, q;
+    if (step == 0) {
+      p = 0;
+      q = N.n;
+    } else {
+      p = N.p + N.npass;
+      q = N.q - (N.w - N.npass);
+    }
+    int w = min(kBlk, q - p);
+    if (w < 0) w = 0;
+    N.p = p;
+    N.q = q;
+    N.w = w;
+    N.npass = 0;
+    sh_p = p;
+    sh_w = w;
+  }
+  __syncthreads();
+  const int p = sh_p, w = sh_w;
+  if (w == 0) return;
+  const int ldl = N.ldl;
+  double *L = N.L;
+  for (int idx = tid; idx < w * w; idx += kThreads) {
+    int r = idx % w, c = idx / w;
+    if (r >= c) {
+      double v = L[(size_t)(p + c) * ldl + p + r];
+      a[r][c] = v;
+      a[c][r] = v;
+      N.dbak[r + c * kBlk] = v;
+      N.dbak[c + r * kBlk] = v;
+    }
+  }
+  for (int i = tid; i < kBlk; i += kThreads) {
+    lp[i] = i;
+    N.colmax[i] = 0ull;
+  }
+  __syncthreads();
+
+  int k = 0;
+  while (k < w) {
+    // ---- max over the remaining lower triangle
+    MaxLoc best{-1.0, w, w};
+    const int rem = w - k;
+    for (int idx = tid; idx < rem * rem; idx += kThreads) {
+      int r = k + idx % rem, c = k + idx / rem;
+      if (r >= c) {
+        MaxLoc cand{fabs(a[r][c]), r, c};
+        if (better(cand, best)) best = cand;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      MaxLoc o;
+      o.v = __shfl_xor(best.v, off);
+      o.r = __shfl_xor(best.r, off);
+      o.c = __shfl_xor(best.c, off);
+      if (better(o, best)) best = o;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = best;
+    __syncthreads();
+    if (tid == 0) {
+      MaxLoc b = red[0];
+      for (int i = 1; i < kThreads / 64; ++i)
+        if (better(red[i], b)) b = red[i];
+      sh_stop = 0;
+      if (!(b.v >= fp.small)) {  // all remaining entries are (near) zero
+        sh_stop = 1;
+        sh_size = 0;
+      } else if (b.r == b.c) {
+        sh_size = 1;
+        sh_piv1 = b.r;
+      } else {
+        int mcol = b.c, trow = b.r;  // mcol < trow
+        double a11 = a[mcol][mcol], a22 = a[trow][trow], a21 = a[trow][mcol];
+        double detscale = 1.0 / fabs(a21);
+        double detpiv = (a11 * detscale) * a22 - fabs(a21);
+        if (fabs(detpiv) >= fabs(a21) / 2) {
+          sh_size = 2;
+          sh_piv1 = mcol;
+          sh_piv2 = trow;
+          sh_d[0] = (a22 * detscale) / detpiv;
+          sh_d[1] = (-a21 * detscale) / detpiv;
+          sh_d[2] = (a11 * detscale) / detpiv;
+        } else {
+          sh_size = 1;
+          sh_piv1 = (fabs(a11) > fabs(a22)) ? mcol : trow;
+        }
+      }
+    }
+    __syncthreads();
+    if (sh_stop) {
+      // zero pivots for all remaining columns
+      for (int j = k + tid; j < w; j += kThreads) {
+        dinv[2 * j] = 0.0;
+        dinv[2 * j + 1] = 0.0;
+      }
+      for (int idx = tid; idx < rem * rem; idx += kThreads) {
+        int r = k + idx % rem, c = k + idx / rem;
+        if (r >= c) a[r][c] = 0.0;
+      }
+      if (tid == 0) {
+        N.nzero += w - k;
+        if (!fp.action) N.flag = -5;
+      }
+      __syncthreads();
+      break;
+    }
+    const int size = sh_size;
+    // ---- symmetric interchanges (full storage: rows then columns)
+    for (int sw = 0; sw < size; ++sw) {
+      const int dst = k + sw, src = (sw == 0) ? sh_piv1 : sh_piv2;
+      if (dst != src) {
+        for (int x = tid; x < w; x += kThreads) {
+          double t = a[dst][x];
+          a[dst][x] = a[src][x];
+          a[src][x] = t;
+        }
+        __syncthreads();
+        for (int x = tid; x < w; x += kThreads) {
+          double t = a[x][dst];
+          a[x][dst] = a[x][src];
+          a[x][src] = t;
+        }
+        if (tid == 0) {
+          int t = lp[dst];
+          lp[dst] = lp[src];
+          lp[src] = t;
+        }
+        __syncthreads();
+      }
+    }
+    if (size == 1) {
+      const double piv = a[k][k];
+      const double di = 1.0 / piv;
+      // trailing update a[i][j] -= a[i][k] a[j][k] / piv  (i,j > k)
+      const int r2 = w - k - 1;
+      for (int idx = tid; idx < r2 * r2; idx += kThreads) {
+        int i = k + 1 + idx % r2, j = k + 1 + idx / r2;
+        a[i][j] -= a[i][k] * a[j][k] * di;
+      }
+      __syncthreads();
+      for (int i = k + 1 + tid; i < w; i += kThreads) {
+        a[i][k] *= di;
+        a[k][i] = a[i][k];
+      }
+      if (tid == 0) {
+        dinv[2 * k] = di;
+        dinv[2 * k + 1] = 0.0;
+        a[k][k] = 1.0;
+      }
+      __syncthreads();
+      k += 1;
+    } else {
+      const double d11 = sh_d[0], d21 = sh_d[1], d22 = sh_d[2];
+      const int r2 = w - k - 2;
+      for (int idx = tid; idx < r2 * r2; idx += kThreads) {
+        int i = k + 2 + idx % r2, j = k + 2 + idx / r2;
+        double x1 = a[i][k], x2 = a[i][k + 1];
+        double l1 = d11 * x1 + d21 * x2, l2 = d21 * x1 + d22 * x2;
+        a[i][j] -= l1 * a[j][k] + l2 * a[j][k + 1];
+      }
+      __syncthreads();
+      for (int i = k + 2 + tid; i < w; i += kThreads) {
+        double x1 = a[i][k], x2 = a[i][k + 1];
+        double l1 = d11 * x1 + d21 * x2, l2 = d21 * x1 + d22 * x2;
+        a[i][k] = l1;
+        a[i][k + 1] = l2;
+      }
+      if (tid == 0) {
+        dinv[2 * k] = d11;
+        dinv[2 * k + 1] = d21;
+        dinv[2 * k + 2] = INFINITY;
+        dinv[2 * k + 3] = d22;
+        a[k][k] = 1.0;
+        a[k + 1][k] = 0.0;
+        a[k + 1][k + 1] = 1.0;
+      }
+      __syncthreads();
+      k += 2;
+    }
+  }
+  // ---- write back L_bb, D^{-1}, local permutation
+  for (int idx = tid; idx < w * w; idx += kThreads) {
+    int r = idx % w, c = idx / w;
+    if (r >= c) L[(size_t)(p + c) * ldl + p + r] = a[r][c];
+  }
+  for (int j = tid; j < w; j += kThreads) {
+    N.D[2 * (p + j)] = dinv[2 * j];
+    N.D[2 * (p + j) + 1] = dinv[2 * j + 1];
+    N.lp[j] = lp[j];
+  }
+  // permute the variable list and the rows of the eliminated columns left of p
+  __shared__ int ptmp[kBlk];
+  if (tid < w) ptmp[tid] = N.perm[p + lp[tid]];
+  __syncthreads();
+  if (tid < w) N.perm[p + tid] = ptmp[tid];
+  for (int c = tid; c < p; c += kThreads) {
+    double *col = L + (size_t)c * ldl + p;
+    double t[kBlk];
+#pragma unroll
+    for (int i = 0; i < kBlk; ++i)
+      if (i < w) t[i] = col[lp[i]];
+#pragma unroll
+    for (int i = 0; i < kBlk; ++i)
+      if (i < w) col[i] = t[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Apply the block pivots to the rows below and record column maxima
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *lvl_slots,
+                                                    const int *wg_off, int nlvl, int step,
+                                                    FactorParams fp) {
+  __shared__ double lbb[kBlk][kBlk + 1];
+  __shared__ double dv[2 * kBlk + 2];
+  __shared__ int lps[kBlk];
+  __shared__ double cmax[kThreads / 64][kBlk];
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  NodeDev &N = nodes[lvl_slots[li]];
+  if (step >= N.nsteps) return;
+  const int p = N.p, w = N.w;
+  if (w == 0) return;
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ldl = N.ldl;
+  double *L = N.L;
+  for (int idx = tid; idx < w * w; idx += kThreads) {
+    int r = idx % w, c = idx / w;
+    lbb[r][c] = (r > c) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
+  }
+  for (int j = tid; j < 2 * w; j += kThreads) dv[j] = N.D[2 * p + j];
+  if (tid == 0) { dv[2 * w] = 0.0; dv[2 * w + 1] = 0.0; }
+  for (int j = tid; j < kBlk; j += kThreads) lps[j] = (j < w) ? N.lp[j] : 0;
+  __syncthreads();
+  double mx[kBlk];
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) mx[j] = 0.0;
+  const int r0 = p + w, m = N.m;
+  const double small = fp.small;
+  for (int r = r0 + local * kThreads + tid; r < m; r += count * kThreads) {
+    double x[kBlk];
+#pragma unroll
+    for (int j = 0; j < kBlk; ++j)
+      if (j < w) x[j] = L[(size_t)(p + lps[j]) * ldl + r];
+#pragma unroll
+    for (int j = 0; j < kBlk; ++j)
+      if (j < w) N.bak[(size_t)j * N.ldbak + (r - r0)] = x[j];
+    // x L_bb^T = a  (forward substitution)
+#pragma unroll
+    for (int j = 1; j < kBlk; ++j) {
+      if (j < w) {
+        double acc = x[j];
+#pragma unroll
+        for (int t = 0; t < j; ++t) acc -= x[t] * lbb[j][t];
+        x[j] = acc;
+      }
+    }
+    // x D^{-1}
+#pragma unroll
+    for (int j = 0; j < kBlk; ++j) {
+      if (j < w) {
+        bool two_first = (j + 1 < w) && isinf(dv[2 * j + 2]);
+        bool two_second = (j > 0) && isinf(dv[2 * j]);
+        if (two_first) {
+          double d11 = dv[2 * j], d21 = dv[2 * j + 1], d22 = dv[2 * j + 3];
+          double x1 = x[j], x2 = x[j + 1];
+          x[j] = d11 * x1 + d21 * x2;
+          x[j + 1] = d21 * x1 + d22 * x2;
+        } else if (!two_second) {
+          double d11 = dv[2 * j];
+          if (d11 == 0.0) x[j] = (fabs(x[j]) < small) ? 0.0 : INFINITY * x[j];
+          else x[j] *= d11;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kBlk; ++j) {
+      if (j < w) {
+        L[(size_t)(p + j) * ldl + r] = x[j];
+        double v = fabs(x[j]);
+        if (!(v <= mx[j])) mx[j] = v;  // NaN propagates as a failure
+      }
+    }
+  }
+  // reduce column maxima: wave then workgroup, one atomic per column
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) {
+    double v = mx[j];
+    for (int off = 32; off > 0; off >>= 1) {
+      double o = __shfl_xor(v, off);
+      v = (o > v || isnan(o)) ? o : v;
+    }
+    if (lane == 0) cmax[wave][j] = v;
+  }
+  __syncthreads();
+  if (tid < w) {
+    double v = cmax[0][tid];
+    for (int i = 1; i < kThreads / 64; ++i) {
+      double o = cmax[i][tid];
+      v = (o > v || isnan(o)) ? o : v;
+    }
+    unsigned long long bits = isnan(v) ? 0x7ff0000000000001ull
+                                       : (unsigned long long)__double_as_longlong(v);
+    atomicMax(&N.colmax[tid], bits);
+  }
+}
+
+// number of block pivots that pass the threshold test (split 2x2 avoided)
+__device__ inline int block_npass(const NodeDev &N, double u) {
+  const int w = N.w, p = N.p;
+  const double lim = 1.0 / u;
+  const unsigned long long lb = (unsigned long long)__double_as_longlong(lim);
+  int np = w;
+  for (int j = 0; j < w; ++j) {
+    unsigned long long b = __atomic_load_n(&N.colmax[j], __ATOMIC_RELAXED);
+    if (b > lb) { np = j; break; }
+  }
+  if (np > 0 && np < w) {
+    double d11 = N.D[2 * (p + np - 1)], d21 = N.D[2 * (p + np - 1) + 1];
+    if (isfinite(d11) && d21 != 0.0) np--;
+  }
+  return np;
+}
+
+// D for pivot k (from the stored D^{-1}); two: 0 = 1x1, 1 = first of 2x2,
+// 2 = second of 2x2.
+__device__ inline void pivot_d(const double *D, int k, int kend, double &e1, double &e2,
+                               int &two) {
+  double d11 = D[2 * k];
+  if (!isfinite(d11)) {  // second of a 2x2 block
+    double a11 = D[2 * (k - 1)], a21 = D[2 * (k - 1) + 1], a22 = D[2 * k + 1];
+    double det = a11 * a22 - a21 * a21;
+    e1 = -a21 / det;  // off-diagonal of D block
+    e2 = a11 / det;   // D22
+    two = 2;
+  } else if (k + 1 < kend && !isfinite(D[2 * k + 2])) {
+    double a11 = d11, a21 = D[2 * k + 1], a22 = D[2 * k + 3];
+    double det = a11 * a22 - a21 * a21;
+    e1 = a22 / det;   // D11
+    e2 = -a21 / det;  // off-diagonal
+    two = 1;
+  } else {
+    e1 = (d11 != 0.0) ? 1.0 / d11 : 0.0;
+    e2 = 0.0;
+    two = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Trailing update with MFMA:  F[i,j] -= sum_k L[i,a0+k] (L D)[j,a0+k]
+// over the lower part of [t0,m)^2; failed block columns restored first.
+// mode 0: APP step; mode 1: contribution update for TPP pivots.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 64;
+
+__global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *lvl_slots,
+                                                     const int *wg_off, int nlvl, int step,
+                                                     int mode, FactorParams fp) {
+  __shared__ double As[kTile][kBlk + 1];
+  __shared__ double Bs[kTile][kBlk + 1];
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  NodeDev &N = nodes[lvl_slots[li]];
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int a0, K, t0, fa, fb, p = 0, w = 0;
+  if (mode == 0) {
+    if (step >= N.nsteps) return;
+    p = N.p;
+    w = N.w;
+    if (w == 0) return;
+    int np = block_npass(N, fp.u);
+    if (local == 0 && tid == 0) N.npass = np;
+    a0 = p;
+    K = np;
+    t0 = p + np;
+    fa = p + np;
+    fb = p + w;
+  } else {
+    a0 = N.nelim_app;
+    K = N.nelim - N.nelim_app;
+    t0 = N.n;
+    fa = fb = 0;
+    if (K <= 0) return;
+  }
+  const int m = N.m;
+  const int nt = m - t0;
+  if (nt <= 0) return;
+  const int T = (nt + kTile - 1) / kTile;
+  const int ntiles = T * (T + 1) / 2;
+  const double *D = N.D;
+  const int ldl = N.ldl;
+  for (int tile = local; tile < ntiles; tile += count) {
+    int I = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
+    while (I * (I + 1) / 2 > tile) --I;
+    while ((I + 1) * (I + 2) / 2 <= tile) ++I;
+    const int J = tile - I * (I + 1) / 2;
+    const int r0 = t0 + I * kTile, c0 = t0 + J * kTile;
+    const bool has_fail = (fb > fa) && (c0 < fb) && (c0 + kTile > fa);
+    if (K == 0 && !has_fail) continue;
+    const int Kp = (K + 3) & ~3;
+    __syncthreads();
+    // stage L rows of the tile (A) and (L D) rows of the tile columns (B)
+    for (int idx = tid; idx < kTile * Kp; idx += kThreads) {
+      int i = idx % kTile, k = idx / kTile;
+      int gi = r0 + i, gj = c0 + i;
+      double av = 0.0, bv = 0.0;
+      if (k < K) {
+        if (gi < m) av = N.L[(size_t)(a0 + k) * ldl + gi];
+        if (gj < m) {
+          double e1, e2;
+          int two;
+          pivot_d(D, a0 + k, a0 + K, e1, e2, two);
+          double lk = N.L[(size_t)(a0 + k) * ldl + gj];
+          if (two == 0) {
+            bv = lk * e1;
+          } else if (two == 1) {
+            double lk1 = N.L[(size_t)(a0 + k + 1) * ldl + gj];
+            bv = lk * e1 + lk1 * e2;
+          } else {
+            double lkm = N.L[(size_t)(a0 + k - 1) * ldl + gj];
+            bv = lkm * e1 + lk * e2;
+          }
+        }
+      }
+      As[i][k] = av;
+      Bs[i][k] = bv;
+    }
+    __syncthreads();
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < Kp; kk += 4) {
+      const int kl = kk + (lane >> 4);
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        double av = As[wr + bi * 16 + (lane & 15)][kl];
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          double bv = Bs[wc + bj * 16 + (lane & 15)][kl];
+          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+        }
+      }
+    }
+    // write back: element (i,j) of a 16x16 block lives in lane j + 16*(i%4),
+    // register i/4 (measured on gfx950).
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg) {
+          int i = wr + bi * 16 + (lane >> 4) + 4 * rg;
+          int j = wc + bj * 16 + (lane & 15);
+          int gi = r0 + i, gj = c0 + j;
+          if (gi < m && gj < m && gi >= gj) {
+            double *dst = front_at(N, gi, gj);
+            double old;
+            if (gj >= fa && gj < fb) {
+              int jj = gj - p;
+              if (gi < p + w) old = N.dbak[N.lp[gi - p] + N.lp[jj] * kBlk];
+              else old = N.bak[(size_t)jj * N.ldbak + (gi - p - w)];
+            } else {
+              old = *dst;
+            }
+            *dst = old - acc[bi][bj][rg];
+          }
+        }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Move failed columns of this step behind the unprocessed candidates:
+// disjoint interchanges (p+npass+i) <-> (q-ns+i), i < ns = min(nf, nu).
+// Two passes through a gather buffer so the permutation is applied in place.
+// ---------------------------------------------------------------------------
+__device__ inline int swp_count(const NodeDev &N) {
+  int nf = N.w - N.npass;
+  int nu = N.q - (N.p + N.w);
+  if (nf <= 0 || nu <= 0) return 0;
+  return min(nf, nu);
+}
+
+__device__ inline int swp_pi(const NodeDev &N, int ns, int x) {
+  int f0 = N.p + N.npass, u0 = N.q - ns;
+  if (x >= f0 && x < f0 + ns) return u0 + (x - f0);
+  if (x >= u0 && x < u0 + ns) return f0 + (x - u0);
+  return x;
+}
+
+__device__ inline double *swp_at(const NodeDev &N, int a, int b) {
+  int hi = max(a, b), lo = min(a, b);
+  return N.L + (size_t)lo * N.ldl + hi;
+}
+
+__global__ __launch_bounds__(kThreads) void k_swap(NodeDev *nodes, const int *lvl_slots,
+                                                   const int *wg_off, int nlvl, int step,
+                                                   int phase) {
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  NodeDev &N = nodes[lvl_slots[li]];
+  if (step >= N.nsteps || N.w == 0) return;
+  const int ns = swp_count(N);
+  if (ns == 0) return;
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  const int m = N.m, e = N.p + N.npass;
+  const int f0 = e, u0 = N.q - ns;
+  const int total = 2 * ns * m;
+  for (int idx = local * kThreads + threadIdx.x; idx < total; idx += count * kThreads) {
+    int si = idx / m, x = idx % m;
+    int sv = (si < ns) ? f0 + si : u0 + (si - ns);
+    if (phase == 0) {
+      int ps = swp_pi(N, ns, sv);
+      double v;
+      if (x < e) v = N.L[(size_t)x * N.ldl + ps];
+      else v = *swp_at(N, ps, swp_pi(N, ns, x));
+      N.swp[(size_t)si * m + x] = v;
+    } else {
+      double v = N.swp[(size_t)si * m + x];
+      if (x < e) N.L[(size_t)x * N.ldl + sv] = v;
+      else *swp_at(N, sv, x) = v;
+    }
+  }
+  if (phase == 1 && local == 0 && threadIdx.x < ns) {
+    int i = threadIdx.x;
+    int t = N.perm[f0 + i];
+    N.perm[f0 + i] = N.perm[u0 + i];
+    N.perm[u0 + i] = t;
+  }
+}
+
+__global__ void k_finalize(NodeDev *nodes, const int *lvl_slots, int nlvl) {
+  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nlvl) return;
+  NodeDev &N = nodes[lvl_slots[li]];
+  if (N.nsteps > 0) {
+    int p = N.p + N.npass;
+    int q = N.q - (N.w - N.npass);
+    N.p = p;
+    N.q = q;
+    N.nelim = p;
+  } else {
+    N.nelim = 0;
+  }
+  N.nelim_app = N.nelim;
+}
+
+// ---------------------------------------------------------------------------
+// Threshold partial pivoting on the remaining fully-summed columns
+// [nelim, n) of the panel, one workgroup per front.  Updates the panel
+// columns only; the contribution block is updated afterwards (k_update mode 1).
+// ---------------------------------------------------------------------------
+struct TppShared {
+  double red[kThreads / 64];
+  int ired[kThreads / 64];
+  int flag;
+};
+
+__device__ double wg_max(double v, TppShared &sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh.red[0];
+  for (int i = 1; i < kThreads / 64; ++i) r = fmax(r, sh.red[i]);
+  __syncthreads();
+  return r;
+}
+
+__device__ int wg_and(int v, TppShared &sh) {
+  for (int off = 32; off > 0; off >>= 1) v &= __shfl_xor(v, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.ired[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = sh.ired[0];
+  for (int i = 1; i < kThreads / 64; ++i) r &= sh.ired[i];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double &Lat(NodeDev &N, int r, int c) {
+  return N.L[(size_t)c * N.ldl + r];
+}
+
+// all |entries| of row/col idx within [from, to) below small?
+__device__ int tpp_col_small(NodeDev &N, int idx, int from, int to, double small,
+                             TppShared &sh) {
+  int ok = 1;
+  for (int c = from + threadIdx.x; c < idx; c += kThreads)
+    if (!(fabs(Lat(N, idx, c)) < small)) ok = 0;
+  for (int r = idx + threadIdx.x; r < to; r += kThreads)
+    if (!(fabs(Lat(N, r, idx)) < small)) ok = 0;
+  return wg_and(ok, sh);
+}
+
+__device__ double tpp_rc_max(NodeDev &N, int col, int nelim, int m, int skip,
+                             TppShared &sh) {
+  double best = 0.0;
+  for (int c = nelim + threadIdx.x; c < col; c += kThreads)
+    if (c != skip) best = fmax(best, fabs(Lat(N, col, c)));
+  for (int r = col + 1 + threadIdx.x; r < m; r += kThreads)
+    if (r != skip) best = fmax(best, fabs(Lat(N, r, col)));
+  return wg_max(best, sh);
+}
+
+__device__ void tpp_swap(NodeDev &N, int c1, int c2, int m) {
+  if (c1 == c2) return;
+  if (c2 < c1) { int t = c1; c1 = c2; c2 = t; }
+  const int tid = threadIdx.x;
+  for (int c = tid; c < c1; c += kThreads) {
+    double t = Lat(N, c1, c); Lat(N, c1, c) = Lat(N, c2, c); Lat(N, c2, c) = t;
+  }
+  for (int i = c1 + 1 + tid; i < c2; i += kThreads) {
+    double t = Lat(N, i, c1); Lat(N, i, c1) = Lat(N, c2, i); Lat(N, c2, i) = t;
+  }
+  for (int r = c2 + 1 + tid; r < m; r += kThreads) {
+    double t = Lat(N, r, c1); Lat(N, r, c1) = Lat(N, r, c2); Lat(N, r, c2) = t;
+  }
+  if (tid == 0) {
+    double t = Lat(N, c1, c1); Lat(N, c1, c1) = Lat(N, c2, c2); Lat(N, c2, c2) = t;
+    int pt = N.perm[c1]; N.perm[c1] = N.perm[c2]; N.perm[c2] = pt;
+  }
+  __syncthreads();
+}
+
+__device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
+  const double d11 = N.D[2 * k];
+  for (int r = k + 1 + threadIdx.x; r < m; r += kThreads) {
+    ld[r] = Lat(N, r, k);
+    Lat(N, r, k) = ld[r] * d11;
+  }
+  __syncthreads();
+  for (int c = k + 1; c < n; ++c) {
+    double sc = ld[c];
+    for (int r = c + threadIdx.x; r < m; r += kThreads) Lat(N, r, c) -= Lat(N, r, k) * sc;
+  }
+  if (threadIdx.x == 0) Lat(N, k, k) = 1.0;
+  __syncthreads();
+}
+
+__device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *ld2) {
+  const double d11 = N.D[2 * k], d21 = N.D[2 * k + 1], d22 = N.D[2 * k + 3];
+  for (int r = k + 2 + threadIdx.x; r < m; r += kThreads) {
+    double x1 = Lat(N, r, k), x2 = Lat(N, r, k + 1);
+    ld[r] = x1;
+    ld2[r] = x2;
+    Lat(N, r, k) = d11 * x1 + d21 * x2;
+    Lat(N, r, k + 1) = d21 * x1 + d22 * x2;
+  }
+  __syncthreads();
+  for (int c = k + 2; c < n; ++c) {
+    double s1 = ld[c], s2 = ld2[c];
+    for (int r = c + threadIdx.x; r < m; r += kThreads)
+      Lat(N, r, c) -= Lat(N, r, k) * s1 + Lat(N, r, k + 1) * s2;
+  }
+  if (threadIdx.x == 0) {
+    Lat(N, k, k) = 1.0;
+    Lat(N, k + 1, k) = 0.0;
+    Lat(N, k + 1, k + 1) = 1.0;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void k_tpp(NodeDev *nodes, const int *lvl_slots,
+                                                  FactorParams fp) {
+  __shared__ TppShared sh;
+  __shared__ int s_t;
+  NodeDev &N = nodes[lvl_slots[blockIdx.x]];
+  const int n = N.n, m = N.m;
+  int nelim = N.nelim;
+  if (nelim >= n) return;
+  if (!(fp.pivot_tpp || N.is_root || fp.tpp_failed)) return;
+  const double u = fp.u, small = fp.small;
+  double *ld = N.bak, *ld2 = N.bak + N.ldbak;
+  const int tid = threadIdx.x;
+  while (nelim < n) {
+    if (tpp_col_small(N, nelim, nelim, m, small, sh)) {
+      if (!fp.action) { if (tid == 0) N.flag = -5; return; }
+      for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
+      if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
+      __syncthreads();
+      ++nelim;
+      continue;
+    }
+    int p;
+    int done = 0;
+    for (p = nelim + 1; p < n; ++p) {
+      if (tpp_col_small(N, p, nelim, m, small, sh)) {
+        if (!fp.action) { if (tid == 0) N.flag = -5; return; }
+        tpp_swap(N, p, nelim, m);
+        for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
+        if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
+        __syncthreads();
+        ++nelim;
+        done = 1;
+        break;
+      }
+      // t = argmax |a(p, nelim..p-1)| (first index on ties)
+      if (tid == 0) {
+        int best = nelim;
+        double bv = fabs(Lat(N, p, nelim));
+        for (int c = nelim + 1; c < p; ++c) {
+          double v = fabs(Lat(N, p, c));
+          if (v > bv) { bv = v; best = c; }
+        }
+        s_t = best;
+      }
+      __syncthreads();
+      const int t = s_t;
+      double maxt = tpp_rc_max(N, t, nelim, m, p, sh);
+      double maxp = tpp_rc_max(N, p, nelim, m, t, sh);
+      // 2x2 test (t < p)
+      double a11 = Lat(N, t, t), a21 = Lat(N, p, t), a22 = Lat(N, p, p);
+      double maxpiv = fmax(fabs(a11), fmax(fabs(a21), fabs(a22)));
+      bool ok2 = false;
+      double d0 = 0, d1 = 0, d3 = 0;
+      if (!(maxpiv < small)) {
+        double detscale = 1 / maxpiv;
+        double det0 = (a11 * detscale) * a22;
+        double det1 = (a21 * detscale) * a21;
+        double det = det0 - det1;
+        if (!(fabs(det) < fmax(small, fmax(fabs(det0 / 2), fabs(det1 / 2))))) {
+          d0 = (a22 * detscale) / det;
+          d1 = (-a21 * detscale) / det;
+          d3 = (a11 * detscale) / det;
+          if (fmax(maxp, maxt) < small) ok2 = true;
+          else {
+            double x1 = fabs(d0) * maxt + fabs(d1) * maxp;
+            double x2 = fabs(d1) * maxt + fabs(d3) * maxp;
+            ok2 = u * fmax(x1, x2) < 1.0;
+          }
+        }
+      }
+      if (ok2) {
+        tpp_swap(N, t, nelim, m);
+        tpp_swap(N, p, nelim + 1, m);
+        if (tid == 0) {
+          N.D[2 * nelim] = d0;
+          N.D[2 * nelim + 1] = d1;
+          N.D[2 * nelim + 2] = INFINITY;
+          N.D[2 * nelim + 3] = d3;
+        }
+        __syncthreads();
+        tpp_elim2(N, nelim, m, n, ld, ld2);
+        nelim += 2;
+        done = 1;
+        break;
+      }
+      maxp = fmax(maxp, fabs(Lat(N, p, t)));
+      if (fabs(Lat(N, p, p)) >= u * maxp) {
+        tpp_swap(N, p, nelim, m);
+        if (tid == 0) {
+          N.D[2 * nelim] = 1.0 / Lat(N, nelim, nelim);
+          N.D[2 * nelim + 1] = 0.0;
+        }
+        __syncthreads();
+        tpp_elim1(N, nelim, m, n, ld);
+        nelim += 1;
+        done = 1;
+        break;
+      }
+    }
+    if (!done) {
+      p = nelim;
+      double maxp = tpp_rc_max(N, p, nelim, m, -1, sh);
+      if (fabs(Lat(N, p, p)) >= u * maxp) {
+        if (tid == 0) {
+          N.D[2 * nelim] = 1.0 / Lat(N, nelim, nelim);
+          N.D[2 * nelim + 1] = 0.0;
+        }
+        __syncthreads();
+        tpp_elim1(N, nelim, m, n, ld);
+        nelim += 1;
+      } else {
+        break;
+      }
+    }
+  }
+  if (tid == 0) N.nelim = nelim;
+}
+
+// inertia and delay counts over all fronts (integer atomics: deterministic)
+__global__ void k_stats(const NodeDev *nodes, int nslots, int *counters) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const NodeDev &N = nodes[s];
+  int neg = 0, two = 0, zero = 0;
+  const double *d = N.D;
+  for (int i = 0; i < N.nelim;) {
+    double a11 = d[2 * i], a21 = d[2 * i + 1];
+    if (i + 1 == N.nelim || isfinite(d[2 * i + 2])) {
+      if (a11 == 0.0) zero++;
+      if (a11 < 0.0) neg++;
+      i += 1;
+    } else {
+      double a22 = d[2 * i + 3];
+      two++;
+      double det = a11 * a22 - a21 * a21, tr = a11 + a22;
+      if (det < 0) neg++;
+      else if (tr < 0) neg += 2;
+      i += 2;
+    }
+  }
+  atomicAdd(&counters[0], neg);
+  atomicAdd(&counters[1], two);
+  atomicAdd(&counters[2], zero);
+  atomicAdd(&counters[3], N.n - N.nelim);
+  atomicAdd(&counters[4], N.n - N.nelim_app);  // not_first_pass
+  atomicAdd(&counters[5], N.flag < 0 ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+void launch_assemble(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                     const int *wg_off, int nlvl, const int *child_list,
+                     const double *aval, const int64_t *asrc, const int *arow,
+                     const double *scaling) {
+  if (nwg > 0)
+    hipLaunchKernelGGL(k_assemble, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
+                       nlvl, child_list, aval, asrc, arow, scaling);
+}
+void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
+                 FactorParams fp) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_diag, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, step, fp);
+}
+void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                  const int *wg_off, int nlvl, int step, FactorParams fp) {
+  if (nwg > 0)
+    hipLaunchKernelGGL(k_apply, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
+                       nlvl, step, fp);
+}
+void launch_update(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                   const int *wg_off, int nlvl, int step, int mode, FactorParams fp) {
+  if (nwg > 0)
+    hipLaunchKernelGGL(k_update, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
+                       nlvl, step, mode, fp);
+}
+void launch_swap(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                 const int *wg_off, int nlvl, int step, int phase) {
+  if (nwg > 0)
+    hipLaunchKernelGGL(k_swap, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
+                       nlvl, step, phase);
+}
+void launch_finalize(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_finalize, dim3((nlvl + 255) / 256), dim3(256), 0, st, nodes, slots,
+                       nlvl);
+}
+void launch_tpp(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
+                FactorParams fp) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_tpp, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, fp);
+}
+void launch_stats(hipStream_t st, const NodeDev *nodes, int nslots, int *counters) {
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_stats, dim3((nslots + 255) / 256), dim3(256), 0, st, nodes, nslots,
+                       counters);
+}
+
+}  // namespace ssids_hip

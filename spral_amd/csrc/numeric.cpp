@@ -1,0 +1,984 @@
+// Host driver of the HIP numeric path: symbolic upload, the tree-level walk
+// over an HBM pool, the solves, enquiry and alteration.  Replaces the GPU
+// numeric subtree host code of the reference (src/ssids/gpu/subtree.f90,
+// factor.f90, solve.f90) with a level schedule whose only host round trip is
+// one small read-back per level (delayed-pivot counts size the parents).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/spral_ssids_hip.h"
+#include "dev.hpp"
+
+namespace ssids_hip {
+
+// kernel launchers (factor_kernels.hip / solve_kernels.hip)
+void launch_assemble(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
+                     const double *, const int64_t *, const int *, const double *);
+void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
+void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, int,
+                  FactorParams);
+void launch_update(hipStream_t, int, NodeDev *, const int *, const int *, int, int, int,
+                   FactorParams);
+void launch_swap(hipStream_t, int, NodeDev *, const int *, const int *, int, int, int);
+void launch_finalize(hipStream_t, int, NodeDev *, const int *);
+void launch_tpp(hipStream_t, int, NodeDev *, const int *, FactorParams);
+void launch_stats(hipStream_t, const NodeDev *, int, int *);
+void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
+void launch_fwd(hipStream_t, int, const SolveNode *, const int *, const int *, double *, int,
+                int);
+void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
+void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int);
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  explicit HipError(hipError_t c) : std::runtime_error(hipGetErrorString(c)), code(c) {}
+};
+
+#define HIPCHK(x)                        \
+  do {                                   \
+    hipError_t e_ = (x);                 \
+    if (e_ != hipSuccess) throw HipError(e_); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory pool: large chunks, first-fit free list with coalescing.
+// ---------------------------------------------------------------------------
+class Pool {
+ public:
+  ~Pool() {
+    for (auto &c : chunks_) (void)hipFree(c.base);
+  }
+  void *alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    for (auto &c : chunks_) {
+      for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+        if (it->second >= bytes) {
+          size_t off = it->first, sz = it->second;
+          c.free.erase(it);
+          if (sz > bytes) c.free[off + bytes] = sz - bytes;
+          used_ += bytes;
+          peak_ = std::max(peak_, used_);
+          return (char *)c.base + off;
+        }
+      }
+    }
+    size_t csz = std::max(bytes, chunk_min_);
+    Chunk c;
+    HIPCHK(hipMalloc(&c.base, csz));
+    c.size = csz;
+    if (csz > bytes) c.free[bytes] = csz - bytes;
+    chunks_.push_back(std::move(c));
+    used_ += bytes;
+    peak_ = std::max(peak_, used_);
+    return chunks_.back().base;
+  }
+  void release(void *p, size_t bytes) {
+    if (!p) return;
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    for (auto &c : chunks_) {
+      char *b = (char *)c.base;
+      if ((char *)p >= b && (char *)p < b + c.size) {
+        size_t off = (char *)p - b;
+        auto it = c.free.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != c.free.end() && it->first + it->second == nx->first) {
+          it->second += nx->second;
+          c.free.erase(nx);
+        }
+        if (it != c.free.begin()) {
+          auto pv = std::prev(it);
+          if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            c.free.erase(it);
+          }
+        }
+        used_ -= bytes;
+        return;
+      }
+    }
+  }
+  void set_chunk_min(size_t b) { chunk_min_ = b; }
+  size_t peak() const { return peak_; }
+
+ private:
+  struct Chunk {
+    void *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free;
+  };
+  std::vector<Chunk> chunks_;
+  size_t chunk_min_ = size_t(1) << 30;
+  size_t used_ = 0, peak_ = 0;
+};
+
+template <class T>
+T *dev_upload(const std::vector<T> &v, std::vector<void *> &owned) {
+  if (v.empty()) return nullptr;
+  void *p;
+  HIPCHK(hipMalloc(&p, v.size() * sizeof(T)));
+  HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  owned.push_back(p);
+  return (T *)p;
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic subtree
+// ---------------------------------------------------------------------------
+struct Symbolic {
+  int device = 0, n = 0, nnodes = 0, sa = 1;
+  std::vector<int> sn, sm, parent;        // per node (local 0-based)
+  std::vector<std::vector<int>> children;
+  std::vector<int> slot;                  // node -> slot
+  std::vector<int> node_of;               // slot -> node
+  std::vector<int> lvl_ptr;               // slots of level L: [lvl_ptr[L], lvl_ptr[L+1])
+  std::vector<int> child_list;            // children slots, per node contiguous
+  std::vector<int> cbeg, cend;            // per node (node-indexed)
+  std::vector<int64_t> row_off, rmap_off, acol_off;
+  std::vector<int> rows_h;                // 0-based elimination indices
+  int64_t naval_min = 0;                  // max A index referenced + 1
+  int64_t num_flops = 0;
+  // device
+  int *d_rows = nullptr, *d_rmap = nullptr, *d_arow = nullptr, *d_child = nullptr;
+  int64_t *d_asrc = nullptr, *d_acol = nullptr;
+  std::vector<void *> owned;
+  ~Symbolic() {
+    for (void *p : owned) (void)hipFree(p);
+  }
+};
+
+struct Numeric {
+  Symbolic *symb = nullptr;
+  hipStream_t stream = nullptr;
+  Pool pool;
+  std::vector<NodeDev> nodes;             // slot-indexed host mirror
+  NodeDev *d_nodes = nullptr;
+  int *d_lvl = nullptr;                   // per-level scratch (slots + offsets)
+  size_t d_lvl_cap = 0;
+  int *d_counters = nullptr;
+  int *d_scratch_n = nullptr;             // n ints (ipos)
+  std::vector<std::pair<void *, size_t>> persistent;
+  // solve
+  SolveNode *d_sn = nullptr;
+  int *d_slots_all = nullptr;             // 0..nnodes-1 (slots in level order)
+  std::vector<SolveNode> sn_h;
+  double *d_X = nullptr;
+  size_t d_X_cap = 0;
+  double *d_x = nullptr;
+  size_t d_x_cap = 0;
+  int solve_nrhs_cap = 0;
+  bool posdef = false;
+  spral_ssids_hip_stats stats{};
+  // contribution export
+  std::vector<double> contrib_val, contrib_delay_val;
+  std::vector<int> contrib_rlist, contrib_delay_perm;
+  ~Numeric() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &p : persistent) pool.release(p.first, p.second);
+    if (d_nodes) (void)hipFree(d_nodes);
+    if (d_lvl) (void)hipFree(d_lvl);
+    if (d_counters) (void)hipFree(d_counters);
+    if (d_scratch_n) (void)hipFree(d_scratch_n);
+    if (d_sn) (void)hipFree(d_sn);
+    if (d_slots_all) (void)hipFree(d_slots_all);
+    if (d_X) (void)hipFree(d_X);
+    if (d_x) (void)hipFree(d_x);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sptr,
+                                const int *sparent, const int64_t *rptr, const int *rlist,
+                                const int64_t *nptr, const int64_t *nlist) {
+  auto S = std::make_unique<Symbolic>();
+  S->device = device;
+  S->n = n;
+  S->sa = sa;
+  const int nn = en - sa;
+  S->nnodes = nn;
+  S->sn.resize(nn);
+  S->sm.resize(nn);
+  S->parent.assign(nn, -1);
+  S->children.assign(nn, {});
+  S->row_off.resize(nn + 1);
+  S->rmap_off.resize(nn + 1);
+  S->acol_off.resize(nn + 1);
+  for (int i = 0; i < nn; ++i) {
+    int s = sa - 1 + i;
+    S->sn[i] = sptr[s + 1] - sptr[s];
+    S->sm[i] = (int)(rptr[s + 1] - rptr[s]);
+    int par = sparent[s] - 1 - (sa - 1);
+    if (par >= 0 && par < nn) S->parent[i] = par;
+  }
+  for (int i = 0; i < nn; ++i)
+    if (S->parent[i] >= 0) S->children[S->parent[i]].push_back(i);
+  // rows (0-based) and child->parent row maps
+  int64_t tot = 0;
+  for (int i = 0; i < nn; ++i) {
+    S->row_off[i] = tot;
+    tot += S->sm[i];
+  }
+  S->row_off[nn] = tot;
+  S->rows_h.resize(tot);
+  for (int i = 0; i < nn; ++i) {
+    int s = sa - 1 + i;
+    for (int k = 0; k < S->sm[i]; ++k) S->rows_h[S->row_off[i] + k] = rlist[rptr[s] - 1 + k] - 1;
+  }
+  std::vector<int> rmap;
+  std::vector<int> pos(n, -1);
+  int64_t rtot = 0;
+  for (int i = 0; i < nn; ++i) {
+    S->rmap_off[i] = rtot;
+    rtot += S->sm[i] - S->sn[i];
+  }
+  S->rmap_off[nn] = rtot;
+  rmap.assign(rtot, 0);
+  for (int p = 0; p < nn; ++p) {
+    for (int k = 0; k < S->sm[p]; ++k) pos[S->rows_h[S->row_off[p] + k]] = k;
+    for (int c : S->children[p])
+      for (int k = S->sn[c]; k < S->sm[c]; ++k)
+        rmap[S->rmap_off[c] + k - S->sn[c]] = pos[S->rows_h[S->row_off[c] + k]];
+  }
+  // A entries sorted by column per node
+  std::vector<int64_t> asrc;
+  std::vector<int> arow;
+  std::vector<int64_t> acol;
+  int64_t amax = 0;
+  for (int i = 0; i < nn; ++i) {
+    int s = sa - 1 + i;
+    int sm = S->sm[i], snn = S->sn[i];
+    int64_t e0 = nptr[s] - 1, e1 = nptr[s + 1] - 1;
+    std::vector<std::vector<std::pair<int64_t, int>>> bycol(snn);
+    for (int64_t e = e0; e < e1; ++e) {
+      int64_t src = nlist[2 * e] - 1, dst = nlist[2 * e + 1] - 1;
+      int c = (int)(dst / sm), r = (int)(dst % sm);
+      bycol[c].push_back({src, r});
+      amax = std::max(amax, src + 1);
+    }
+    S->acol_off[i] = (int64_t)acol.size();
+    for (int c = 0; c < snn; ++c) {
+      acol.push_back((int64_t)asrc.size());
+      for (auto &pr : bycol[c]) {
+        asrc.push_back(pr.first);
+        arow.push_back(pr.second);
+      }
+    }
+    acol.push_back((int64_t)asrc.size());
+  }
+  S->acol_off[nn] = (int64_t)acol.size();
+  S->naval_min = amax;
+  // levels by height (leaves = 0)
+  std::vector<int> lev(nn, 0);
+  int nlev = 0;
+  for (int i = 0; i < nn; ++i) {  // children precede parents (postorder)
+    for (int c : S->children[i]) lev[i] = std::max(lev[i], lev[c] + 1);
+    nlev = std::max(nlev, lev[i] + 1);
+  }
+  S->lvl_ptr.assign(nlev + 1, 0);
+  for (int i = 0; i < nn; ++i) S->lvl_ptr[lev[i] + 1]++;
+  for (int L = 0; L < nlev; ++L) S->lvl_ptr[L + 1] += S->lvl_ptr[L];
+  S->slot.assign(nn, 0);
+  S->node_of.assign(nn, 0);
+  {
+    std::vector<int> fill(S->lvl_ptr.begin(), S->lvl_ptr.end() - 1);
+    for (int i = 0; i < nn; ++i) {
+      int sl = fill[lev[i]]++;
+      S->slot[i] = sl;
+      S->node_of[sl] = i;
+    }
+  }
+  S->cbeg.assign(nn, 0);
+  S->cend.assign(nn, 0);
+  for (int i = 0; i < nn; ++i) {
+    S->cbeg[i] = (int)S->child_list.size();
+    for (int c : S->children[i]) S->child_list.push_back(S->slot[c]);
+    S->cend[i] = (int)S->child_list.size();
+  }
+  int64_t fl = 0;
+  for (int i = 0; i < nn; ++i) {
+    int64_t ne = S->sn[i], mm = S->sm[i] - ne;
+    for (int64_t j = 1; j <= ne; ++j) fl += (mm + j) * (mm + j);
+  }
+  S->num_flops = fl;
+  HIPCHK(hipSetDevice(device));
+  S->d_rows = dev_upload(S->rows_h, S->owned);
+  S->d_rmap = dev_upload(rmap, S->owned);
+  S->d_asrc = dev_upload(asrc, S->owned);
+  S->d_arow = dev_upload(arow, S->owned);
+  S->d_acol = dev_upload(acol, S->owned);
+  S->d_child = dev_upload(S->child_list, S->owned);
+  return S.release();
+}
+
+// ---------------------------------------------------------------------------
+// Numeric factorization
+// ---------------------------------------------------------------------------
+static void ensure_lvl(Numeric &N, size_t ints) {
+  if (ints <= N.d_lvl_cap) return;
+  if (N.d_lvl) HIPCHK(hipFree(N.d_lvl));
+  size_t cap = std::max(ints, size_t(1) << 16);
+  HIPCHK(hipMalloc(&N.d_lvl, cap * sizeof(int)));
+  N.d_lvl_cap = cap;
+}
+
+static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
+                        const spral_ssids_hip_options &opt) {
+  Symbolic &S = *N.symb;
+  const int nn = S.nnodes;
+  hipStream_t st = N.stream;
+  FactorParams fp;
+  fp.u = opt.u;
+  fp.small = opt.small;
+  fp.action = opt.action;
+  fp.tpp_failed = (opt.failed_pivot_method != 2);
+  fp.pivot_tpp = (opt.pivot_method == 3);
+  // release factors of a previous factorization on this handle
+  for (auto &p : N.persistent) N.pool.release(p.first, p.second);
+  N.persistent.clear();
+  N.nodes.assign(nn, NodeDev{});
+  if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
+  if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
+  if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
+  std::vector<int> nelim_h(nn, 0);  // slot indexed
+  const int nlev = (int)S.lvl_ptr.size() - 1;
+  std::vector<std::pair<void *, size_t>> contrib_of(nn);  // slot -> C buffer
+  std::vector<std::pair<void *, size_t>> work_of(nn);
+  int maxfront = 0;
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, st));
+  auto palloc = [&](size_t bytes) {
+    void *p = N.pool.alloc(bytes);
+    N.persistent.push_back({p, bytes});
+    return p;
+  };
+  for (int L = 0; L < nlev; ++L) {
+    const int s0 = S.lvl_ptr[L], s1 = S.lvl_ptr[L + 1], nl = s1 - s0;
+    int maxsteps = 0;
+    bool multi = false;
+    for (int sl = s0; sl < s1; ++sl) {
+      int i = S.node_of[sl];
+      NodeDev &D = N.nodes[sl];
+      D = NodeDev{};
+      int nd = 0;
+      for (int c : S.children[i]) {
+        int cs = S.slot[c];
+        nd += N.nodes[cs].n - nelim_h[cs];
+      }
+      D.sm = S.sm[i];
+      D.sn = S.sn[i];
+      D.nd = nd;
+      D.m = D.sm + nd;
+      D.n = D.sn + nd;
+      D.ldl = D.m;
+      D.ldc = D.m - D.n;
+      D.ldbak = std::max(D.m, 1);
+      D.parent = S.parent[i] >= 0 ? S.slot[S.parent[i]] : -1;
+      D.cbeg = S.cbeg[i];
+      D.cend = S.cend[i];
+      D.is_root = (S.parent[i] < 0) ? 1 : 0;
+      D.nsteps = fp.pivot_tpp ? 0 : cdiv(D.n, kBlk);
+      D.rows = S.d_rows + S.row_off[i];
+      D.rmap = S.d_rmap + S.rmap_off[i];
+      D.acol = S.d_acol + S.acol_off[i];
+      maxfront = std::max(maxfront, D.m);
+      maxsteps = std::max(maxsteps, D.nsteps);
+      if (D.nsteps > 1) multi = true;
+      size_t lb = (size_t)D.ldl * D.n * sizeof(double);
+      D.L = (double *)palloc(lb);
+      D.D = (double *)palloc(2 * (size_t)D.n * sizeof(double));
+      D.perm = (int *)palloc(std::max(D.n, 1) * sizeof(int));
+      D.perm0 = (int *)palloc(std::max(D.n, 1) * sizeof(int));
+      D.ipos = (int *)palloc(std::max(D.n, 1) * sizeof(int));
+      size_t cb = (size_t)D.ldc * (D.m - D.n) * sizeof(double);
+      D.C = cb ? (double *)N.pool.alloc(cb) : nullptr;
+      contrib_of[sl] = {D.C, cb};
+      size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
+      if (D.nsteps > 1) wb += (size_t)2 * kBlk * D.m * sizeof(double);
+      char *w = (char *)N.pool.alloc(wb);
+      work_of[sl] = {w, wb};
+      D.bak = (double *)w;
+      D.dbak = D.bak + (size_t)D.ldbak * kBlk;
+      D.swp = (D.nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
+    }
+    HIPCHK(hipMemcpyAsync(N.d_nodes + s0, N.nodes.data() + s0, nl * sizeof(NodeDev),
+                          hipMemcpyHostToDevice, st));
+    // per-level launch tables: slots | asm | apply | swap | update(step..) | update tpp
+    std::vector<int> tab;
+    tab.reserve(nl * (5 + maxsteps) + 8);
+    for (int sl = s0; sl < s1; ++sl) tab.push_back(sl);
+    auto add_off = [&](auto count_fn) {
+      int base = (int)tab.size();
+      int acc = 0;
+      for (int li = 0; li < nl; ++li) {
+        tab.push_back(acc);
+        acc += count_fn(N.nodes[s0 + li]);
+      }
+      tab.push_back(acc);
+      return std::make_pair(base, acc);
+    };
+    auto asm_t = add_off([](const NodeDev &D) { return cdiv(D.m, 32); });
+    auto app_t = add_off([](const NodeDev &D) { return std::max(1, cdiv(D.m, kThreads)); });
+    auto swp_t = add_off([](const NodeDev &D) {
+      return D.nsteps > 1 ? std::max(1, cdiv((int64_t)2 * kBlk * D.m, kThreads * 8)) : 0;
+    });
+    std::vector<std::pair<int, int>> upd_t(maxsteps);
+    for (int stp = 0; stp < maxsteps; ++stp) {
+      upd_t[stp] = add_off([stp](const NodeDev &D) {
+        if (stp >= D.nsteps) return 0;
+        int nt = D.m - std::min(D.n, (stp + 1) * kBlk);
+        nt = std::max(nt, kBlk);
+        int T = cdiv(nt, kBlk);
+        return std::max(1, std::min(T * (T + 1) / 2, 4096));
+      });
+    }
+    auto tpp_t = add_off([](const NodeDev &D) {
+      int nt = D.m - D.n;
+      if (nt <= 0) return 1;
+      int T = cdiv(nt, kBlk);
+      return std::max(1, std::min(T * (T + 1) / 2, 4096));
+    });
+    ensure_lvl(N, tab.size());
+    HIPCHK(hipMemcpyAsync(N.d_lvl, tab.data(), tab.size() * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    const int *d_slots = N.d_lvl;
+    auto off = [&](std::pair<int, int> t) { return (const int *)(N.d_lvl + t.first); };
+    launch_assemble(st, asm_t.second, N.d_nodes, d_slots, off(asm_t), nl, S.d_child, d_aval,
+                    S.d_asrc, S.d_arow, d_scaling);
+    for (int stp = 0; stp < maxsteps; ++stp) {
+      launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      launch_apply(st, app_t.second, N.d_nodes, d_slots, off(app_t), nl, stp, fp);
+      launch_update(st, upd_t[stp].second, N.d_nodes, d_slots, off(upd_t[stp]), nl, stp, 0,
+                    fp);
+      if (multi) {
+        launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 0);
+        launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 1);
+      }
+    }
+    launch_finalize(st, nl, N.d_nodes, d_slots);
+    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
+    launch_update(st, tpp_t.second, N.d_nodes, d_slots, off(tpp_t), nl, 0, 1, fp);
+    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
+    HIPCHK(hipGetLastError());
+    // read back the level's results (nelim, flag)
+    std::vector<NodeDev> back(nl);
+    HIPCHK(hipMemcpyAsync(back.data(), N.d_nodes + s0, nl * sizeof(NodeDev),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int bad = 0;
+    for (int li = 0; li < nl; ++li) {
+      int sl = s0 + li;
+      nelim_h[sl] = back[li].nelim;
+      N.nodes[sl].nelim = back[li].nelim;
+      N.nodes[sl].nelim_app = back[li].nelim_app;
+      if (back[li].flag < 0) bad = back[li].flag;
+    }
+    // children contributions have been assembled: release them and the
+    // workspaces of this level
+    for (int sl = s0; sl < s1; ++sl) {
+      int i = S.node_of[sl];
+      for (int c : S.children[i]) {
+        int cs = S.slot[c];
+        N.pool.release(contrib_of[cs].first, contrib_of[cs].second);
+        contrib_of[cs] = {nullptr, 0};
+      }
+      N.pool.release(work_of[sl].first, work_of[sl].second);
+      work_of[sl] = {nullptr, 0};
+    }
+    if (bad < 0) {
+      N.stats.flag = bad;
+      break;
+    }
+  }
+  // roots keep their contribution block (exported by get_contrib)
+  for (int sl = 0; sl < nn; ++sl)
+    if (contrib_of[sl].first) N.persistent.push_back(contrib_of[sl]);
+  HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
+  launch_stats(st, N.d_nodes, nn, N.d_counters);
+  int cnt[16];
+  HIPCHK(hipMemcpyAsync(cnt, N.d_counters, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ev1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+  HIPCHK(hipEventDestroy(ev0));
+  HIPCHK(hipEventDestroy(ev1));
+  spral_ssids_hip_stats &o = N.stats;
+  if (o.flag >= 0) {
+    o.num_neg = cnt[0];
+    o.num_two = cnt[1];
+    o.num_zero = cnt[2];
+    o.num_delay = cnt[3];
+    o.not_first_pass = cnt[4];
+    o.not_second_pass = cnt[3];
+    o.flag = (cnt[2] > 0) ? 7 : 0;  // SSIDS_WARNING_FACT_SINGULAR
+    if (cnt[5] > 0) o.flag = -5;
+  }
+  int64_t nf = 0, fl = 0;
+  int maxsn = 0;
+  for (int sl = 0; sl < nn; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    for (int64_t j = D.m; j >= D.m - D.nelim + 1; --j) {
+      nf += j;
+      fl += j * j;
+    }
+    maxsn = std::max(maxsn, S.sn[S.node_of[sl]]);
+  }
+  o.num_factor = nf;
+  o.num_flops = fl;
+  o.maxfront = maxfront;
+  o.maxsupernode = maxsn;
+  o.num_levels = nlev;
+  o.factor_ms = ms;
+  // solve descriptors
+  N.sn_h.assign(nn, SolveNode{});
+  std::vector<int> doff(nn, 0);
+  for (int i = 0; i < nn; ++i) {
+    int acc = 0;
+    for (int c : S.children[i]) {
+      int cs = S.slot[c];
+      doff[cs] = acc;
+      acc += N.nodes[cs].n - N.nodes[cs].nelim;
+    }
+  }
+  for (int sl = 0; sl < nn; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    SolveNode &Q = N.sn_h[sl];
+    Q.L = D.L;
+    Q.D = D.D;
+    Q.perm = D.perm;
+    Q.ipos = D.ipos;
+    Q.rows = D.rows;
+    Q.rmap = D.rmap;
+    Q.m = D.m;
+    Q.n = D.n;
+    Q.nd = D.nd;
+    Q.sn = D.sn;
+    Q.nelim = D.nelim;
+    Q.ldl = D.ldl;
+    Q.doff = doff[sl];
+    Q.cbeg = D.cbeg;
+    Q.cend = D.cend;
+  }
+  if (!N.d_sn) HIPCHK(hipMalloc(&N.d_sn, std::max(nn, 1) * sizeof(SolveNode)));
+  if (!N.d_slots_all) {
+    std::vector<int> iota(std::max(nn, 1));
+    for (int i = 0; i < nn; ++i) iota[i] = i;
+    HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
+  N.solve_nrhs_cap = 0;  // X pointers need refreshing
+}
+
+static void ensure_solve_ws(Numeric &N, int nrhs) {
+  Symbolic &S = *N.symb;
+  if (nrhs <= N.solve_nrhs_cap) return;
+  size_t tot = 0;
+  for (int sl = 0; sl < S.nnodes; ++sl) tot += (size_t)N.nodes[sl].m * nrhs;
+  if (tot > N.d_X_cap) {
+    if (N.d_X) HIPCHK(hipFree(N.d_X));
+    HIPCHK(hipMalloc(&N.d_X, std::max(tot, size_t(1)) * sizeof(double)));
+    N.d_X_cap = tot;
+  }
+  size_t offx = 0;
+  for (int sl = 0; sl < S.nnodes; ++sl) {
+    N.sn_h[sl].X = N.d_X + offx;
+    offx += (size_t)N.nodes[sl].m * nrhs;
+  }
+  HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), S.nnodes * sizeof(SolveNode),
+                   hipMemcpyHostToDevice));
+  N.solve_nrhs_cap = nrhs;
+}
+
+static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
+                         hipStream_t st) {
+  Symbolic &S = *N.symb;
+  ensure_solve_ws(N, nrhs);
+  const int nlev = (int)S.lvl_ptr.size() - 1;
+  // the slot list of all levels is simply 0..nnodes-1 in level order
+  const int *slots = N.d_slots_all;
+  bool fwd = (job == 0 || job == 1), bwd = (job == 0 || job == 3 || job == 4);
+  bool diag = (job == 0 || job == 2 || job == 4);
+  if (fwd)
+    for (int L = 0; L < nlev; ++L)
+      launch_fwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L],
+                 S.d_child, d_x, ldx, nrhs);
+  if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs);
+  if (bwd)
+    for (int L = nlev - 1; L >= 0; --L)
+      launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
+                 ldx, nrhs);
+  HIPCHK(hipGetLastError());
+}
+
+static int solve_host(Numeric &N, int job, int nrhs, double *x, int ldx) {
+  Symbolic &S = *N.symb;
+  if (nrhs < 1 || ldx < S.n) return -10;
+  HIPCHK(hipSetDevice(S.device));
+  size_t need = (size_t)S.n * nrhs;
+  if (need > N.d_x_cap) {
+    if (N.d_x) HIPCHK(hipFree(N.d_x));
+    HIPCHK(hipMalloc(&N.d_x, need * sizeof(double)));
+    N.d_x_cap = need;
+  }
+  HIPCHK(hipMemcpy2DAsync(N.d_x, S.n * sizeof(double), x, ldx * sizeof(double),
+                          S.n * sizeof(double), nrhs, hipMemcpyHostToDevice, N.stream));
+  solve_device(N, job, nrhs, N.d_x, S.n, N.stream);
+  HIPCHK(hipMemcpy2DAsync(x, ldx * sizeof(double), N.d_x, S.n * sizeof(double),
+                          S.n * sizeof(double), nrhs, hipMemcpyDeviceToHost, N.stream));
+  HIPCHK(hipStreamSynchronize(N.stream));
+  return 0;
+}
+
+}  // namespace ssids_hip
+
+using namespace ssids_hip;
+
+static void set_error(spral_ssids_hip_stats *st, const HipError &e) {
+  if (st) {
+    st->flag = -51;  // SSIDS_ERROR_CUDA_UNKNOWN slot
+    st->hip_error = (int)e.code;
+  }
+}
+
+extern "C" {
+
+int spral_ssids_hip_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+void *spral_ssids_hip_create_symbolic(int device, int n, int sa, int en, const int *sptr,
+                                      const int *sparent, const int64_t *rptr,
+                                      const int *rlist, const int64_t *nptr,
+                                      const int64_t *nlist,
+                                      const struct spral_ssids_hip_options *options,
+                                      int *flag) {
+  (void)options;
+  *flag = 0;
+  try {
+    return build_symbolic(device, n, sa, en, sptr, sparent, rptr, rlist, nptr, nlist);
+  } catch (HipError &e) {
+    *flag = -51;
+  } catch (std::bad_alloc &) {
+    *flag = -50;
+  }
+  return nullptr;
+}
+
+void spral_ssids_hip_destroy_symbolic(void *symbolic) {
+  delete static_cast<Symbolic *>(symbolic);
+}
+
+void *spral_ssids_hip_factor_device(bool posdef, void *symbolic, const double *d_aval,
+                                    int64_t naval, const double *d_scaling,
+                                    const struct spral_ssids_hip_options *options,
+                                    struct spral_ssids_hip_stats *stats) {
+  Symbolic *S = static_cast<Symbolic *>(symbolic);
+  spral_ssids_hip_options opt;
+  if (options) opt = *options;
+  else spral_ssids_hip_default_options(&opt);
+  std::memset(stats, 0, sizeof(*stats));
+  if (posdef) {
+    stats->flag = -98;  // LL^T path not provided by this backend
+    return nullptr;
+  }
+  if (naval < S->naval_min) {
+    stats->flag = -3;
+    return nullptr;
+  }
+  auto *N = new Numeric();
+  N->symb = S;
+  try {
+    HIPCHK(hipSetDevice(S->device));
+    HIPCHK(hipStreamCreate(&N->stream));
+    factor_impl(*N, d_aval, d_scaling, opt);
+    *stats = N->stats;
+    if (stats->flag < 0) {
+      delete N;
+      return nullptr;
+    }
+    return N;
+  } catch (HipError &e) {
+    set_error(stats, e);
+  } catch (std::bad_alloc &) {
+    stats->flag = -50;
+  }
+  delete N;
+  return nullptr;
+}
+
+void *spral_ssids_hip_factor(bool posdef, void *symbolic, const double *aval, int64_t naval,
+                             const double *scaling,
+                             const struct spral_ssids_hip_options *options,
+                             struct spral_ssids_hip_stats *stats) {
+  Symbolic *S = static_cast<Symbolic *>(symbolic);
+  double *d_a = nullptr, *d_s = nullptr;
+  void *res = nullptr;
+  try {
+    HIPCHK(hipSetDevice(S->device));
+    HIPCHK(hipMalloc(&d_a, std::max<int64_t>(naval, 1) * sizeof(double)));
+    HIPCHK(hipMemcpy(d_a, aval, naval * sizeof(double), hipMemcpyHostToDevice));
+    if (scaling) {
+      HIPCHK(hipMalloc(&d_s, std::max(S->n, 1) * sizeof(double)));
+      HIPCHK(hipMemcpy(d_s, scaling, S->n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    res = spral_ssids_hip_factor_device(posdef, symbolic, d_a, naval, d_s, options, stats);
+  } catch (HipError &e) {
+    std::memset(stats, 0, sizeof(*stats));
+    set_error(stats, e);
+  }
+  if (d_a) (void)hipFree(d_a);
+  if (d_s) (void)hipFree(d_s);
+  return res;
+}
+
+int spral_ssids_hip_refactor_device(void *numeric, const double *d_aval,
+                                    const double *d_scaling,
+                                    const struct spral_ssids_hip_options *options,
+                                    struct spral_ssids_hip_stats *stats) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  spral_ssids_hip_options opt;
+  if (options) opt = *options;
+  else spral_ssids_hip_default_options(&opt);
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    N->stats = spral_ssids_hip_stats{};
+    factor_impl(*N, d_aval, d_scaling, opt);
+    *stats = N->stats;
+    return stats->flag;
+  } catch (HipError &e) {
+    std::memset(stats, 0, sizeof(*stats));
+    set_error(stats, e);
+    return -51;
+  } catch (std::bad_alloc &) {
+    stats->flag = -50;
+    return -50;
+  }
+}
+
+void spral_ssids_hip_destroy_numeric(void *numeric) { delete static_cast<Numeric *>(numeric); }
+
+static int solve_entry(void *numeric, int job, int nrhs, double *x, int ldx) {
+  try {
+    return solve_host(*static_cast<Numeric *>(numeric), job, nrhs, x, ldx);
+  } catch (HipError &) {
+    return -51;
+  } catch (std::bad_alloc &) {
+    return -50;
+  }
+}
+
+int spral_ssids_hip_solve_fwd(void *numeric, int nrhs, double *x, int ldx) {
+  return solve_entry(numeric, 1, nrhs, x, ldx);
+}
+int spral_ssids_hip_solve_diag(void *numeric, int nrhs, double *x, int ldx) {
+  return solve_entry(numeric, 2, nrhs, x, ldx);
+}
+int spral_ssids_hip_solve_diag_bwd(void *numeric, int nrhs, double *x, int ldx) {
+  return solve_entry(numeric, 4, nrhs, x, ldx);
+}
+int spral_ssids_hip_solve_bwd(void *numeric, int nrhs, double *x, int ldx) {
+  return solve_entry(numeric, 3, nrhs, x, ldx);
+}
+
+int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x, int ldx,
+                                 void *stream) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  if (job < 0 || job > 4) return -11;
+  if (nrhs < 1 || ldx < N->symb->n) return -10;
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    hipStream_t st = stream ? (hipStream_t)stream : N->stream;
+    solve_device(*N, job, nrhs, d_x, ldx, st);
+    return 0;
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
+int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d) {
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  try {
+    if (piv_order)
+      for (int i = 0; i < S.n; ++i) piv_order[i] = 0;
+    if (d)
+      for (int i = 0; i < 2 * S.n; ++i) d[i] = 0.0;
+    int piv = 0;
+    double *dp = d;
+    // pivot order follows the original node order (postorder), as the
+    // reference walks nodes 0..nnodes-1
+    for (int i = 0; i < S.nnodes; ++i) {
+      const NodeDev &D = N->nodes[S.slot[i]];
+      int ne = D.nelim;
+      std::vector<double> dh(2 * (size_t)std::max(ne, 1));
+      std::vector<int> ph(std::max(ne, 1));
+      if (ne > 0) {
+        HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ph.data(), D.perm, ne * sizeof(int), hipMemcpyDeviceToHost));
+      }
+      for (int k = 0; k < ne;) {
+        if (k + 1 == ne || std::isfinite(dh[2 * k + 2])) {
+          if (piv_order) piv_order[ph[k]] = piv;
+          ++piv;
+          if (dp) { *dp++ = dh[2 * k]; *dp++ = 0.0; }
+          k += 1;
+        } else {
+          if (piv_order) {
+            piv_order[ph[k]] = -(piv++);
+            piv_order[ph[k + 1]] = -(piv++);
+          } else {
+            piv += 2;
+          }
+          if (dp) {
+            *dp++ = dh[2 * k];
+            *dp++ = dh[2 * k + 1];
+            *dp++ = dh[2 * k + 3];
+            *dp++ = 0.0;
+          }
+          k += 2;
+        }
+      }
+    }
+    return 0;
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
+int spral_ssids_hip_alter(void *numeric, const double *d) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  try {
+    const double *dp = d;
+    for (int i = 0; i < S.nnodes; ++i) {
+      const NodeDev &D = N->nodes[S.slot[i]];
+      int ne = D.nelim;
+      if (ne == 0) continue;
+      std::vector<double> dh(2 * (size_t)ne);
+      HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
+      for (int k = 0; k < ne;) {
+        if (k + 1 == ne || std::isfinite(dh[2 * k + 2])) {
+          dh[2 * k] = dp[0];
+          dh[2 * k + 1] = 0.0;
+          dp += 2;
+          k += 1;
+        } else {
+          dh[2 * k] = dp[0];
+          dh[2 * k + 1] = dp[1];
+          dh[2 * k + 3] = dp[2];
+          dp += 4;
+          k += 2;
+        }
+      }
+      HIPCHK(hipMemcpy(D.D, dh.data(), 2 * ne * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return 0;
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
+void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *ldval,
+                                 const int **rlist, int *ndelay, const int **delay_perm,
+                                 const double **delay_val, int *lddelay) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  *n = 0;
+  *val = nullptr;
+  *ldval = 0;
+  *rlist = nullptr;
+  *ndelay = 0;
+  *delay_perm = nullptr;
+  *delay_val = nullptr;
+  *lddelay = 0;
+  // the subtree root is the last node in postorder
+  if (S.nnodes == 0) return;
+  const NodeDev &D = N->nodes[S.slot[S.nnodes - 1]];
+  int cm = D.m - D.n;
+  N->contrib_val.assign((size_t)cm * cm, 0.0);
+  if (cm > 0) {
+    (void)hipMemcpy(N->contrib_val.data(), D.C, (size_t)cm * cm * sizeof(double),
+                    hipMemcpyDeviceToHost);
+    // symmetrise the lower triangle into the full square the reference passes
+    for (int j = 0; j < cm; ++j)
+      for (int i = 0; i < j; ++i)
+        N->contrib_val[(size_t)j * cm + i] = N->contrib_val[(size_t)i * cm + j];
+  }
+  N->contrib_rlist.resize(cm);
+  const int i_root = S.nnodes - 1;
+  for (int k = 0; k < cm; ++k)
+    N->contrib_rlist[k] = S.rows_h[S.row_off[i_root] + D.sn + k] + 1;
+  int nf = D.n - D.nelim;
+  N->contrib_delay_perm.resize(nf);
+  N->contrib_delay_val.assign((size_t)nf * D.m, 0.0);
+  if (nf > 0) {
+    std::vector<int> ph(D.n);
+    (void)hipMemcpy(ph.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost);
+    for (int k = 0; k < nf; ++k) N->contrib_delay_perm[k] = ph[D.nelim + k] + 1;
+    (void)hipMemcpy2D(N->contrib_delay_val.data(), D.m * sizeof(double),
+                      D.L + (size_t)D.nelim * D.ldl, D.ldl * sizeof(double),
+                      D.m * sizeof(double), nf, hipMemcpyDeviceToHost);
+  }
+  *n = cm;
+  *val = N->contrib_val.data();
+  *ldval = cm;
+  *rlist = N->contrib_rlist.data();
+  *ndelay = nf;
+  *delay_perm = N->contrib_delay_perm.data();
+  *delay_val = nf ? N->contrib_delay_val.data() + D.nelim : nullptr;
+  *lddelay = D.m;
+}
+
+int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, double *L,
+                                 double *D, int *perm) {
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  if (node < 0 || node >= S.nnodes) return -1;
+  const NodeDev &Dn = N->nodes[S.slot[node]];
+  info[0] = Dn.m;
+  info[1] = Dn.n;
+  info[2] = Dn.nd;
+  info[3] = Dn.nelim;
+  info[4] = Dn.ldl;
+  info[5] = Dn.sn;
+  try {
+    if (L && Dn.n > 0)
+      HIPCHK(hipMemcpy(L, Dn.L, (size_t)Dn.ldl * Dn.n * sizeof(double), hipMemcpyDeviceToHost));
+    if (D && Dn.n > 0)
+      HIPCHK(hipMemcpy(D, Dn.D, 2 * (size_t)Dn.n * sizeof(double), hipMemcpyDeviceToHost));
+    if (perm && Dn.n > 0)
+      HIPCHK(hipMemcpy(perm, Dn.perm, Dn.n * sizeof(int), hipMemcpyDeviceToHost));
+  } catch (HipError &) {
+    return -51;
+  }
+  return 0;
+}
+
+void spral_ssids_hip_free_contrib(void *numeric) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  std::vector<double>().swap(N->contrib_val);
+  std::vector<double>().swap(N->contrib_delay_val);
+  std::vector<int>().swap(N->contrib_rlist);
+  std::vector<int>().swap(N->contrib_delay_perm);
+}
+
+}  // extern "C"

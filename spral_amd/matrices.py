@@ -1,0 +1,153 @@
+"""Seeded synthetic matrices of the benchmark workloads (lower-triangle CSC,
+1-based, as SSIDS takes them: src/ssids/ssids.f90:148 analyse_double).
+
+* laplace3d(nx, ny, nz, diag)      : 7-point stencil, off-diagonals -1;
+* kkt_laplace(...)                 : [H A^T; A 0] with H a shifted 7-point
+                                      Laplacian and A random with exactly k
+                                      distinct N(0,1) entries per row;
+* random_indef / random_posdef     : small random test matrices in the spirit of
+                                      gen_random_indef / gen_random_posdef
+                                      (tests/ssids/ssids.f90:2830-2900).
+Seeds are numpy PCG64; every generator is deterministic for a given seed.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+SEED_BASE = 20261202
+
+
+class LowerCSC:
+    """Lower triangle of a symmetric matrix in 1-based CSC."""
+
+    def __init__(self, a: sp.spmatrix):
+        a = sp.csc_matrix(a)
+        low = sp.tril(a, format="csc")
+        low.sort_indices()
+        self.n = a.shape[0]
+        self.ptr = (low.indptr.astype(np.int64) + 1)
+        self.row = (low.indices.astype(np.int32) + 1)
+        self.val = low.data.astype(np.float64)
+        self.full = a
+
+    @property
+    def nz(self):
+        return int(self.ptr[-1] - 1)
+
+    def matvec(self, x):
+        return self.full @ x
+
+
+def _lap7(nx, ny, nz):
+    def d1(k):
+        e = np.ones(k)
+        return sp.diags([-e[:-1], -e[:-1]], [-1, 1], shape=(k, k))
+    ix, iy, iz = sp.identity(nx), sp.identity(ny), sp.identity(nz)
+    off = (sp.kron(iz, sp.kron(iy, d1(nx))) + sp.kron(iz, sp.kron(d1(ny), ix))
+           + sp.kron(d1(nz), sp.kron(iy, ix)))
+    return sp.csc_matrix(off)
+
+
+def laplace3d(nx, ny=None, nz=None, diag=6.0) -> LowerCSC:
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    off = _lap7(nx, ny, nz)
+    n = nx * ny * nz
+    return LowerCSC(off + diag * sp.identity(n, format="csc"))
+
+
+def random_rows(rng, nrows, ncols, k):
+    """A (nrows x ncols) with exactly k distinct uniformly chosen columns per
+    row, N(0,1) values."""
+    cols = np.empty((nrows, k), dtype=np.int64)
+    for i in range(nrows):
+        cols[i] = rng.choice(ncols, size=k, replace=False)
+    vals = rng.standard_normal((nrows, k))
+    rows = np.repeat(np.arange(nrows), k)
+    return sp.csr_matrix((vals.ravel(), (rows, cols.ravel())), shape=(nrows, ncols))
+
+
+def kkt_laplace(grid=16, shift=-0.5, m_con=1024, k=3, seed=SEED_BASE + 1) -> LowerCSC:
+    """Saddle-point K = [H A^T; A 0]: H = 7-pt grid^3 Laplacian (diag 6) +
+    shift*I, A m_con x grid^3 with k N(0,1) entries per row."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nh = grid ** 3
+    h = _lap7(grid, grid, grid) + (6.0 + shift) * sp.identity(nh, format="csc")
+    a = random_rows(rng, m_con, nh, k)
+    kk = sp.bmat([[h, a.T], [a, None]], format="csc")
+    return LowerCSC(kk)
+
+
+def random_indef(n, density=0.05, seed=0, zero_diag_frac=0.2, big_last=True) -> LowerCSC:
+    """Random symmetric indefinite matrix with some zero diagonals and, as in
+    the reference's generator, the last entry of each column scaled up to
+    provoke delayed pivots."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nnz = max(n, int(density * n * n / 2))
+    r = rng.integers(0, n, nnz)
+    c = rng.integers(0, n, nnz)
+    lo, hi = np.minimum(r, c), np.maximum(r, c)
+    v = rng.uniform(-1, 1, nnz)
+    a = sp.coo_matrix((v, (hi, lo)), shape=(n, n)).tocsc()
+    a.sum_duplicates()
+    d = rng.uniform(-1, 1, n)
+    d[rng.random(n) < zero_diag_frac] = 0.0
+    a = sp.tril(a, k=-1, format="csc")
+    if big_last:
+        for j in range(n):
+            s, e = a.indptr[j], a.indptr[j + 1]
+            if e > s:
+                a.data[e - 1] *= 1000.0
+    full = a + a.T + sp.diags(d)
+    full = sp.csc_matrix(full)
+    # keep explicit zero diagonals in the pattern
+    full = full + sp.diags(np.where(d == 0.0, 0.0, 0.0), format="csc")
+    pat = sp.csc_matrix((np.ones(n), (np.arange(n), np.arange(n))), shape=(n, n))
+    full = sp.csc_matrix(full + 0.0 * pat)
+    low = sp.tril(full, format="csc")
+    low.sort_indices()
+    out = LowerCSC.__new__(LowerCSC)
+    out.n = n
+    # ensure every diagonal position exists structurally
+    low = sp.csc_matrix(low)
+    low.eliminate_zeros()
+    low = low + sp.csc_matrix((np.full(n, 1e-300), (np.arange(n), np.arange(n))), shape=(n, n))
+    low.data[np.abs(low.data) == 1e-300] = 0.0
+    low.sort_indices()
+    out.ptr = low.indptr.astype(np.int64) + 1
+    out.row = low.indices.astype(np.int32) + 1
+    out.val = low.data.astype(np.float64)
+    f = low + sp.tril(low, k=-1).T
+    out.full = sp.csc_matrix(f)
+    return out
+
+
+def random_posdef(n, density=0.05, seed=0) -> LowerCSC:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nnz = max(n, int(density * n * n / 2))
+    r = rng.integers(0, n, nnz)
+    c = rng.integers(0, n, nnz)
+    lo, hi = np.minimum(r, c), np.maximum(r, c)
+    v = rng.uniform(-1, 1, nnz)
+    a = sp.coo_matrix((v, (hi, lo)), shape=(n, n)).tocsc()
+    a.sum_duplicates()
+    a = sp.tril(a, k=-1, format="csc")
+    full = a + a.T
+    dd = np.asarray(abs(full).sum(axis=1)).ravel() + 1.0
+    return LowerCSC(full + sp.diags(dd))
+
+
+def bwd_error(full, x, b):
+    """Scaled backward error max|b-Ax| / (||A||_inf ||x||_inf + ||b||_inf)
+    (tests/ssids/ssids.f90:1762-1810), per right-hand side."""
+    x = np.asarray(x)
+    b = np.asarray(b)
+    if x.ndim == 1:
+        x = x[:, None]
+        b = b[:, None]
+    r = b - full @ x
+    anorm = abs(full).sum(axis=1).max()
+    out = []
+    for j in range(x.shape[1]):
+        den = anorm * np.abs(x[:, j]).max() + np.abs(b[:, j]).max()
+        out.append(np.abs(r[:, j]).max() / den if den > 0 else 0.0)
+    return np.array(out)

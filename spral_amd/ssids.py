@@ -1,0 +1,354 @@
+"""Host-side mirror of the SSIDS API (src/ssids/ssids.f90) over the HIP C-ABI.
+
+Names, argument meaning and flag values follow the reference:
+  ssids_analyse        src/ssids/ssids.f90:148-389   (analyse_double)
+  ssids_factor         src/ssids/ssids.f90:767-1109  (ssids_factor_ptr64_double)
+  ssids_solve          src/ssids/ssids.f90:1140-1250 (ssids_solve_mult_double)
+  ssids_enquire_indef  src/ssids/ssids.f90:1300-1356
+  ssids_alter          src/ssids/ssids.f90:1362-1399
+  ssids_free           src/ssids/ssids.f90:1403-1434
+The numeric work runs on the GPU through libspral_ssids_hip.so; there is no
+CPU fallback.  Matrices are lower-triangle CSC with 1-based ptr/row as in the
+reference.  Orderings: options.ordering = 0 takes the user's order (as the
+reference); 1 (the reference's METIS) uses this package's minimum degree since
+METIS is not available; 3 is geometric nested dissection for an nx*ny*nz grid
+(pass grid=(nx,ny,nz)).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+# flags (src/ssids/datatypes.f90:25-59)
+SSIDS_SUCCESS = 0
+SSIDS_ERROR_CALL_SEQUENCE = -1
+SSIDS_ERROR_A_N_OOR = -2
+SSIDS_ERROR_A_PTR = -3
+SSIDS_ERROR_SINGULAR = -5
+SSIDS_ERROR_NOT_POS_DEF = -6
+SSIDS_ERROR_ORDER = -8
+SSIDS_ERROR_X_SIZE = -10
+SSIDS_ERROR_JOB_OOR = -11
+SSIDS_ERROR_NOT_LDLT = -14
+SSIDS_ERROR_ALLOCATION = -50
+SSIDS_ERROR_CUDA_UNKNOWN = -51
+SSIDS_ERROR_UNIMPLEMENTED = -98
+SSIDS_WARNING_ANAL_SINGULAR = 6
+SSIDS_WARNING_FACT_SINGULAR = 7
+
+SSIDS_SOLVE_JOB_ALL = 0
+SSIDS_SOLVE_JOB_FWD = 1
+SSIDS_SOLVE_JOB_DIAG = 2
+SSIDS_SOLVE_JOB_BWD = 3
+SSIDS_SOLVE_JOB_DIAG_BWD = 4
+
+
+@dataclass
+class SSIDSOptions:
+    """Subset of ssids_options (src/ssids/datatypes.f90:188-284) the path uses."""
+    print_level: int = 0
+    ordering: int = 1
+    nemin: int = 32
+    action: bool = True
+    pivot_method: int = 2
+    small: float = 1e-20
+    u: float = 0.01
+    multiplier: float = 1.1
+    failed_pivot_method: int = 1
+    scaling: int = 0
+    use_gpu: bool = True
+    schur_precision: int = 0
+    device: int = 0
+
+    def to_c(self):
+        o = _lib.default_options()
+        o.print_level = self.print_level
+        o.action = 1 if self.action else 0
+        o.small = self.small
+        o.u = self.u
+        o.multiplier = self.multiplier
+        o.pivot_method = min(3, max(1, self.pivot_method))
+        o.failed_pivot_method = min(2, max(1, self.failed_pivot_method))
+        o.schur_precision = self.schur_precision
+        return o
+
+
+@dataclass
+class SSIDSInform:
+    """Subset of ssids_inform (src/ssids/inform.f90:17-50)."""
+    flag: int = 0
+    matrix_rank: int = 0
+    maxfront: int = 0
+    maxsupernode: int = 0
+    num_delay: int = 0
+    num_factor: int = 0
+    num_flops: int = 0
+    num_neg: int = 0
+    num_sup: int = 0
+    num_two: int = 0
+    not_first_pass: int = 0
+    not_second_pass: int = 0
+    cuda_error: int = 0
+    factor_ms: float = 0.0
+    num_levels: int = 0
+
+
+@dataclass
+class AKeep:
+    n: int = 0
+    ptr: np.ndarray = None
+    row: np.ndarray = None
+    order: np.ndarray = None   # final pivot order (1-based; 0 unused)
+    invp: np.ndarray = None    # invp[i] = variable at pivot position i (0-based)
+    nnodes: int = 0
+    sptr: np.ndarray = None
+    sparent: np.ndarray = None
+    rptr: np.ndarray = None
+    rlist: np.ndarray = None
+    nptr: np.ndarray = None
+    nlist: np.ndarray = None
+    num_factor: int = 0
+    num_flops: int = 0
+    symbolic: int = None       # C handle of the device symbolic subtree
+    inform: SSIDSInform = field(default_factory=SSIDSInform)
+    nz: int = 0
+
+
+@dataclass
+class FKeep:
+    numeric: int = None
+    pos_def: bool = False
+    scaling: np.ndarray = None
+    inform: SSIDSInform = field(default_factory=SSIDSInform)
+
+
+def _as_i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _as_i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None):
+    """Analyse the sparsity pattern; returns (akeep, inform)."""
+    options = options or SSIDSOptions()
+    L = _lib.lib()
+    inform = SSIDSInform()
+    ak = AKeep(n=n)
+    if n < 0:
+        inform.flag = SSIDS_ERROR_A_N_OOR
+        ak.inform = inform
+        return ak, inform
+    ptr = _as_i64(ptr)
+    row = _as_i32(row)
+    ak.ptr, ak.row, ak.nz = ptr, row, int(ptr[-1] - 1)
+    if options.ordering == 0:
+        if order is None:
+            inform.flag = SSIDS_ERROR_ORDER
+            ak.inform = inform
+            return ak, inform
+        ordv = _as_i32(np.abs(order)).copy()
+    elif options.ordering == 3:
+        nx, ny, nz = grid
+        ordv = np.zeros(n, np.int32)
+        L.spral_ssids_hip_order_grid_nd(nx, ny, nz, ordv)
+    else:
+        ordv = np.zeros(n, np.int32)
+        L.spral_ssids_hip_order_min_degree(n, ptr, row, ordv)
+    flag = C.c_int(0)
+    h = L.spral_ssids_hip_analyse(n, ptr, row, ordv, options.nemin, C.byref(flag))
+    if not h:
+        inform.flag = flag.value
+        ak.inform = inform
+        return ak, inform
+    try:
+        nn, rl, nl = C.c_int(), C.c_int64(), C.c_int64()
+        nf, fl, mf, ms = C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+        L.spral_ssids_hip_analyse_info(h, C.byref(nn), C.byref(rl), C.byref(nl), C.byref(nf),
+                                       C.byref(fl), C.byref(mf), C.byref(ms))
+        ak.nnodes = nn.value
+        ak.sptr = np.zeros(nn.value + 1, np.int32)
+        ak.sparent = np.zeros(nn.value, np.int32)
+        ak.rptr = np.zeros(nn.value + 1, np.int64)
+        ak.rlist = np.zeros(rl.value, np.int32)
+        ak.nptr = np.zeros(nn.value + 1, np.int64)
+        ak.nlist = np.zeros(2 * nl.value, np.int64)
+        L.spral_ssids_hip_analyse_arrays(h, ak.sptr, ak.sparent, ak.rptr, ak.rlist, ak.nptr,
+                                         ak.nlist)
+        ak.num_factor, ak.num_flops = nf.value, fl.value
+        inform.maxfront, inform.maxsupernode = mf.value, ms.value
+    finally:
+        L.spral_ssids_hip_analyse_free(h)
+    ak.order = ordv
+    invp = np.zeros(n, np.int64)
+    used = ordv > 0
+    invp[ordv[used] - 1] = np.nonzero(used)[0]
+    ak.invp = invp
+    inform.flag = max(inform.flag, flag.value)
+    inform.num_factor, inform.num_flops = ak.num_factor, ak.num_flops
+    inform.num_sup = ak.nnodes
+    inform.matrix_rank = int(ak.sptr[-1] - 1) if ak.nnodes else 0
+    # device symbolic subtree covering the whole tree
+    if options.use_gpu and ak.nnodes > 0:
+        f = C.c_int(0)
+        opts = options.to_c()
+        sym = L.spral_ssids_hip_create_symbolic(options.device, n, 1, ak.nnodes + 1, ak.sptr,
+                                                ak.sparent, ak.rptr, ak.rlist, ak.nptr,
+                                                ak.nlist, C.byref(opts), C.byref(f))
+        if not sym:
+            inform.flag = f.value
+        ak.symbolic = sym
+    ak.inform = inform
+    return ak, inform
+
+
+def _inform_from_stats(st, akeep):
+    inf = SSIDSInform()
+    inf.flag = st.flag
+    inf.num_delay = st.num_delay
+    inf.num_factor = st.num_factor
+    inf.num_flops = st.num_flops
+    inf.num_neg = st.num_neg
+    inf.num_two = st.num_two
+    inf.maxfront = st.maxfront
+    inf.maxsupernode = st.maxsupernode
+    inf.not_first_pass = st.not_first_pass
+    inf.not_second_pass = st.not_second_pass
+    inf.cuda_error = st.hip_error
+    inf.matrix_rank = akeep.inform.matrix_rank - st.num_zero
+    inf.num_sup = akeep.nnodes
+    inf.factor_ms = st.factor_ms
+    inf.num_levels = st.num_levels
+    return inf
+
+
+def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
+    """Numeric factorization on the GPU; returns (fkeep, inform)."""
+    options = options or SSIDSOptions()
+    L = _lib.lib()
+    if akeep.inform.flag < 0 or akeep.symbolic is None:
+        inf = SSIDSInform(flag=SSIDS_ERROR_CALL_SEQUENCE)
+        return FKeep(inform=inf), inf
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    st = _lib.Stats()
+    opts = options.to_c()
+    sc = None
+    scp = None
+    if scale is not None:
+        # scaling is applied in elimination order inside the subtree
+        sc = np.ascontiguousarray(np.asarray(scale, np.float64)[akeep.invp])
+        scp = sc.ctypes.data_as(C.c_void_p)
+    if fkeep is not None and fkeep.numeric:
+        L.spral_ssids_hip_destroy_numeric(fkeep.numeric)
+    num = L.spral_ssids_hip_factor(bool(posdef), akeep.symbolic, val, val.size, scp,
+                                   C.byref(opts), C.byref(st))
+    inf = _inform_from_stats(st, akeep)
+    fk = FKeep(numeric=num, pos_def=bool(posdef), scaling=sc, inform=inf)
+    if num and inf.flag == SSIDS_WARNING_FACT_SINGULAR and not options.action:
+        inf.flag = SSIDS_ERROR_SINGULAR
+    return fk, inf
+
+
+def ssids_solve(x, akeep, fkeep, options=None, job=None):
+    """Solve in place; x is (n,) or (n, nrhs).  Returns inform."""
+    options = options or SSIDSOptions()
+    L = _lib.lib()
+    inf = SSIDSInform()
+    if fkeep is None or not fkeep.numeric or fkeep.inform.flag < 0:
+        inf.flag = SSIDS_ERROR_CALL_SEQUENCE
+        return inf
+    inf.flag = max(SSIDS_SUCCESS, fkeep.inform.flag)
+    job = SSIDS_SOLVE_JOB_ALL if job is None else job
+    if job < 0 or job > 4:
+        inf.flag = SSIDS_ERROR_JOB_OOR
+        return inf
+    xa = np.asarray(x)
+    one = xa.ndim == 1
+    X = xa.reshape(akeep.n, -1)
+    n = akeep.n
+    x2 = np.asfortranarray(X[akeep.invp, :], dtype=np.float64)
+    if fkeep.scaling is not None and job in (0, 1):
+        x2 *= fkeep.scaling[:, None]
+    x2c = np.ascontiguousarray(x2.T)  # (nrhs, n) row-major == (n, nrhs) col-major
+    fn = {0: None, 1: L.spral_ssids_hip_solve_fwd, 2: L.spral_ssids_hip_solve_diag,
+          3: L.spral_ssids_hip_solve_bwd, 4: L.spral_ssids_hip_solve_diag_bwd}[job]
+    nrhs = x2c.shape[0]
+    if job == 0:
+        # fkeep%inner_solve: solve_fwd then solve_diag_bwd (src/ssids/fkeep.F90:269-297)
+        flag = _solve_all(L, fkeep.numeric, nrhs, x2c, n)
+    else:
+        flag = fn(fkeep.numeric, nrhs, x2c.reshape(-1), n)
+    if flag < 0:
+        inf.flag = flag
+        return inf
+    res = x2c.T
+    if fkeep.scaling is not None and job in (0, 3, 4):
+        res = res * fkeep.scaling[:, None]
+    out = np.empty_like(res)
+    out[akeep.invp, :] = res
+    if one:
+        xa[:] = out[:, 0]
+    else:
+        X[:, :] = out
+    return inf
+
+
+def _solve_all(L, num, nrhs, x2c, n):
+    flag = L.spral_ssids_hip_solve_fwd(num, nrhs, x2c.reshape(-1), n)
+    if flag < 0:
+        return flag
+    return L.spral_ssids_hip_solve_diag_bwd(num, nrhs, x2c.reshape(-1), n)
+
+
+def ssids_enquire_indef(akeep, fkeep, options=None):
+    """Returns (piv_order (1-based, negative for 2x2 members), d (2,n), inform)
+    as ssids_enquire_indef (src/ssids/ssids.f90:1300-1356)."""
+    L = _lib.lib()
+    inf = SSIDSInform()
+    if fkeep is None or not fkeep.numeric:
+        inf.flag = SSIDS_ERROR_CALL_SEQUENCE
+        return None, None, inf
+    if fkeep.pos_def:
+        inf.flag = SSIDS_ERROR_NOT_LDLT
+        return None, None, inf
+    n = akeep.n
+    po = np.zeros(n, np.int32)
+    d = np.zeros(2 * n, np.float64)
+    flag = L.spral_ssids_hip_enquire_indef(fkeep.numeric, po.ctypes.data_as(C.c_void_p),
+                                           d.ctypes.data_as(C.c_void_p))
+    if flag < 0:
+        inf.flag = flag
+        return None, None, inf
+    piv = np.zeros(n, np.int32)
+    piv[akeep.invp] = po  # subtree result is by elimination position
+    piv = np.where(piv >= 0, piv + 1, piv - 1)
+    return piv, d.reshape(n, 2).T.copy(), inf
+
+
+def ssids_alter(d, akeep, fkeep, options=None):
+    """Overwrite D^{-1} (d is (2, n) in pivot order)."""
+    L = _lib.lib()
+    inf = SSIDSInform()
+    if fkeep is None or not fkeep.numeric:
+        inf.flag = SSIDS_ERROR_CALL_SEQUENCE
+        return inf
+    if fkeep.pos_def:
+        inf.flag = SSIDS_ERROR_NOT_LDLT
+        return inf
+    dd = np.ascontiguousarray(np.asarray(d, np.float64).T.reshape(-1))
+    inf.flag = L.spral_ssids_hip_alter(fkeep.numeric, dd)
+    return inf
+
+
+def ssids_free(akeep=None, fkeep=None):
+    L = _lib.lib()
+    if fkeep is not None and fkeep.numeric:
+        L.spral_ssids_hip_destroy_numeric(fkeep.numeric)
+        fkeep.numeric = None
+    if akeep is not None and akeep.symbolic:
+        L.spral_ssids_hip_destroy_symbolic(akeep.symbolic)
+        akeep.symbolic = None
+    return 0

@@ -1,0 +1,30 @@
+// Test-only probe of the v_mfma_f64_16x16x4_f64 operand/result layout
+// (in-bounds: one wave, 16x4 A, 4x16 B, 16x16 D).  Built into
+// tests/hip/libprobe.so; never linked by the product library.
+#include <hip/hip_runtime.h>
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__global__ void k_probe(const double *A, const double *B, double *out) {
+  const int lane = threadIdx.x;  // 64 lanes
+  // assumed layout: A[i][k] at lane i + 16k, B[k][j] at lane j + 16k
+  double a = A[(lane & 15) * 4 + (lane >> 4)];
+  double b = B[(lane >> 4) * 16 + (lane & 15)];
+  double4_t acc = {0.0, 0.0, 0.0, 0.0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = acc[r];  // raw lane/register dump
+}
+
+extern "C" int probe_mfma_f64(const double *hA, const double *hB, double *hout) {
+  double *A, *B, *O;
+  if (hipMalloc(&A, 64 * sizeof(double)) != hipSuccess) return -1;
+  if (hipMalloc(&B, 64 * sizeof(double)) != hipSuccess) return -1;
+  if (hipMalloc(&O, 256 * sizeof(double)) != hipSuccess) return -1;
+  (void)hipMemcpy(A, hA, 64 * sizeof(double), hipMemcpyHostToDevice);
+  (void)hipMemcpy(B, hB, 64 * sizeof(double), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, 0, A, B, O);
+  hipError_t e = hipMemcpy(hout, O, 256 * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  (void)hipFree(O);
+  return e == hipSuccess ? 0 : -2;
+}
