@@ -1,0 +1,179 @@
+#!/usr/bin/env python
+"""Benchmark of the SSIDS numeric path on one MI355X.
+
+Workload: indefinite shifted 3D Laplacian, 7-point stencil on a 64^3 grid
+(n = 262,144), diagonal 6 - 0.3, off-diagonals -1, FP64, geometric nested
+dissection order, one right-hand side from x_true ~ U(-1,1).
+
+One step = one numeric factorization of the values already resident in HBM
+(re-using the symbolic analysis and the handle's device pool) followed by
+one device-resident solve.  `value` is factor GFLOP/s with the analyse-phase
+flop count (inform%num_flops); solve p50/p99 come from extra solves after the
+timed steps.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_MFMA_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--solves", type=int, default=50, help="solve latency samples")
+    ap.add_argument("--cpu-grid", type=int, default=24,
+                    help="grid of the bounded CPU-baseline sample (oracle)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(grid):
+    """Time the CPU oracle (single-threaded restatement of the reference TPP
+    path) on a bounded sample: the same stencil on a grid^3 box."""
+    from oracle.oracle_py import OracleFactor  # checker / baseline only
+    from spral_amd import matrices, ssids
+    m = matrices.laplace3d(grid, diag=5.7)
+    opts = ssids.SSIDSOptions(ordering=3, use_gpu=False)
+    ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, grid=(grid,) * 3)
+    t = time.perf_counter()
+    of = OracleFactor(ak, m.val)
+    dt = time.perf_counter() - t
+    return {"value": ak.num_flops / dt / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"oracle factor of the same stencil on {grid}^3 (n={m.n}, "
+                      f"{ak.num_flops:.3e} flops) in {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
+              "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from spral_amd import _lib, matrices, ssids
+    if not _lib.LIB_PATH.exists():
+        from spral_amd import build
+        build.build(verbose=False)
+    import torch
+    L = _lib.lib()
+    g = args.grid
+    m = matrices.laplace3d(g, diag=6.0 - 0.3)
+    opts = ssids.SSIDSOptions(ordering=3)
+    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, grid=(g, g, g))
+    assert ainf.flag >= 0, ainf
+    flops = ak.num_flops
+    dev = torch.device("cuda:0")
+    d_val = torch.from_numpy(m.val).to(dev)
+    rng = np.random.Generator(np.random.PCG64(20261202 + 2))
+    x_true = rng.uniform(-1, 1, m.n)
+    b = m.full @ x_true
+    b2 = torch.from_numpy(np.ascontiguousarray(b[ak.invp])).to(dev)
+    d_x = torch.empty_like(b2)
+    copt = opts.to_c()
+    st = _lib.Stats()
+    torch.cuda.synchronize()
+    num = L.spral_ssids_hip_factor_device(False, ak.symbolic, C.c_void_p(d_val.data_ptr()),
+                                          m.val.size, None, C.byref(copt), C.byref(st))
+    assert num, f"factor failed flag={st.flag}"
+
+    def refactor(timing=0):
+        copt.timing = timing
+        s = _lib.Stats()
+        f = L.spral_ssids_hip_refactor_device(num, C.c_void_p(d_val.data_ptr()), None,
+                                              C.byref(copt), C.byref(s))
+        assert f >= 0, f
+        return s
+
+    def solve():
+        d_x.copy_(b2)
+        f = L.spral_ssids_hip_solve_device(num, 0, 1, C.c_void_p(d_x.data_ptr()), m.n, None)
+        assert f == 0, f
+
+    for _ in range(args.warmup):
+        refactor()
+        solve()
+    torch.cuda.synchronize()
+    fac_ms, sol_ms = [], []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s = refactor()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        solve()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fac_ms.append((t1 - t0) * 1e3)
+        sol_ms.append((t2 - t1) * 1e3)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_all
+    # solve latency samples on the resident factors
+    lat = []
+    for _ in range(args.solves):
+        t0 = time.perf_counter()
+        solve()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    x = d_x.cpu().numpy()
+    xo = np.empty_like(x)
+    xo[ak.invp] = x
+    bwd = float(matrices.bwd_error(m.full, xo, b)[0])
+    # profiled factorization: events around every Schur-update launch
+    sp = refactor(timing=1)
+    torch.cuda.synchronize()
+    ach = sp.update_flops / (sp.update_ms * 1e-3) / 1e12
+    fac_mean = float(np.mean(fac_ms))
+    value = flops / (fac_mean * 1e-3) / 1e9
+    out = {
+        "metric": "factor GFLOP/s + solve p50/p99 ms at bwd err <=1e-14, FP64 vs mixed",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"laplace7pt-{g}^3 sigma=0.3 FP64 nested-dissection, 1 RHS",
+                   "n": m.n, "nz_lower": m.nz, "num_flops": flops,
+                   "num_factor": ak.num_factor, "parallelism": "single-gpu"},
+        "factor_ms": round(fac_mean, 3),
+        "factor_ms_device": round(s.factor_ms, 3),
+        "solve_ms_p50": round(float(np.percentile(lat, 50)), 3),
+        "solve_ms_p99": round(float(np.percentile(lat, 99)), 3),
+        "bwd_err": bwd,
+        "num_neg": s.num_neg,
+        "num_delay": s.num_delay,
+        "num_two": s.num_two,
+        "roofline": {"bound": "mfma", "kernel": "k_update (FP64 MFMA Schur update)",
+                     "achieved": round(ach, 3), "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_MFMA_TFLOPS, 4),
+                     "traffic": None, "launches": sp.update_launches,
+                     "avg_launch_us": round(sp.update_ms * 1e3 / max(1, sp.update_launches), 2),
+                     "flops_per_launch": sp.update_flops / max(1, sp.update_launches)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_grid)
+    L.spral_ssids_hip_destroy_numeric(num)
+    ssids.ssids_free(ak)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,6 +44,7 @@ struct spral_ssids_hip_options {
   int schur_precision;     /* 0 fp64, 1 fp32, 2 fp16 trailing updates       */
   int refine_max;          /* max FP64 refinement steps in solve_refine     */
   double refine_tol;       /* target scaled backward error                  */
+  int timing;              /* 1: time the Schur-update launches with events */
 };
 
 /* Interoperable subset of ssids_inform, as cpu_factor_stats
@@ -63,6 +64,10 @@ struct spral_ssids_hip_stats {
   int hip_error;           /* hipError_t of the first failing call, else 0  */
   int num_levels;          /* tree levels walked                            */
   double factor_ms;        /* device time of the numeric factorization      */
+  double update_ms;        /* summed device time of Schur-update launches   */
+  double update_flops;     /* algorithmic flops of those launches           */
+  int update_launches;     /* number of Schur-update launches timed         */
+  int64_t pool_peak_bytes; /* peak device pool use                          */
 };
 
 /* Fill *opt with the reference defaults (src/ssids/datatypes.f90:188-284). */
@@ -154,6 +159,19 @@ int spral_ssids_hip_solve_bwd(void *numeric, int nrhs, double *x, int ldx);
  * (NULL = the handle's own stream).  Asynchronous with respect to the host. */
 int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x,
                                  int ldx, void *stream);
+
+/* FP64 iterative refinement (no reference counterpart: the reference solve has
+ * none).  x = solve(b), then x += solve(b - A x) with A the factorized values
+ * (scaled system when a scaling was given) until every column's scaled
+ * backward error max|b-Ax| / (||A||inf ||x||inf + ||b||inf)
+ * (tests/ssids/ssids.f90:1762-1810) is <= tol, or maxit corrections.
+ * bwd (host, nrhs) and *iters receive the final state.  b, x in elimination
+ * order.  _device: b and x are distinct buffers in HBM. */
+int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, double *x,
+                                 int ldx, int maxit, double tol, double *bwd, int *iters);
+int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,
+                                        double *d_x, int ldx, int maxit, double tol,
+                                        double *bwd, int *iters, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Enquiry / alteration: replace gpu_numeric_subtree%enquire_indef and %alter

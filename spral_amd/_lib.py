@@ -28,6 +28,7 @@ class Options(C.Structure):
         ("schur_precision", C.c_int),
         ("refine_max", C.c_int),
         ("refine_tol", C.c_double),
+        ("timing", C.c_int),
     ]
 
 
@@ -48,6 +49,10 @@ class Stats(C.Structure):
         ("hip_error", C.c_int),
         ("num_levels", C.c_int),
         ("factor_ms", C.c_double),
+        ("update_ms", C.c_double),
+        ("update_flops", C.c_double),
+        ("update_launches", C.c_int),
+        ("pool_peak_bytes", C.c_int64),
     ]
 
 
@@ -86,6 +91,13 @@ SIGNATURES = {
     "spral_ssids_hip_solve_bwd": (C.c_int, [C.c_void_p, C.c_int, f64p, C.c_int]),
     "spral_ssids_hip_solve_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                                C.c_void_p, C.c_int, C.c_void_p]),
+    "spral_ssids_hip_solve_refine": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p, C.c_int,
+                                               C.c_int, C.c_double, f64p,
+                                               C.POINTER(C.c_int)]),
+    "spral_ssids_hip_solve_refine_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                                      C.c_void_p, C.c_int, C.c_int,
+                                                      C.c_double, f64p, C.POINTER(C.c_int),
+                                                      C.c_void_p]),
     "spral_ssids_hip_enquire_indef": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "spral_ssids_hip_alter": (C.c_int, [C.c_void_p, f64p]),
     "spral_ssids_hip_get_contrib": (None, [C.c_void_p] + [C.c_void_p] * 8),

@@ -65,6 +65,14 @@ def build_oracle() -> None:
     _run(["make", "-s", "-C", HERE.parent / "oracle", "all"])
 
 
+def build_probe() -> None:
+    """Compile the test-only MFMA layout probe (tests/hip/libprobe.so)."""
+    src = HERE.parent / "tests" / "hip" / "probe.hip"
+    if src.exists():
+        _run([HIPCC, "-O2", "-fPIC", "-shared", *DEVICE, "-x", "hip", src,
+              "-o", src.with_name("libprobe.so")])
+
+
 if __name__ == "__main__":
     build(force="--force" in sys.argv)
     build_oracle()

@@ -175,10 +175,17 @@ __global__ __launch_bounds__(kThreads) void k_assemble(
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block factorization (complete pivoting inside the block)
+// Diagonal block factorization (complete pivoting inside the block).
+// Each thread keeps 16 entries of the symmetric w x w block in registers:
+// row i = tid % 64, columns j = tid/64 + 4q (q < 16).  Pivots are not swapped
+// physically: eliminated indices are flagged, the pivot column is broadcast
+// through LDS, and the elimination order lp[] permutes rows/columns when the
+// block is written back.  Pivot rules follow
+// src/ssids/cpu/kernels/block_ldlt.hxx:213-414.
 // ---------------------------------------------------------------------------
 struct MaxLoc {
-  double v;
+  double v;   // |a|
+  double s;   // signed a
   int r, c;
 };
 
@@ -191,16 +198,17 @@ __device__ inline bool better(const MaxLoc &a, const MaxLoc &b) {
 
 __global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lvl_slots,
                                                    int step, FactorParams fp) {
-  __shared__ double a[kBlk][kBlk + 1];
-  __shared__ double dinv[2 * kBlk];
-  __shared__ int lp[kBlk];
+  __shared__ double col1[kBlk], col2[kBlk], diag[kBlk];
+  __shared__ double lout[kBlk][kBlk + 1];  // L column k (elimination order), row = original
+  __shared__ double dinv[2 * kBlk + 2];
+  __shared__ int lp[kBlk], inv_lp[kBlk], elim[kBlk];
   __shared__ MaxLoc red[kThreads / 64];
-  __shared__ int sh_p, sh_w, sh_piv1, sh_piv2, sh_size, sh_stop;
-  __shared__ double sh_d[4];
+  __shared__ int sh_p, sh_w, sh_piv1, sh_piv2, sh_size;
+  __shared__ double sh_d[3];
   const int s = lvl_slots[blockIdx.x];
   NodeDev &N = nodes[s];
   if (step >= N.nsteps) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
   if (tid == 0) {
     int p, q;
     if (step == 0) {
@@ -224,57 +232,61 @@ __global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lv
   if (w == 0) return;
   const int ldl = N.ldl;
   double *L = N.L;
-  for (int idx = tid; idx < w * w; idx += kThreads) {
-    int r = idx % w, c = idx / w;
-    if (r >= c) {
-      double v = L[(size_t)(p + c) * ldl + p + r];
-      a[r][c] = v;
-      a[c][r] = v;
-      N.dbak[r + c * kBlk] = v;
-      N.dbak[c + r * kBlk] = v;
+  const int ri = tid & 63, c0 = tid >> 6;  // row and first column owned
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    int cj = c0 + 4 * q;
+    double x = 0.0;
+    if (ri < w && cj < w) {
+      int hi = max(ri, cj), lo = min(ri, cj);
+      x = L[(size_t)(p + lo) * ldl + p + hi];
+      N.dbak[ri + cj * kBlk] = x;
     }
+    v[q] = x;
+    if (ri == cj && ri < w) diag[ri] = x;
   }
   for (int i = tid; i < kBlk; i += kThreads) {
-    lp[i] = i;
+    elim[i] = (i < w) ? 0 : 1;
     N.colmax[i] = 0ull;
   }
   __syncthreads();
-
   int k = 0;
   while (k < w) {
-    // ---- max over the remaining lower triangle
-    MaxLoc best{-1.0, w, w};
-    const int rem = w - k;
-    for (int idx = tid; idx < rem * rem; idx += kThreads) {
-      int r = k + idx % rem, c = k + idx / rem;
-      if (r >= c) {
-        MaxLoc cand{fabs(a[r][c]), r, c};
-        if (better(cand, best)) best = cand;
+    // ---- search the remaining lower triangle
+    MaxLoc best{-1.0, 0.0, w, w};
+    if (ri < w && !elim[ri]) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        int cj = c0 + 4 * q;
+        if (cj <= ri && !elim[cj]) {
+          MaxLoc cand{fabs(v[q]), v[q], ri, cj};
+          if (better(cand, best)) best = cand;
+        }
       }
     }
     for (int off = 32; off > 0; off >>= 1) {
       MaxLoc o;
       o.v = __shfl_xor(best.v, off);
+      o.s = __shfl_xor(best.s, off);
       o.r = __shfl_xor(best.r, off);
       o.c = __shfl_xor(best.c, off);
       if (better(o, best)) best = o;
     }
-    if ((tid & 63) == 0) red[tid >> 6] = best;
+    if (lane == 0) red[tid >> 6] = best;
     __syncthreads();
     if (tid == 0) {
       MaxLoc b = red[0];
       for (int i = 1; i < kThreads / 64; ++i)
         if (better(red[i], b)) b = red[i];
-      sh_stop = 0;
-      if (!(b.v >= fp.small)) {  // all remaining entries are (near) zero
-        sh_stop = 1;
-        sh_size = 0;
+      if (!(b.v >= fp.small)) {
+        sh_size = 0;  // everything left is (near) zero
       } else if (b.r == b.c) {
         sh_size = 1;
         sh_piv1 = b.r;
       } else {
-        int mcol = b.c, trow = b.r;  // mcol < trow
-        double a11 = a[mcol][mcol], a22 = a[trow][trow], a21 = a[trow][mcol];
+        int mcol = b.c, trow = b.r;
+        double a11 = diag[mcol], a22 = diag[trow], a21 = b.s;
         double detscale = 1.0 / fabs(a21);
         double detpiv = (a11 * detscale) * a22 - fabs(a21);
         if (fabs(detpiv) >= fabs(a21) / 2) {
@@ -291,101 +303,101 @@ __global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lv
       }
     }
     __syncthreads();
-    if (sh_stop) {
-      // zero pivots for all remaining columns
-      for (int j = k + tid; j < w; j += kThreads) {
-        dinv[2 * j] = 0.0;
-        dinv[2 * j + 1] = 0.0;
-      }
-      for (int idx = tid; idx < rem * rem; idx += kThreads) {
-        int r = k + idx % rem, c = k + idx / rem;
-        if (r >= c) a[r][c] = 0.0;
-      }
+    const int size = sh_size;
+    if (size == 0) {
+      // zero pivots for all remaining indices, in increasing index order
       if (tid == 0) {
+        int kk = k;
+        for (int i = 0; i < w; ++i)
+          if (!elim[i]) {
+            lp[kk] = i;
+            dinv[2 * kk] = 0.0;
+            dinv[2 * kk + 1] = 0.0;
+            ++kk;
+          }
         N.nzero += w - k;
         if (!fp.action) N.flag = -5;
       }
+      for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
+        int r = idx % kBlk, kk = idx / kBlk;
+        if (kk >= k) lout[r][kk] = 0.0;
+      }
       __syncthreads();
+      k = w;
       break;
     }
-    const int size = sh_size;
-    // ---- symmetric interchanges (full storage: rows then columns)
-    for (int sw = 0; sw < size; ++sw) {
-      const int dst = k + sw, src = (sw == 0) ? sh_piv1 : sh_piv2;
-      if (dst != src) {
-        for (int x = tid; x < w; x += kThreads) {
-          double t = a[dst][x];
-          a[dst][x] = a[src][x];
-          a[src][x] = t;
-        }
-        __syncthreads();
-        for (int x = tid; x < w; x += kThreads) {
-          double t = a[x][dst];
-          a[x][dst] = a[x][src];
-          a[x][src] = t;
-        }
-        if (tid == 0) {
-          int t = lp[dst];
-          lp[dst] = lp[src];
-          lp[src] = t;
-        }
-        __syncthreads();
-      }
+    const int p1 = sh_piv1, p2 = (size == 2) ? sh_piv2 : -1;
+    // ---- broadcast pivot column(s): owners of column p1 / p2 write them
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int cj = c0 + 4 * q;
+      if (cj == p1 && ri < w) col1[ri] = v[q];
+      if (cj == p2 && ri < w) col2[ri] = v[q];
     }
+    __syncthreads();
     if (size == 1) {
-      const double piv = a[k][k];
-      const double di = 1.0 / piv;
-      // trailing update a[i][j] -= a[i][k] a[j][k] / piv  (i,j > k)
-      const int r2 = w - k - 1;
-      for (int idx = tid; idx < r2 * r2; idx += kThreads) {
-        int i = k + 1 + idx % r2, j = k + 1 + idx / r2;
-        a[i][j] -= a[i][k] * a[j][k] * di;
-      }
-      __syncthreads();
-      for (int i = k + 1 + tid; i < w; i += kThreads) {
-        a[i][k] *= di;
-        a[k][i] = a[i][k];
+      const double di = 1.0 / col1[p1];
+      // L column for every non-eliminated row other than the pivot
+      if (tid < w) lout[tid][k] = (tid == p1) ? 1.0 : (elim[tid] ? 0.0 : col1[tid] * di);
+      const double li = (ri < w) ? col1[ri] * di : 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        int cj = c0 + 4 * q;
+        if (cj < w) v[q] -= li * col1[cj];
       }
       if (tid == 0) {
         dinv[2 * k] = di;
         dinv[2 * k + 1] = 0.0;
-        a[k][k] = 1.0;
+        lp[k] = p1;
+        elim[p1] = 1;
       }
-      __syncthreads();
-      k += 1;
     } else {
       const double d11 = sh_d[0], d21 = sh_d[1], d22 = sh_d[2];
-      const int r2 = w - k - 2;
-      for (int idx = tid; idx < r2 * r2; idx += kThreads) {
-        int i = k + 2 + idx % r2, j = k + 2 + idx / r2;
-        double x1 = a[i][k], x2 = a[i][k + 1];
-        double l1 = d11 * x1 + d21 * x2, l2 = d21 * x1 + d22 * x2;
-        a[i][j] -= l1 * a[j][k] + l2 * a[j][k + 1];
+      if (tid < w) {
+        double x1 = col1[tid], x2 = col2[tid];
+        bool pv = (tid == p1 || tid == p2), el = elim[tid];
+        lout[tid][k] = (tid == p1) ? 1.0 : ((pv || el) ? 0.0 : d11 * x1 + d21 * x2);
+        lout[tid][k + 1] = (tid == p2) ? 1.0 : ((pv || el) ? 0.0 : d21 * x1 + d22 * x2);
       }
-      __syncthreads();
-      for (int i = k + 2 + tid; i < w; i += kThreads) {
-        double x1 = a[i][k], x2 = a[i][k + 1];
-        double l1 = d11 * x1 + d21 * x2, l2 = d21 * x1 + d22 * x2;
-        a[i][k] = l1;
-        a[i][k + 1] = l2;
+      double l1 = 0.0, l2 = 0.0;
+      if (ri < w) {
+        double x1 = col1[ri], x2 = col2[ri];
+        l1 = d11 * x1 + d21 * x2;
+        l2 = d21 * x1 + d22 * x2;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        int cj = c0 + 4 * q;
+        if (cj < w) v[q] -= l1 * col1[cj] + l2 * col2[cj];
       }
       if (tid == 0) {
         dinv[2 * k] = d11;
         dinv[2 * k + 1] = d21;
         dinv[2 * k + 2] = INFINITY;
         dinv[2 * k + 3] = d22;
-        a[k][k] = 1.0;
-        a[k + 1][k] = 0.0;
-        a[k + 1][k + 1] = 1.0;
+        lp[k] = p1;
+        lp[k + 1] = p2;
+        elim[p1] = 1;
+        elim[p2] = 1;
       }
-      __syncthreads();
-      k += 2;
     }
+    // refresh the diagonal cache for the next search
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      int cj = c0 + 4 * q;
+      if (cj == ri && ri < w) diag[ri] = v[q];
+    }
+    __syncthreads();
+    k += size;
   }
-  // ---- write back L_bb, D^{-1}, local permutation
+  // ---- write back L_bb (rows permuted into elimination order), D^{-1}, lp
+  for (int i = tid; i < w; i += kThreads) inv_lp[lp[i]] = i;
+  __syncthreads();
   for (int idx = tid; idx < w * w; idx += kThreads) {
-    int r = idx % w, c = idx / w;
-    if (r >= c) L[(size_t)(p + c) * ldl + p + r] = a[r][c];
+    int r = idx % w, kk = idx / w;  // original row r, elimination column kk
+    int nr = inv_lp[r];
+    if (nr > kk) L[(size_t)(p + kk) * ldl + p + nr] = lout[r][kk];
+    else if (nr == kk) L[(size_t)(p + kk) * ldl + p + nr] = lout[r][kk];
   }
   for (int j = tid; j < w; j += kThreads) {
     N.D[2 * (p + j)] = dinv[2 * j];
@@ -397,15 +409,18 @@ __global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lv
   if (tid < w) ptmp[tid] = N.perm[p + lp[tid]];
   __syncthreads();
   if (tid < w) N.perm[p + tid] = ptmp[tid];
-  for (int c = tid; c < p; c += kThreads) {
-    double *col = L + (size_t)c * ldl + p;
-    double t[kBlk];
-#pragma unroll
-    for (int i = 0; i < kBlk; ++i)
-      if (i < w) t[i] = col[lp[i]];
-#pragma unroll
-    for (int i = 0; i < kBlk; ++i)
-      if (i < w) col[i] = t[i];
+  // (64-column chunks staged through LDS, reusing lout)
+  for (int cb = 0; cb < p; cb += kBlk) {
+    __syncthreads();
+    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
+      int r = idx % kBlk, cc = idx / kBlk;
+      if (r < w && cb + cc < p) lout[r][cc] = L[(size_t)(cb + cc) * ldl + p + r];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
+      int r = idx % kBlk, cc = idx / kBlk;
+      if (r < w && cb + cc < p) L[(size_t)(cb + cc) * ldl + p + r] = lout[lp[r]][cc];
+    }
   }
 }
 

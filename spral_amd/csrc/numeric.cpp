@@ -40,6 +40,14 @@ void launch_fwd(hipStream_t, int, const SolveNode *, const int *, const int *, d
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int);
 
+void launch_residual(hipStream_t, int, const int64_t *, const int *, const int64_t *,
+                     const double *, const double *, const double *, double *, int, int);
+void launch_anorm(hipStream_t, int, const int64_t *, const int64_t *, const double *,
+                  unsigned long long *);
+void launch_colmax(hipStream_t, int, const double *, int, int, int, unsigned long long *);
+void launch_axpy(hipStream_t, int, int, int, const double *, double *, const int *);
+void launch_scale_rows(hipStream_t, int, int, int, const double *, double *);
+
 struct HipError : std::runtime_error {
   hipError_t code;
   explicit HipError(hipError_t c) : std::runtime_error(hipGetErrorString(c)), code(c) {}
@@ -153,6 +161,9 @@ struct Symbolic {
   // device
   int *d_rows = nullptr, *d_rmap = nullptr, *d_arow = nullptr, *d_child = nullptr;
   int64_t *d_asrc = nullptr, *d_acol = nullptr;
+  // full symmetric matrix in elimination order (CSR) for refinement residuals
+  int64_t *d_fptr = nullptr, *d_fsrc = nullptr;
+  int *d_fcol = nullptr;
   std::vector<void *> owned;
   ~Symbolic() {
     for (void *p : owned) (void)hipFree(p);
@@ -181,6 +192,14 @@ struct Numeric {
   int solve_nrhs_cap = 0;
   bool posdef = false;
   spral_ssids_hip_stats stats{};
+  double *d_aval = nullptr;               // copy of the factorized values
+  size_t d_aval_cap = 0;
+  double *d_scal = nullptr;               // scaling in elimination order (or null)
+  double *d_r = nullptr;                  // refinement workspace (2 x n x nrhs)
+  size_t d_r_cap = 0;
+  unsigned long long *d_norms = nullptr;  // 1 + 3*nrhs ordered-bit maxima
+  int *d_active = nullptr;
+  double anorm = -1.0;
   // contribution export
   std::vector<double> contrib_val, contrib_delay_val;
   std::vector<int> contrib_rlist, contrib_delay_perm;
@@ -195,6 +214,11 @@ struct Numeric {
     if (d_slots_all) (void)hipFree(d_slots_all);
     if (d_X) (void)hipFree(d_X);
     if (d_x) (void)hipFree(d_x);
+    if (d_aval) (void)hipFree(d_aval);
+    if (d_scal) (void)hipFree(d_scal);
+    if (d_r) (void)hipFree(d_r);
+    if (d_norms) (void)hipFree(d_norms);
+    if (d_active) (void)hipFree(d_active);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -319,6 +343,48 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   S->d_arow = dev_upload(arow, S->owned);
   S->d_acol = dev_upload(acol, S->owned);
   S->d_child = dev_upload(S->child_list, S->owned);
+  // full CSR (rows sorted by column) of the matrix in elimination order
+  {
+    std::vector<int64_t> cnt(n + 1, 0);
+    auto each = [&](auto fn) {
+      for (int i = 0; i < nn; ++i) {
+        const int *rows = &S->rows_h[S->row_off[i]];
+        int64_t a0 = acol[S->acol_off[i]];
+        for (int c = 0; c < S->sn[i]; ++c)
+          for (int64_t e = acol[S->acol_off[i] + c]; e < acol[S->acol_off[i] + c + 1]; ++e)
+            fn(rows[arow[e]], rows[c], asrc[e]);
+        (void)a0;
+      }
+    };
+    each([&](int r, int c, int64_t) {
+      cnt[r + 1]++;
+      if (r != c) cnt[c + 1]++;
+    });
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> fsrc(cnt[n]);
+    std::vector<int> fcol(cnt[n]);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    each([&](int r, int c, int64_t src) {
+      fcol[pos[r]] = c;
+      fsrc[pos[r]++] = src;
+      if (r != c) {
+        fcol[pos[c]] = r;
+        fsrc[pos[c]++] = src;
+      }
+    });
+    for (int i = 0; i < n; ++i) {
+      std::vector<std::pair<int, int64_t>> rowv;
+      for (int64_t e = cnt[i]; e < cnt[i + 1]; ++e) rowv.push_back({fcol[e], fsrc[e]});
+      std::sort(rowv.begin(), rowv.end());
+      for (int64_t e = cnt[i]; e < cnt[i + 1]; ++e) {
+        fcol[e] = rowv[e - cnt[i]].first;
+        fsrc[e] = rowv[e - cnt[i]].second;
+      }
+    }
+    S->d_fptr = dev_upload(cnt, S->owned);
+    S->d_fcol = dev_upload(fcol, S->owned);
+    S->d_fsrc = dev_upload(fsrc, S->owned);
+  }
   return S.release();
 }
 
@@ -336,8 +402,26 @@ static void ensure_lvl(Numeric &N, size_t ints) {
 static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
-                        const spral_ssids_hip_options &opt) {
+                        const spral_ssids_hip_options &opt, int64_t naval) {
   Symbolic &S = *N.symb;
+  // keep the values (and scaling) for refinement residuals
+  if ((size_t)naval > N.d_aval_cap) {
+    if (N.d_aval) HIPCHK(hipFree(N.d_aval));
+    HIPCHK(hipMalloc(&N.d_aval, std::max<int64_t>(naval, 1) * sizeof(double)));
+    N.d_aval_cap = naval;
+  }
+  if (naval > 0)
+    HIPCHK(hipMemcpyAsync(N.d_aval, d_aval, naval * sizeof(double), hipMemcpyDeviceToDevice,
+                          N.stream));
+  if (d_scaling) {
+    if (!N.d_scal) HIPCHK(hipMalloc(&N.d_scal, std::max(S.n, 1) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(N.d_scal, d_scaling, S.n * sizeof(double), hipMemcpyDeviceToDevice,
+                          N.stream));
+  } else if (N.d_scal) {
+    HIPCHK(hipFree(N.d_scal));
+    N.d_scal = nullptr;
+  }
+  N.anorm = -1.0;
   const int nn = S.nnodes;
   hipStream_t st = N.stream;
   FactorParams fp;
@@ -358,6 +442,13 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   std::vector<std::pair<void *, size_t>> contrib_of(nn);  // slot -> C buffer
   std::vector<std::pair<void *, size_t>> work_of(nn);
   int maxfront = 0;
+  std::vector<hipEvent_t> tev;  // pairs around Schur-update launches
+  auto timed_event = [](hipStream_t s) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(e, s));
+    return e;
+  };
   hipEvent_t ev0, ev1;
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
@@ -463,8 +554,10 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     for (int stp = 0; stp < maxsteps; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
       launch_apply(st, app_t.second, N.d_nodes, d_slots, off(app_t), nl, stp, fp);
+      if (opt.timing) tev.emplace_back(timed_event(st));
       launch_update(st, upd_t[stp].second, N.d_nodes, d_slots, off(upd_t[stp]), nl, stp, 0,
                     fp);
+      if (opt.timing) tev.emplace_back(timed_event(st));
       if (multi) {
         launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 0);
         launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 1);
@@ -472,7 +565,9 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     }
     launch_finalize(st, nl, N.d_nodes, d_slots);
     launch_tpp(st, nl, N.d_nodes, d_slots, fp);
+    if (opt.timing) tev.emplace_back(timed_event(st));
     launch_update(st, tpp_t.second, N.d_nodes, d_slots, off(tpp_t), nl, 0, 1, fp);
+    if (opt.timing) tev.emplace_back(timed_event(st));
     launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
     HIPCHK(hipGetLastError());
     // read back the level's results (nelim, flag)
@@ -545,6 +640,38 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   o.maxsupernode = maxsn;
   o.num_levels = nlev;
   o.factor_ms = ms;
+  o.pool_peak_bytes = (int64_t)N.pool.peak();
+  o.update_ms = 0.0;
+  o.update_launches = 0;
+  for (size_t k = 0; k + 1 < tev.size(); k += 2) {
+    float e = 0.f;
+    HIPCHK(hipEventElapsedTime(&e, tev[k], tev[k + 1]));
+    o.update_ms += e;
+    o.update_launches++;
+  }
+  for (auto e : tev) (void)hipEventDestroy(e);
+  // algorithmic flops of the Schur updates: a rank-K update of the lower
+  // triangle of an N x N trailing matrix costs K*N*(N+1) flops (2 per FMA).
+  // Blocks are assumed to pass in full (the count is exact without delays).
+  double uf = 0.0;
+  for (int sl = 0; sl < nn; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    if (fp.pivot_tpp) {
+      double K = D.nelim, Nn = D.m - D.n;
+      uf += K * Nn * (Nn + 1);
+      continue;
+    }
+    int p = 0;
+    for (int stp = 0; stp < D.nsteps && p < D.nelim_app; ++stp) {
+      double K = std::min(kBlk, D.nelim_app - p);
+      double Nn = D.m - (p + K);
+      uf += K * Nn * (Nn + 1);
+      p += (int)K;
+    }
+    double K2 = D.nelim - D.nelim_app, N2 = D.m - D.n;
+    uf += K2 * N2 * (N2 + 1);
+  }
+  o.update_flops = uf;
   // solve descriptors
   N.sn_h.assign(nn, SolveNode{});
   std::vector<int> doff(nn, 0);
@@ -646,6 +773,80 @@ static int solve_host(Numeric &N, int job, int nrhs, double *x, int ldx) {
   return 0;
 }
 
+// Solve A x = b (scaled factorization S A S when scaling was given) with FP64
+// iterative refinement against the original values: x0 = solve(b),
+// x_{k+1} = x_k + solve(b - A x_k) until the scaled backward error of every
+// column is <= tol or maxit corrections were made.  bwd[k] and *iters report
+// the final state.  All vectors are device, in elimination order, ld >= n.
+static void solve_scaled(Numeric &N, int nrhs, double *d_v, int ld, hipStream_t st) {
+  Symbolic &S = *N.symb;
+  if (N.d_scal) launch_scale_rows(st, S.n, nrhs, ld, N.d_scal, d_v);
+  solve_device(N, 0, nrhs, d_v, ld, st);
+  if (N.d_scal) launch_scale_rows(st, S.n, nrhs, ld, N.d_scal, d_v);
+}
+
+static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, int ld,
+                         int maxit, double tol, double *bwd, int *iters, hipStream_t st) {
+  Symbolic &S = *N.symb;
+  const int n = S.n;
+  size_t need = (size_t)ld * nrhs;
+  if (need > N.d_r_cap) {
+    if (N.d_r) HIPCHK(hipFree(N.d_r));
+    HIPCHK(hipMalloc(&N.d_r, std::max(need, size_t(1)) * sizeof(double)));
+    N.d_r_cap = need;
+    if (N.d_norms) HIPCHK(hipFree(N.d_norms));
+    if (N.d_active) HIPCHK(hipFree(N.d_active));
+    HIPCHK(hipMalloc(&N.d_norms, (1 + 3 * (size_t)nrhs) * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&N.d_active, nrhs * sizeof(int)));
+  }
+  std::vector<unsigned long long> nb(1 + 3 * (size_t)nrhs);
+  if (N.anorm < 0) {
+    HIPCHK(hipMemsetAsync(N.d_norms, 0, sizeof(unsigned long long), st));
+    launch_anorm(st, n, S.d_fptr, S.d_fsrc, N.d_aval, N.d_norms);
+    unsigned long long a = 0;
+    HIPCHK(hipMemcpyAsync(&a, N.d_norms, sizeof(a), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double v;
+    std::memcpy(&v, &a, sizeof(v));
+    N.anorm = v;
+  }
+  if (d_x != d_b)
+    HIPCHK(hipMemcpy2DAsync(d_x, ld * sizeof(double), d_b, ld * sizeof(double),
+                            n * sizeof(double), nrhs, hipMemcpyDeviceToDevice, st));
+  solve_scaled(N, nrhs, d_x, ld, st);
+  std::vector<int> active(nrhs, 1);
+  int it = 0;
+  for (;;) {
+    launch_residual(st, n, S.d_fptr, S.d_fcol, S.d_fsrc, N.d_aval, d_x, d_b, N.d_r, ld, nrhs);
+    HIPCHK(hipMemsetAsync(N.d_norms, 0, (1 + 3 * (size_t)nrhs) * sizeof(unsigned long long), st));
+    launch_colmax(st, n, N.d_r, ld, nrhs, 0, N.d_norms + 1);
+    launch_colmax(st, n, d_x, ld, nrhs, 1, N.d_norms + 1);
+    launch_colmax(st, n, d_b, ld, nrhs, 2, N.d_norms + 1);
+    HIPCHK(hipMemcpyAsync(nb.data(), N.d_norms, nb.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool all_ok = true;
+    for (int k = 0; k < nrhs; ++k) {
+      double rv, xv, bv;
+      std::memcpy(&rv, &nb[1 + 3 * k], 8);
+      std::memcpy(&xv, &nb[1 + 3 * k + 1], 8);
+      std::memcpy(&bv, &nb[1 + 3 * k + 2], 8);
+      double den = N.anorm * xv + bv;
+      bwd[k] = den > 0 ? rv / den : 0.0;
+      active[k] = bwd[k] > tol ? 1 : 0;
+      if (active[k]) all_ok = false;
+    }
+    if (all_ok || it >= maxit) break;
+    HIPCHK(hipMemcpyAsync(N.d_active, active.data(), nrhs * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    solve_scaled(N, nrhs, N.d_r, ld, st);
+    launch_axpy(st, n, nrhs, ld, N.d_r, d_x, N.d_active);
+    ++it;
+  }
+  *iters = it;
+  return 0;
+}
+
 }  // namespace ssids_hip
 
 using namespace ssids_hip;
@@ -709,7 +910,7 @@ void *spral_ssids_hip_factor_device(bool posdef, void *symbolic, const double *d
   try {
     HIPCHK(hipSetDevice(S->device));
     HIPCHK(hipStreamCreate(&N->stream));
-    factor_impl(*N, d_aval, d_scaling, opt);
+    factor_impl(*N, d_aval, d_scaling, opt, naval);
     *stats = N->stats;
     if (stats->flag < 0) {
       delete N;
@@ -761,7 +962,7 @@ int spral_ssids_hip_refactor_device(void *numeric, const double *d_aval,
   try {
     HIPCHK(hipSetDevice(N->symb->device));
     N->stats = spral_ssids_hip_stats{};
-    factor_impl(*N, d_aval, d_scaling, opt);
+    factor_impl(*N, d_aval, d_scaling, opt, (int64_t)N->d_aval_cap);
     *stats = N->stats;
     return stats->flag;
   } catch (HipError &e) {
@@ -812,6 +1013,45 @@ int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x, 
   } catch (HipError &) {
     return -51;
   }
+}
+
+int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,
+                                        double *d_x, int ldx, int maxit, double tol,
+                                        double *bwd, int *iters, void *stream) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  if (nrhs < 1 || ldx < N->symb->n) return -10;
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    hipStream_t st = stream ? (hipStream_t)stream : N->stream;
+    return refine_device(*N, nrhs, d_b, d_x, ldx, maxit, tol, bwd, iters, st);
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
+int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, double *x, int ldx,
+                                 int maxit, double tol, double *bwd, int *iters) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  const int n = N->symb->n;
+  if (nrhs < 1 || ldx < n) return -10;
+  double *db = nullptr, *dx = nullptr;
+  int rc = 0;
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    HIPCHK(hipMalloc(&db, (size_t)n * nrhs * sizeof(double)));
+    HIPCHK(hipMalloc(&dx, (size_t)n * nrhs * sizeof(double)));
+    HIPCHK(hipMemcpy2D(db, n * sizeof(double), b, ldx * sizeof(double), n * sizeof(double),
+                       nrhs, hipMemcpyHostToDevice));
+    rc = refine_device(*N, nrhs, db, dx, n, maxit, tol, bwd, iters, N->stream);
+    HIPCHK(hipMemcpy2DAsync(x, ldx * sizeof(double), dx, n * sizeof(double),
+                            n * sizeof(double), nrhs, hipMemcpyDeviceToHost, N->stream));
+    HIPCHK(hipStreamSynchronize(N->stream));
+  } catch (HipError &) {
+    rc = -51;
+  }
+  if (db) (void)hipFree(db);
+  if (dx) (void)hipFree(dx);
+  return rc;
 }
 
 int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d) {

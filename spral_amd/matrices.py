@@ -77,48 +77,47 @@ def kkt_laplace(grid=16, shift=-0.5, m_con=1024, k=3, seed=SEED_BASE + 1) -> Low
     return LowerCSC(kk)
 
 
+def _lower_from_coo(n, r, c, v):
+    """LowerCSC from lower-triangle coordinates, explicit zeros kept."""
+    low = sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsc()
+    low.sum_duplicates()
+    low.sort_indices()
+    out = LowerCSC.__new__(LowerCSC)
+    out.n = n
+    out.ptr = low.indptr.astype(np.int64) + 1
+    out.row = low.indices.astype(np.int32) + 1
+    out.val = low.data.astype(np.float64)
+    out.full = sp.csc_matrix(low + sp.tril(low, k=-1).T)
+    return out
+
+
 def random_indef(n, density=0.05, seed=0, zero_diag_frac=0.2, big_last=True) -> LowerCSC:
-    """Random symmetric indefinite matrix with some zero diagonals and, as in
-    the reference's generator, the last entry of each column scaled up to
-    provoke delayed pivots."""
+    """Random symmetric indefinite matrix: U(-1,1) strictly-lower entries, a
+    structurally full diagonal of which a fraction is zero-valued, and (as the
+    reference generator does, tests/ssids/ssids.f90:2830-2871) the last entry
+    of each column scaled by 1000 to provoke delayed pivots."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nnz = max(n, int(density * n * n / 2))
     r = rng.integers(0, n, nnz)
     c = rng.integers(0, n, nnz)
-    lo, hi = np.minimum(r, c), np.maximum(r, c)
-    v = rng.uniform(-1, 1, nnz)
-    a = sp.coo_matrix((v, (hi, lo)), shape=(n, n)).tocsc()
-    a.sum_duplicates()
+    keep = r != c
+    lo, hi = np.minimum(r, c)[keep], np.maximum(r, c)[keep]
+    key = np.unique(hi.astype(np.int64) * n + lo)
+    hi, lo = key // n, key % n
+    v = rng.uniform(-1, 1, key.size)
+    if big_last:
+        # last (largest-row) entry of each column
+        order = np.lexsort((hi, lo))
+        hi, lo, v = hi[order], lo[order], v[order]
+        last = np.ones(key.size, bool)
+        last[:-1] = lo[1:] != lo[:-1]
+        v[last] *= 1000.0
     d = rng.uniform(-1, 1, n)
     d[rng.random(n) < zero_diag_frac] = 0.0
-    a = sp.tril(a, k=-1, format="csc")
-    if big_last:
-        for j in range(n):
-            s, e = a.indptr[j], a.indptr[j + 1]
-            if e > s:
-                a.data[e - 1] *= 1000.0
-    full = a + a.T + sp.diags(d)
-    full = sp.csc_matrix(full)
-    # keep explicit zero diagonals in the pattern
-    full = full + sp.diags(np.where(d == 0.0, 0.0, 0.0), format="csc")
-    pat = sp.csc_matrix((np.ones(n), (np.arange(n), np.arange(n))), shape=(n, n))
-    full = sp.csc_matrix(full + 0.0 * pat)
-    low = sp.tril(full, format="csc")
-    low.sort_indices()
-    out = LowerCSC.__new__(LowerCSC)
-    out.n = n
-    # ensure every diagonal position exists structurally
-    low = sp.csc_matrix(low)
-    low.eliminate_zeros()
-    low = low + sp.csc_matrix((np.full(n, 1e-300), (np.arange(n), np.arange(n))), shape=(n, n))
-    low.data[np.abs(low.data) == 1e-300] = 0.0
-    low.sort_indices()
-    out.ptr = low.indptr.astype(np.int64) + 1
-    out.row = low.indices.astype(np.int32) + 1
-    out.val = low.data.astype(np.float64)
-    f = low + sp.tril(low, k=-1).T
-    out.full = sp.csc_matrix(f)
-    return out
+    rr = np.concatenate([hi, np.arange(n)])
+    cc = np.concatenate([lo, np.arange(n)])
+    vv = np.concatenate([v, d])
+    return _lower_from_coo(n, rr, cc, vv)
 
 
 def random_posdef(n, density=0.05, seed=0) -> LowerCSC:

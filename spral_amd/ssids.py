@@ -296,6 +296,38 @@ def ssids_solve(x, akeep, fkeep, options=None, job=None):
     return inf
 
 
+def ssids_solve_refine(x, akeep, fkeep, options=None, maxit=5, tol=1e-14):
+    """Solve with FP64 iterative refinement on the device (no counterpart in
+    the reference).  x (n,) or (n, nrhs) holds b on entry and the solution on
+    exit.  Returns (inform, bwd per RHS, refinement steps)."""
+    L = _lib.lib()
+    inf = SSIDSInform()
+    if fkeep is None or not fkeep.numeric or fkeep.inform.flag < 0:
+        inf.flag = SSIDS_ERROR_CALL_SEQUENCE
+        return inf, None, 0
+    inf.flag = max(SSIDS_SUCCESS, fkeep.inform.flag)
+    xa = np.asarray(x)
+    one = xa.ndim == 1
+    X = xa.reshape(akeep.n, -1)
+    n, nrhs = akeep.n, X.shape[1]
+    b2 = np.ascontiguousarray(np.asarray(X[akeep.invp, :], np.float64).T)
+    x2 = np.zeros_like(b2)
+    bwd = np.zeros(nrhs)
+    it = C.c_int(0)
+    flag = L.spral_ssids_hip_solve_refine(fkeep.numeric, nrhs, b2.reshape(-1), x2.reshape(-1),
+                                          n, maxit, tol, bwd, C.byref(it))
+    if flag < 0:
+        inf.flag = flag
+        return inf, None, 0
+    out = np.empty((n, nrhs))
+    out[akeep.invp, :] = x2.T
+    if one:
+        xa[:] = out[:, 0]
+    else:
+        X[:, :] = out
+    return inf, bwd, it.value
+
+
 def _solve_all(L, num, nrhs, x2c, n):
     flag = L.spral_ssids_hip_solve_fwd(num, nrhs, x2c.reshape(-1), n)
     if flag < 0:
