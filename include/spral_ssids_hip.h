@@ -44,7 +44,7 @@ struct spral_ssids_hip_options {
   int schur_precision;     /* 0 fp64, 1 fp32, 2 fp16 trailing updates       */
   int refine_max;          /* max FP64 refinement steps in solve_refine     */
   double refine_tol;       /* target scaled backward error                  */
-  int timing;              /* 1: time the Schur-update launches with events */
+  int timing;              /* 1: time the gather/Schur launches with events */
 };
 
 /* Interoperable subset of ssids_inform, as cpu_factor_stats
@@ -64,9 +64,10 @@ struct spral_ssids_hip_stats {
   int hip_error;           /* hipError_t of the first failing call, else 0  */
   int num_levels;          /* tree levels walked                            */
   double factor_ms;        /* device time of the numeric factorization      */
-  double update_ms;        /* summed device time of Schur-update launches   */
-  double update_flops;     /* algorithmic flops of those launches           */
-  int update_launches;     /* number of Schur-update launches timed         */
+  double kernel_ms;        /* summed device time of the gather launches     */
+  double kernel_flops;     /* algorithmic flops of those launches           */
+  double kernel_bytes;     /* algorithmic HBM bytes of those launches       */
+  int kernel_launches;     /* number of gather launches timed               */
   int64_t pool_peak_bytes; /* peak device pool use                          */
 };
 

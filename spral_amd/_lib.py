@@ -49,9 +49,10 @@ class Stats(C.Structure):
         ("hip_error", C.c_int),
         ("num_levels", C.c_int),
         ("factor_ms", C.c_double),
-        ("update_ms", C.c_double),
-        ("update_flops", C.c_double),
-        ("update_launches", C.c_int),
+        ("kernel_ms", C.c_double),
+        ("kernel_flops", C.c_double),
+        ("kernel_bytes", C.c_double),
+        ("kernel_launches", C.c_int),
         ("pool_peak_bytes", C.c_int64),
     ]
 

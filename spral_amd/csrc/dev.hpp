@@ -22,6 +22,7 @@ struct NodeDev {
   double *L;
   double *C;
   double *D;        // 2*n, D^{-1} as [d11, d21, (inf for 2x2 second), d22]
+  double *Dd;       // 3*n: per pivot (D_kk, D_k,partner, partner offset 0/+1/-1)
   int *perm;        // n fully-summed variables (0-based elimination index)
   int *perm0;       // the same list before pivoting (as assembled)
   int *ipos;        // pre-pivot -> post-pivot position (filled after factor)
@@ -31,6 +32,9 @@ struct NodeDev {
   double *bak;      // (m x kBlk) backup of the panel rows below the block
   double *dbak;     // kBlk x kBlk backup of the diagonal block
   double *swp;      // 2*kBlk x m gather buffer for the failed-column swap
+  int *pmap;        // (m - nelim) parent positions of the trailing indices
+  int *ctab;        // per child (stride T+1, T = 64-row tiles of this front):
+                    // tile t -> first child trailing index mapping to >= 64t
   int sm, sn, nd, m, n, ldl, ldc, ldbak;
   int parent;       // slot of parent or -1
   int cbeg, cend;   // children slots range in the child list
@@ -56,6 +60,7 @@ struct SolveNode {
   const int *rows;   // symbolic rows
   const int *rmap;
   double *X;         // m x nrhs workspace (ld m)
+  double *P;         // backward GEMV partial sums (gemv workgroups x nrhs x nelim)
   int m, n, nd, sn, nelim, ldl, doff, cbeg, cend;
 };
 

@@ -4,10 +4,10 @@
 // front of the level.  Workgroups find their front by binary search in a
 // per-level prefix array of workgroup counts (wg_off).
 //
-//  k_assemble   owner-computes extend-add: a workgroup owns 32 columns of one
-//               parent front, zeroes them, scatters A, then adds each child's
-//               trailing matrix (children in order => deterministic sums)
-//               through an LDS-staged child->parent row map.
+//  k_gather     gather assembly: a workgroup builds a 64x64 tile of one parent
+//               front in LDS from its children (children in order =>
+//               deterministic sums) through LDS-staged child->parent maps,
+//               computing the children's Schur complements on MFMA on the way.
 //  k_diag       one workgroup per front: factor the w<=64 column diagonal
 //               block with complete pivoting (1x1/2x2) in LDS
 //               (pivot rules of src/ssids/cpu/kernels/block_ldlt.hxx:213-414).
@@ -15,8 +15,8 @@
 //               backup and the column max |l| (a posteriori test,
 //               src/ssids/cpu/kernels/ldlt_app.cxx:304-414).
 //  k_update     restore failed columns and apply the accepted pivots to the
-//               trailing lower matrix (panel and contribution block) with
-//               v_mfma_f64_16x16x4_f64 on 64x64 tiles.
+//               trailing panel columns with v_mfma_f64_16x16x4_f64 on 64x64
+//               tiles (the contribution block is updated lazily by k_gather).
 //  k_swap_*     move failed columns behind the unprocessed candidates.
 //  k_tpp        threshold partial pivoting on the failed columns
 //               (src/ssids/cpu/kernels/ldlt_tpp.cxx:166-270).
@@ -59,369 +59,212 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Assembly
+// Diagonal block factorization, one wavefront per front, block resident in
+// LDS.  Lane j updates column j of the symmetric w x w block; pivots are
+// tracked in a wave-uniform bit mask (no physical swaps), only live rows are
+// updated, and the L column of an eliminated pivot is stored in that pivot's
+// (now dead) column.  The elimination order lp[] permutes the block on write
+// back.  Pivot choice: complete pivoting over the live block with the rules
+// of src/ssids/cpu/kernels/block_ldlt.hxx:289-414 (max entry on the diagonal:
+// 1x1; else the 2x2 test, else the larger diagonal as 1x1); the a posteriori
+// test of k_apply then checks |l| <= 1/u over the full column.
 // ---------------------------------------------------------------------------
-constexpr int kAsmCols = 32;
-constexpr int kAsmMap = 8192;  // LDS entries for the staged child row map
-
-__global__ __launch_bounds__(kThreads) void k_assemble(
-    NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
-    const int *child_list, const double *aval, const int64_t *asrc,
-    const int *arow, const double *scaling) {
-  __shared__ int smap[kAsmMap];
-  const int li = find_node(wg_off, nlvl, blockIdx.x);
-  const int s = lvl_slots[li];
-  NodeDev &P = nodes[s];
-  const int m = P.m, n = P.n, nd = P.nd;
-  const int J0 = (blockIdx.x - wg_off[li]) * kAsmCols;
-  if (J0 >= m) return;
-  const int J1 = min(J0 + kAsmCols, m);
-  const int tid = threadIdx.x;
-
-  // fully-summed variable list for owned columns
-  for (int j = J0 + tid; j < min(J1, n); j += kThreads) {
-    if (j >= nd) {
-      P.perm[j] = P.rows[j - nd];
-      P.perm0[j] = P.rows[j - nd];
-    } else {
-      int doff = 0;
-      for (int ci = P.cbeg; ci < P.cend; ++ci) {
-        const NodeDev &c = nodes[child_list[ci]];
-        int nf = c.n - c.nelim;
-        if (j >= doff && j < doff + nf) {
-          P.perm[j] = c.perm[c.nelim + (j - doff)];
-          P.perm0[j] = P.perm[j];
-          break;
-        }
-        doff += nf;
-      }
-    }
-  }
-  // zero the owned columns (lower part)
-  for (int j = J0; j < J1; ++j) {
-    if (j < n) {
-      double *col = P.L + (size_t)j * P.ldl;
-      for (int i = j + tid; i < m; i += kThreads) col[i] = 0.0;
-    } else {
-      double *col = P.C + (size_t)(j - n) * P.ldc - n;
-      for (int i = j + tid; i < m; i += kThreads) col[i] = 0.0;
-    }
-  }
-  __syncthreads();
-  // A entries of owned fully-summed own columns
-  {
-    int c0 = max(J0, nd) - nd, c1 = min(J1, n) - nd;
-    if (c1 > c0) {
-      int64_t e0 = P.acol[c0], e1 = P.acol[c1];
-      for (int64_t e = e0 + tid; e < e1; e += kThreads) {
-        int r = arow[e];
-        // column of entry e: search acol
-        int lo = c0, hi = c1;
-        while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (P.acol[mid] <= e) lo = mid;
-          else hi = mid;
-        }
-        int c = lo;
-        double v = aval[asrc[e]];
-        if (scaling) v *= scaling[P.rows[r]] * scaling[P.rows[c]];
-        P.L[(size_t)(c + nd) * P.ldl + (r + nd)] = v;
-      }
-    }
-  }
-  __syncthreads();
-  // children
-  int doff = 0;
-  for (int ci = P.cbeg; ci < P.cend; ++ci) {
-    const NodeDev &c = nodes[child_list[ci]];
-    const int kb0 = c.nelim, ke0 = c.m;
-    const int cdoff = doff;
-    doff += c.n - c.nelim;
-    if (ke0 <= kb0) continue;
-    // first trailing index with pos >= J0, and >= J1
-    int lo = kb0, hi = ke0;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (child_pos(c, cdoff, nd, mid) < J0) lo = mid + 1;
-      else hi = mid;
-    }
-    const int kb = lo;
-    hi = ke0;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (child_pos(c, cdoff, nd, mid) < J1) lo = mid + 1;
-      else hi = mid;
-    }
-    const int ke = lo;
-    if (ke <= kb) continue;
-    const int nmap = ke0 - kb;
-    const bool staged = nmap <= kAsmMap;
-    if (staged) {
-      for (int t = tid; t < nmap; t += kThreads) smap[t] = child_pos(c, cdoff, nd, kb + t);
-      __syncthreads();
-    }
-    for (int k = kb; k < ke; ++k) {
-      const int pc = staged ? smap[k - kb] : child_pos(c, cdoff, nd, k);
-      const double *src = child_src(c, k, k);  // column k from row k
-      double *dst = (pc < n) ? P.L + (size_t)pc * P.ldl
-                             : P.C + (size_t)(pc - n) * P.ldc - n;
-      for (int kr = k + tid; kr < ke0; kr += kThreads) {
-        int pr = staged ? smap[kr - kb] : child_pos(c, cdoff, nd, kr);
-        dst[pr] += src[kr - k];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Diagonal block factorization (complete pivoting inside the block).
-// Each thread keeps 16 entries of the symmetric w x w block in registers:
-// row i = tid % 64, columns j = tid/64 + 4q (q < 16).  Pivots are not swapped
-// physically: eliminated indices are flagged, the pivot column is broadcast
-// through LDS, and the elimination order lp[] permutes rows/columns when the
-// block is written back.  Pivot rules follow
-// src/ssids/cpu/kernels/block_ldlt.hxx:213-414.
-// ---------------------------------------------------------------------------
-struct MaxLoc {
-  double v;   // |a|
-  double s;   // signed a
-  int r, c;
-};
-
-__device__ inline bool better(const MaxLoc &a, const MaxLoc &b) {
-  // larger value first; ties: smaller column, then smaller row
-  if (a.v != b.v) return a.v > b.v;
-  if (a.c != b.c) return a.c < b.c;
-  return a.r < b.r;
-}
-
-__global__ __launch_bounds__(kThreads) void k_diag(NodeDev *nodes, const int *lvl_slots,
-                                                   int step, FactorParams fp) {
-  __shared__ double col1[kBlk], col2[kBlk], diag[kBlk];
-  __shared__ double lout[kBlk][kBlk + 1];  // L column k (elimination order), row = original
+__global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slots, int step,
+                                             FactorParams fp) {
+  __shared__ double t[kBlk][kBlk + 1];   // t[row][col], symmetric
+  __shared__ double pc1[kBlk], pc2[kBlk];
   __shared__ double dinv[2 * kBlk + 2];
-  __shared__ int lp[kBlk], inv_lp[kBlk], elim[kBlk];
-  __shared__ MaxLoc red[kThreads / 64];
-  __shared__ int sh_p, sh_w, sh_piv1, sh_piv2, sh_size;
-  __shared__ double sh_d[3];
+  __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
   const int s = lvl_slots[blockIdx.x];
   NodeDev &N = nodes[s];
   if (step >= N.nsteps) return;
-  const int tid = threadIdx.x, lane = tid & 63;
-  if (tid == 0) {
-    int p, q;
-    if (step == 0) {
-      p = 0;
-      q = N.n;
-    } else {
-      p = N.p + N.npass;
-      q = N.q - (N.w - N.npass);
-    }
-    int w = min(kBlk, q - p);
-    if (w < 0) w = 0;
+  const int lane = threadIdx.x;
+  int p, q;
+  if (step == 0) {
+    p = 0;
+    q = N.n;
+  } else {
+    p = N.p + N.npass;
+    q = N.q - (N.w - N.npass);
+  }
+  const int w = max(0, min(kBlk, q - p));
+  __syncthreads();  // all lanes have read N.p/N.npass before lane 0 updates them
+  if (lane == 0) {
     N.p = p;
     N.q = q;
     N.w = w;
     N.npass = 0;
-    sh_p = p;
-    sh_w = w;
   }
-  __syncthreads();
-  const int p = sh_p, w = sh_w;
   if (w == 0) return;
   const int ldl = N.ldl;
   double *L = N.L;
-  const int ri = tid & 63, c0 = tid >> 6;  // row and first column owned
-  double v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    int cj = c0 + 4 * q;
-    double x = 0.0;
-    if (ri < w && cj < w) {
-      int hi = max(ri, cj), lo = min(ri, cj);
-      x = L[(size_t)(p + lo) * ldl + p + hi];
-      N.dbak[ri + cj * kBlk] = x;
+  for (int idx = lane; idx < w * w; idx += 64) {
+    int r = idx % w, c = idx / w;
+    if (r >= c) {
+      double v = L[(size_t)(p + c) * ldl + p + r];
+      t[r][c] = v;
+      t[c][r] = v;
+      N.dbak[r + c * kBlk] = v;
+      N.dbak[c + r * kBlk] = v;
     }
-    v[q] = x;
-    if (ri == cj && ri < w) diag[ri] = x;
   }
-  for (int i = tid; i < kBlk; i += kThreads) {
-    elim[i] = (i < w) ? 0 : 1;
-    N.colmax[i] = 0ull;
-  }
+  N.colmax[lane] = 0ull;
   __syncthreads();
+  double dg = (lane < w) ? t[lane][lane] : 0.0;  // a(lane, lane), kept current
+  unsigned long long elim = (w < 64) ? (~0ull << w) : 0ull;  // eliminated / absent
   int k = 0;
   while (k < w) {
-    // ---- search the remaining lower triangle
-    MaxLoc best{-1.0, 0.0, w, w};
-    if (ri < w && !elim[ri]) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int cj = c0 + 4 * q;
-        if (cj <= ri && !elim[cj]) {
-          MaxLoc cand{fabs(v[q]), v[q], ri, cj};
-          if (better(cand, best)) best = cand;
+    // ---- complete pivoting: max |a(i,j)|, i >= j, over the live block
+    const bool live = !((elim >> lane) & 1ull);
+    double bv = -1.0, bs = 0.0;
+    int br = 64;
+    if (live) {
+      for (int i = lane; i < w; ++i) {
+        if ((elim >> i) & 1ull) continue;
+        double v = t[i][lane];
+        if (fabs(v) > bv) {  // rows ascend: first maximal row kept
+          bv = fabs(v);
+          bs = v;
+          br = i;
         }
       }
     }
+    int bc = live ? lane : 64;
     for (int off = 32; off > 0; off >>= 1) {
-      MaxLoc o;
-      o.v = __shfl_xor(best.v, off);
-      o.s = __shfl_xor(best.s, off);
-      o.r = __shfl_xor(best.r, off);
-      o.c = __shfl_xor(best.c, off);
-      if (better(o, best)) best = o;
-    }
-    if (lane == 0) red[tid >> 6] = best;
-    __syncthreads();
-    if (tid == 0) {
-      MaxLoc b = red[0];
-      for (int i = 1; i < kThreads / 64; ++i)
-        if (better(red[i], b)) b = red[i];
-      if (!(b.v >= fp.small)) {
-        sh_size = 0;  // everything left is (near) zero
-      } else if (b.r == b.c) {
-        sh_size = 1;
-        sh_piv1 = b.r;
-      } else {
-        int mcol = b.c, trow = b.r;
-        double a11 = diag[mcol], a22 = diag[trow], a21 = b.s;
-        double detscale = 1.0 / fabs(a21);
-        double detpiv = (a11 * detscale) * a22 - fabs(a21);
-        if (fabs(detpiv) >= fabs(a21) / 2) {
-          sh_size = 2;
-          sh_piv1 = mcol;
-          sh_piv2 = trow;
-          sh_d[0] = (a22 * detscale) / detpiv;
-          sh_d[1] = (-a21 * detscale) / detpiv;
-          sh_d[2] = (a11 * detscale) / detpiv;
-        } else {
-          sh_size = 1;
-          sh_piv1 = (fabs(a11) > fabs(a22)) ? mcol : trow;
-        }
+      double ov = __shfl_xor(bv, off), os = __shfl_xor(bs, off);
+      int orr = __shfl_xor(br, off), oc = __shfl_xor(bc, off);
+      bool take = (ov > bv) || (ov == bv && (oc < bc || (oc == bc && orr < br)));
+      if (take) {
+        bv = ov;
+        bs = os;
+        br = orr;
+        bc = oc;
       }
     }
-    __syncthreads();
-    const int size = sh_size;
+    int size, p1, p2 = -1;
+    double d11 = 0.0, d21 = 0.0, d22 = 0.0;
+    double x = 0.0, y = 0.0;
+    if (!(bv >= fp.small)) {
+      size = 0;  // everything left is (near) zero
+      p1 = __ffsll((long long)~elim) - 1;
+    } else if (br == bc) {
+      size = 1;
+      p1 = br;
+    } else {
+      // 2x2 test of src/ssids/cpu/kernels/block_ldlt.hxx:213-218 on (bc, br)
+      const double a11 = __shfl(dg, bc), a22 = __shfl(dg, br), a21 = bs;
+      const double detscale = 1.0 / fabs(a21);
+      const double detpiv = (a11 * detscale) * a22 - fabs(a21);
+      if (fabs(detpiv) >= fabs(a21) / 2) {
+        size = 2;
+        p1 = bc;
+        p2 = br;
+        d11 = (a22 * detscale) / detpiv;
+        d21 = (-a21 * detscale) / detpiv;
+        d22 = (a11 * detscale) / detpiv;
+      } else {
+        size = 1;
+        p1 = (fabs(a11) > fabs(a22)) ? bc : br;
+      }
+    }
+    x = t[lane][p1];
+    if (size == 2) y = t[lane][p2];
     if (size == 0) {
-      // zero pivots for all remaining indices, in increasing index order
-      if (tid == 0) {
-        int kk = k;
-        for (int i = 0; i < w; ++i)
-          if (!elim[i]) {
-            lp[kk] = i;
-            dinv[2 * kk] = 0.0;
-            dinv[2 * kk + 1] = 0.0;
-            ++kk;
-          }
+      // zero pivots for every remaining index, in increasing index order
+      int kk = k;
+      for (int i = 0; i < w; ++i) {
+        if ((elim >> i) & 1ull) continue;
+        if (live) t[lane][i] = 0.0;  // zero L column of pivot i
+        if (lane == 0) {
+          lp[kk] = i;
+          kind[kk] = 0;
+          dinv[2 * kk] = 0.0;
+          dinv[2 * kk + 1] = 0.0;
+        }
+        ++kk;
+      }
+      if (lane == 0) {
         N.nzero += w - k;
         if (!fp.action) N.flag = -5;
-      }
-      for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
-        int r = idx % kBlk, kk = idx / kBlk;
-        if (kk >= k) lout[r][kk] = 0.0;
       }
       __syncthreads();
       k = w;
       break;
     }
-    const int p1 = sh_piv1, p2 = (size == 2) ? sh_piv2 : -1;
-    // ---- broadcast pivot column(s): owners of column p1 / p2 write them
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      int cj = c0 + 4 * q;
-      if (cj == p1 && ri < w) col1[ri] = v[q];
-      if (cj == p2 && ri < w) col2[ri] = v[q];
-    }
+    const double c1 = x;
+    const double c2 = (size == 2) ? y : 0.0;
+    pc1[lane] = c1;
+    pc2[lane] = c2;
     __syncthreads();
     if (size == 1) {
-      const double di = 1.0 / col1[p1];
-      // L column for every non-eliminated row other than the pivot
-      if (tid < w) lout[tid][k] = (tid == p1) ? 1.0 : (elim[tid] ? 0.0 : col1[tid] * di);
-      const double li = (ri < w) ? col1[ri] * di : 0.0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int cj = c0 + 4 * q;
-        if (cj < w) v[q] -= li * col1[cj];
+      const double di = 1.0 / __shfl(dg, p1);
+      elim |= 1ull << p1;
+      const bool upd = !((elim >> lane) & 1ull);  // live and not the pivot
+      const double lj = c1 * di;                 // L(lane, k)
+      if (upd) {
+        for (int i = 0; i < w; ++i)
+          if (!((elim >> i) & 1ull)) t[i][lane] -= pc1[i] * lj;
+        dg -= c1 * lj;
       }
-      if (tid == 0) {
+      __syncthreads();
+      if (upd) t[lane][p1] = lj;  // L column stored in the dead column p1
+      if (lane == 0) {
         dinv[2 * k] = di;
         dinv[2 * k + 1] = 0.0;
         lp[k] = p1;
-        elim[p1] = 1;
+        kind[k] = 1;
       }
     } else {
-      const double d11 = sh_d[0], d21 = sh_d[1], d22 = sh_d[2];
-      if (tid < w) {
-        double x1 = col1[tid], x2 = col2[tid];
-        bool pv = (tid == p1 || tid == p2), el = elim[tid];
-        lout[tid][k] = (tid == p1) ? 1.0 : ((pv || el) ? 0.0 : d11 * x1 + d21 * x2);
-        lout[tid][k + 1] = (tid == p2) ? 1.0 : ((pv || el) ? 0.0 : d21 * x1 + d22 * x2);
+      elim |= (1ull << p1) | (1ull << p2);
+      const bool upd = !((elim >> lane) & 1ull);
+      const double l1 = d11 * c1 + d21 * c2, l2 = d21 * c1 + d22 * c2;
+      if (upd) {
+        for (int i = 0; i < w; ++i)
+          if (!((elim >> i) & 1ull)) t[i][lane] -= pc1[i] * l1 + pc2[i] * l2;
+        dg -= c1 * l1 + c2 * l2;
       }
-      double l1 = 0.0, l2 = 0.0;
-      if (ri < w) {
-        double x1 = col1[ri], x2 = col2[ri];
-        l1 = d11 * x1 + d21 * x2;
-        l2 = d21 * x1 + d22 * x2;
+      __syncthreads();
+      if (upd) {
+        t[lane][p1] = l1;
+        t[lane][p2] = l2;
       }
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        int cj = c0 + 4 * q;
-        if (cj < w) v[q] -= l1 * col1[cj] + l2 * col2[cj];
-      }
-      if (tid == 0) {
+      if (lane == 0) {
         dinv[2 * k] = d11;
         dinv[2 * k + 1] = d21;
         dinv[2 * k + 2] = INFINITY;
         dinv[2 * k + 3] = d22;
         lp[k] = p1;
         lp[k + 1] = p2;
-        elim[p1] = 1;
-        elim[p2] = 1;
+        kind[k] = 2;
+        kind[k + 1] = 3;
       }
-    }
-    // refresh the diagonal cache for the next search
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      int cj = c0 + 4 * q;
-      if (cj == ri && ri < w) diag[ri] = v[q];
     }
     __syncthreads();
     k += size;
   }
-  // ---- write back L_bb (rows permuted into elimination order), D^{-1}, lp
-  for (int i = tid; i < w; i += kThreads) inv_lp[lp[i]] = i;
   __syncthreads();
-  for (int idx = tid; idx < w * w; idx += kThreads) {
+  // ---- write back L_bb (rows permuted into elimination order), D^{-1}, lp
+  for (int i = lane; i < w; i += 64) inv_lp[lp[i]] = i;
+  __syncthreads();
+  for (int idx = lane; idx < w * w; idx += 64) {
     int r = idx % w, kk = idx / w;  // original row r, elimination column kk
     int nr = inv_lp[r];
-    if (nr > kk) L[(size_t)(p + kk) * ldl + p + nr] = lout[r][kk];
-    else if (nr == kk) L[(size_t)(p + kk) * ldl + p + nr] = lout[r][kk];
+    if (nr < kk) continue;
+    double v;
+    if (nr == kk) v = (kind[kk] == 0) ? 0.0 : 1.0;
+    else if (nr == kk + 1 && kind[kk] == 2) v = 0.0;
+    else v = t[r][lp[kk]];
+    L[(size_t)(p + kk) * ldl + p + nr] = v;
   }
-  for (int j = tid; j < w; j += kThreads) {
+  for (int j = lane; j < w; j += 64) {
     N.D[2 * (p + j)] = dinv[2 * j];
     N.D[2 * (p + j) + 1] = dinv[2 * j + 1];
     N.lp[j] = lp[j];
   }
-  // permute the variable list and the rows of the eliminated columns left of p
-  __shared__ int ptmp[kBlk];
-  if (tid < w) ptmp[tid] = N.perm[p + lp[tid]];
+  // permute the variable list (rows of earlier eliminated columns: k_apply)
+  int pv = (lane < w) ? N.perm[p + lp[lane]] : 0;
   __syncthreads();
-  if (tid < w) N.perm[p + tid] = ptmp[tid];
-  // (64-column chunks staged through LDS, reusing lout)
-  for (int cb = 0; cb < p; cb += kBlk) {
-    __syncthreads();
-    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
-      int r = idx % kBlk, cc = idx / kBlk;
-      if (r < w && cb + cc < p) lout[r][cc] = L[(size_t)(cb + cc) * ldl + p + r];
-    }
-    __syncthreads();
-    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
-      int r = idx % kBlk, cc = idx / kBlk;
-      if (r < w && cb + cc < p) L[(size_t)(cb + cc) * ldl + p + r] = lout[lp[r]][cc];
-    }
-  }
+  if (lane < w) N.perm[p + lane] = pv;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,10 +282,29 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
   if (step >= N.nsteps) return;
   const int p = N.p, w = N.w;
   if (w == 0) return;
-  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ldl = N.ldl;
   double *L = N.L;
+  // the first nleft workgroups permute rows p..p+w-1 of the eliminated
+  // columns left of p into the block's elimination order (64-column chunks)
+  const int nleft = step * kBlk / kBlk;  // upper bound on ceil(p / 64)
+  if (local < nleft) {
+    const int cb = local * kBlk;
+    if (cb >= p) return;
+    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
+      int r = idx % kBlk, cc = idx / kBlk;
+      if (r < w && cb + cc < p) lbb[r][cc] = L[(size_t)(cb + cc) * ldl + p + r];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
+      int r = idx % kBlk, cc = idx / kBlk;
+      if (r < w && cb + cc < p) L[(size_t)(cb + cc) * ldl + p + r] = lbb[N.lp[r]][cc];
+    }
+    return;
+  }
+  local -= nleft;
+  count -= nleft;
   for (int idx = tid; idx < w * w; idx += kThreads) {
     int r = idx % w, c = idx / w;
     lbb[r][c] = (r > c) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
@@ -566,11 +428,26 @@ __device__ inline void pivot_d(const double *D, int k, int kend, double &e1, dou
 }
 
 // ---------------------------------------------------------------------------
-// Trailing update with MFMA:  F[i,j] -= sum_k L[i,a0+k] (L D)[j,a0+k]
-// over the lower part of [t0,m)^2; failed block columns restored first.
-// mode 0: APP step; mode 1: contribution update for TPP pivots.
+// Panel update with MFMA:  F[i,j] -= sum_k L[i,a0+k] (L D)[j,a0+k]
+// mode 0 (APP step): accepted pivots [p, p+npass) applied to the panel's
+//   trailing columns [t0, n), rows [t0, m); failed block columns are restored
+//   from the backup first.  The contribution block is NOT touched here: its
+//   Schur update is applied lazily when the parent gathers it (k_gather).
+// mode 1: subtree roots with a contribution block: all nelim pivots applied
+//   to [n, m)^2 so the block can be exported.
 // ---------------------------------------------------------------------------
 constexpr int kTile = 64;
+
+__device__ inline void tile_of(int tile, int TI, int TJ, int &I, int &J) {
+  // lower tiles (I >= J), column-major over J
+  int t = tile;
+  J = 0;
+  while (J < TJ && t >= TI - J) {
+    t -= TI - J;
+    ++J;
+  }
+  I = J + t;
+}
 
 __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *lvl_slots,
                                                      const int *wg_off, int nlvl, int step,
@@ -581,7 +458,7 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
   NodeDev &N = nodes[lvl_slots[li]];
   const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int a0, K, t0, fa, fb, p = 0, w = 0;
+  int a0, K, t0, cend, fa, fb, p = 0, w = 0;
   if (mode == 0) {
     if (step >= N.nsteps) return;
     p = N.p;
@@ -592,79 +469,80 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
     a0 = p;
     K = np;
     t0 = p + np;
+    cend = N.n;
     fa = p + np;
     fb = p + w;
   } else {
-    a0 = N.nelim_app;
-    K = N.nelim - N.nelim_app;
+    if (!N.is_root || N.m <= N.n) return;
+    a0 = 0;
+    K = N.nelim;
     t0 = N.n;
+    cend = N.m;
     fa = fb = 0;
     if (K <= 0) return;
   }
   const int m = N.m;
-  const int nt = m - t0;
-  if (nt <= 0) return;
-  const int T = (nt + kTile - 1) / kTile;
-  const int ntiles = T * (T + 1) / 2;
+  if (m - t0 <= 0 || cend - t0 <= 0) return;
+  const int TI = (m - t0 + kTile - 1) / kTile, TJ = (cend - t0 + kTile - 1) / kTile;
+  const int ntiles = TJ * TI - TJ * (TJ - 1) / 2;
   const double *D = N.D;
   const int ldl = N.ldl;
+  const int Kc = kBlk;  // K chunk
   for (int tile = local; tile < ntiles; tile += count) {
-    int I = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
-    while (I * (I + 1) / 2 > tile) --I;
-    while ((I + 1) * (I + 2) / 2 <= tile) ++I;
-    const int J = tile - I * (I + 1) / 2;
+    int I, J;
+    tile_of(tile, TI, TJ, I, J);
     const int r0 = t0 + I * kTile, c0 = t0 + J * kTile;
     const bool has_fail = (fb > fa) && (c0 < fb) && (c0 + kTile > fa);
     if (K == 0 && !has_fail) continue;
-    const int Kp = (K + 3) & ~3;
-    __syncthreads();
-    // stage L rows of the tile (A) and (L D) rows of the tile columns (B)
-    for (int idx = tid; idx < kTile * Kp; idx += kThreads) {
-      int i = idx % kTile, k = idx / kTile;
-      int gi = r0 + i, gj = c0 + i;
-      double av = 0.0, bv = 0.0;
-      if (k < K) {
-        if (gi < m) av = N.L[(size_t)(a0 + k) * ldl + gi];
-        if (gj < m) {
-          double e1, e2;
-          int two;
-          pivot_d(D, a0 + k, a0 + K, e1, e2, two);
-          double lk = N.L[(size_t)(a0 + k) * ldl + gj];
-          if (two == 0) {
-            bv = lk * e1;
-          } else if (two == 1) {
-            double lk1 = N.L[(size_t)(a0 + k + 1) * ldl + gj];
-            bv = lk * e1 + lk1 * e2;
-          } else {
-            double lkm = N.L[(size_t)(a0 + k - 1) * ldl + gj];
-            bv = lkm * e1 + lk * e2;
-          }
-        }
-      }
-      As[i][k] = av;
-      Bs[i][k] = bv;
-    }
-    __syncthreads();
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     double4_t acc[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
       for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (int kk = 0; kk < Kp; kk += 4) {
-      const int kl = kk + (lane >> 4);
+    for (int k0 = 0; k0 < K; k0 += Kc) {
+      const int kn = min(Kc, K - k0);
+      const int Kp = (kn + 3) & ~3;
+      __syncthreads();
+      for (int idx = tid; idx < kTile * Kp; idx += kThreads) {
+        int i = idx % kTile, kk = idx / kTile;
+        int k = k0 + kk;
+        int gi = r0 + i, gj = c0 + i;
+        double av = 0.0, bv = 0.0;
+        if (kk < kn) {
+          if (gi < m) av = N.L[(size_t)(a0 + k) * ldl + gi];
+          if (gj < m && gj < cend) {
+            double e1, e2;
+            int two;
+            pivot_d(D, a0 + k, a0 + K, e1, e2, two);
+            double lk = N.L[(size_t)(a0 + k) * ldl + gj];
+            if (two == 0) {
+              bv = lk * e1;
+            } else if (two == 1) {
+              bv = lk * e1 + N.L[(size_t)(a0 + k + 1) * ldl + gj] * e2;
+            } else {
+              bv = N.L[(size_t)(a0 + k - 1) * ldl + gj] * e1 + lk * e2;
+            }
+          }
+        }
+        As[i][kk] = av;
+        Bs[i][kk] = bv;
+      }
+      __syncthreads();
+      for (int kk = 0; kk < Kp; kk += 4) {
+        const int kl = kk + (lane >> 4);
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi) {
-        double av = As[wr + bi * 16 + (lane & 15)][kl];
+        for (int bi = 0; bi < 2; ++bi) {
+          double av = As[wr + bi * 16 + (lane & 15)][kl];
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj) {
-          double bv = Bs[wc + bj * 16 + (lane & 15)][kl];
-          acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+          for (int bj = 0; bj < 2; ++bj) {
+            double bv = Bs[wc + bj * 16 + (lane & 15)][kl];
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[bi][bj], 0, 0, 0);
+          }
         }
       }
     }
-    // write back: element (i,j) of a 16x16 block lives in lane j + 16*(i%4),
-    // register i/4 (measured on gfx950).
+    // element (i,j) of a 16x16 block lives in lane j + 16*(i%4), register i/4
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -674,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
           int i = wr + bi * 16 + (lane >> 4) + 4 * rg;
           int j = wc + bj * 16 + (lane & 15);
           int gi = r0 + i, gj = c0 + j;
-          if (gi < m && gj < m && gi >= gj) {
+          if (gi < m && gj < cend && gi >= gj) {
             double *dst = front_at(N, gi, gj);
             double old;
             if (gj >= fa && gj < fb) {
@@ -687,6 +565,239 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
             *dst = old - acc[bi][bj][rg];
           }
         }
+  }
+}
+
+// D (not its inverse) per pivot for the lazy contribution update:
+// Dd[3k] = D_kk, Dd[3k+1] = D_k,partner, Dd[3k+2] = partner offset.
+__global__ void k_dprep(NodeDev *nodes, const int *lvl_slots) {
+  NodeDev &N = nodes[lvl_slots[blockIdx.x]];
+  const int ne = N.nelim;
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+    double e1, e2;
+    int two;
+    pivot_d(N.D, k, ne, e1, e2, two);
+    if (two == 0) {
+      N.Dd[3 * k] = e1;
+      N.Dd[3 * k + 1] = 0.0;
+      N.Dd[3 * k + 2] = 0.0;
+    } else if (two == 1) {
+      N.Dd[3 * k] = e1;
+      N.Dd[3 * k + 1] = e2;
+      N.Dd[3 * k + 2] = 1.0;
+    } else {
+      N.Dd[3 * k] = e2;
+      N.Dd[3 * k + 1] = e1;
+      N.Dd[3 * k + 2] = -1.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather assembly of one level (replaces zero + scatter + extend-add + the
+// Schur update of the children's contribution blocks).  A workgroup owns a
+// 64x64 tile of one parent front and builds it in LDS:
+//   for each child in order, the child's trailing index ranges that map into
+//   the tile are found by binary search on the monotone child->parent map and
+//   staged in LDS; failed (delayed) columns are copied from the child panel;
+//   contribution columns are computed as C_c - L21 (L21 D)^T on MFMA
+//   (K = the child's nelim, in 64-wide chunks) and added through the maps;
+//   then the tile's A entries are added and the tile is stored once.
+// ---------------------------------------------------------------------------
+constexpr int kGT = 64;
+constexpr int kGK = 32;  // K chunk of the contribution GEMM
+
+// Per level, for every child of the level's fronts: the parent position of
+// each trailing index (pmap) and, for each 64-row tile of the parent, the
+// first trailing index whose parent position reaches the tile (ptab).
+__global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *lvl_slots,
+                                                     const int *child_list) {
+  const NodeDev &P = nodes[lvl_slots[blockIdx.x]];
+  const int T = (P.m + kGT - 1) / kGT;
+  int doff = 0;
+  for (int ci = P.cbeg; ci < P.cend; ++ci) {
+    NodeDev &c = nodes[child_list[ci]];
+    int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
+    const int cdoff = doff;
+    doff += c.n - c.nelim;
+    const int nt = c.m - c.nelim;
+    for (int t = threadIdx.x; t < nt; t += kThreads) {
+      const int k = c.nelim + t;
+      const int pk = child_pos(c, cdoff, P.nd, k);
+      c.pmap[t] = pk;
+      const int tile = pk / kGT;
+      const int prev = (t == 0) ? -1 : child_pos(c, cdoff, P.nd, k - 1) / kGT;
+      for (int I = prev + 1; I <= tile; ++I) tab[I] = k;
+      if (t == nt - 1)
+        for (int I = tile + 1; I <= T; ++I) tab[I] = c.m;
+    }
+    if (nt == 0)
+      for (int I = threadIdx.x; I <= T; I += kThreads) tab[I] = c.m;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_gather(
+    NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
+    const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
+    const double *scaling) {
+  __shared__ double acc[kGT][kGT + 1];
+  __shared__ double As[kGT][kGK + 1];
+  __shared__ double Bs[kGT][kGK + 1];
+  __shared__ int mr[kGT], mc[kGT];
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  const int s = lvl_slots[li];
+  NodeDev &P = nodes[s];
+  const int m = P.m, n = P.n, nd = P.nd;
+  const int T = (m + kGT - 1) / kGT;
+  const int local = blockIdx.x - wg_off[li];
+  if (local >= T * (T + 1) / 2) return;
+  int I, J;
+  tile_of(local, T, T, I, J);
+  const int r0 = I * kGT, c0 = J * kGT;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int idx = tid; idx < kGT * kGT; idx += kThreads) acc[idx % kGT][idx / kGT] = 0.0;
+  // fully-summed variable list (diagonal tiles own their columns)
+  if (I == J) {
+    for (int j = c0 + tid; j < min(c0 + kGT, n); j += kThreads) {
+      int v;
+      if (j >= nd) {
+        v = P.rows[j - nd];
+      } else {
+        int doff = 0;
+        v = -1;
+        for (int ci = P.cbeg; ci < P.cend; ++ci) {
+          const NodeDev &c = nodes[child_list[ci]];
+          int nf = c.n - c.nelim;
+          if (j >= doff && j < doff + nf) {
+            v = c.perm[c.nelim + (j - doff)];
+            break;
+          }
+          doff += nf;
+        }
+      }
+      P.perm[j] = v;
+      P.perm0[j] = v;
+    }
+  }
+  __syncthreads();
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  int doff = 0;
+  for (int ci = P.cbeg; ci < P.cend; ++ci) {
+    const NodeDev &c = nodes[child_list[ci]];
+    const int cdoff = doff;
+    doff += c.n - c.nelim;
+    (void)cdoff;
+    const int kb = c.nelim, ke = c.m;
+    if (ke <= kb) continue;
+    const int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
+    const int kr0 = tab[I], kr1 = tab[I + 1];
+    if (kr1 <= kr0) continue;
+    const int k0 = tab[J], k1 = tab[J + 1];
+    if (k1 <= k0 || kr1 - 1 < k0) continue;
+    for (int t = tid; t < kr1 - kr0; t += kThreads) mr[t] = c.pmap[kr0 + t - kb] - r0;
+    for (int t = tid; t < k1 - k0; t += kThreads) mc[t] = c.pmap[k0 + t - kb] - c0;
+    __syncthreads();
+    // failed (delayed) columns of the child: copy from its panel
+    const int kf1 = min(k1, c.n);
+    if (kf1 > k0) {
+      const int nr = kr1 - kr0, ncf = kf1 - k0;
+      for (int idx = tid; idx < nr * ncf; idx += kThreads) {
+        int a = idx % nr, b = idx / nr;
+        int kr = kr0 + a, k = k0 + b;
+        if (kr >= k) acc[mr[a]][mc[b]] += c.L[(size_t)k * c.ldl + kr];
+      }
+    }
+    // contribution columns: C_c - L21 (L21 D)^T
+    const int kc0 = max(k0, c.n);
+    const int rr0 = max(kr0, c.n);
+    if (k1 > kc0 && kr1 > rr0) {
+      const int nrow = kr1 - rr0, ncol = k1 - kc0;
+      const int K = c.nelim;
+      double4_t g[2][2];
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      for (int t0 = 0; t0 < K; t0 += kGK) {
+        const int kn = min(kGK, K - t0);
+        const int Kp = (kn + 3) & ~3;
+        __syncthreads();
+        for (int idx = tid; idx < kGT * Kp; idx += kThreads) {
+          int i = idx % kGT, kk = idx / kGT;
+          double av = 0.0, bv = 0.0;
+          if (kk < kn) {
+            int t = t0 + kk;
+            if (i < nrow) av = c.L[(size_t)t * c.ldl + rr0 + i];
+            if (i < ncol) {
+              int row = kc0 + i;
+              double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
+              int off = (int)c.Dd[3 * t + 2];
+              bv = c.L[(size_t)t * c.ldl + row] * e;
+              if (off != 0) bv += c.L[(size_t)(t + off) * c.ldl + row] * f;
+            }
+          }
+          As[i][kk] = av;
+          Bs[i][kk] = bv;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < Kp; kk += 4) {
+          const int kl = kk + (lane >> 4);
+#pragma unroll
+          for (int bi = 0; bi < 2; ++bi) {
+            double av = As[wr + bi * 16 + (lane & 15)][kl];
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+              double bv = Bs[wc + bj * 16 + (lane & 15)][kl];
+              g[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, g[bi][bj], 0, 0, 0);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+          for (int rg = 0; rg < 4; ++rg) {
+            int i = wr + bi * 16 + (lane >> 4) + 4 * rg;
+            int j = wc + bj * 16 + (lane & 15);
+            if (i < nrow && j < ncol) {
+              int kr = rr0 + i, k = kc0 + j;
+              if (kr >= k) {
+                double cv = c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
+                acc[mr[kr - kr0]][mc[k - k0]] += cv - g[bi][bj][rg];
+              }
+            }
+          }
+    }
+    __syncthreads();
+  }
+  // A entries of the tile's own fully-summed columns
+  {
+    int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
+    if (cs1 > cs0) {
+      int64_t e0 = P.acol[cs0], e1 = P.acol[cs1];
+      for (int64_t e = e0 + tid; e < e1; e += kThreads) {
+        int r = arow[e] + nd;
+        if (r < r0 || r >= r0 + kGT) continue;
+        int lo = cs0, hi = cs1;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (P.acol[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        int cc = lo;
+        double v = aval[asrc[e]];
+        if (scaling) v *= scaling[P.rows[r - nd]] * scaling[P.rows[cc]];
+        acc[r - r0][cc + nd - c0] += v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kGT * kGT; idx += kThreads) {
+    int i = idx % kGT, j = idx / kGT;
+    int gr = r0 + i, gc = c0 + j;
+    if (gr < m && gc < m && gr >= gc) *front_at(P, gr, gc) = acc[i][j];
   }
 }
 
@@ -1032,18 +1143,25 @@ __global__ void k_stats(const NodeDev *nodes, int nslots, int *counters) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-void launch_assemble(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
-                     const int *wg_off, int nlvl, const int *child_list,
-                     const double *aval, const int64_t *asrc, const int *arow,
-                     const double *scaling) {
+void launch_gather(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                   const int *wg_off, int nlvl, const int *child_list, const double *aval,
+                   const int64_t *asrc, const int *arow, const double *scaling) {
   if (nwg > 0)
-    hipLaunchKernelGGL(k_assemble, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
-                       nlvl, child_list, aval, asrc, arow, scaling);
+    hipLaunchKernelGGL(k_gather, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,
+                       child_list, aval, asrc, arow, scaling);
+}
+void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
+                   const int *child_list) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_maptab, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, child_list);
+}
+void launch_dprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
+  if (nlvl > 0) hipLaunchKernelGGL(k_dprep, dim3(nlvl), dim3(256), 0, st, nodes, slots);
 }
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_diag, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, step, fp);
+    hipLaunchKernelGGL(k_diag, dim3(nlvl), dim3(64), 0, st, nodes, slots, step, fp);
 }
 void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                   const int *wg_off, int nlvl, int step, FactorParams fp) {

@@ -23,8 +23,10 @@
 namespace ssids_hip {
 
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
-void launch_assemble(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
-                     const double *, const int64_t *, const int *, const double *);
+void launch_gather(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
+                   const double *, const int64_t *, const int *, const double *);
+void launch_dprep(hipStream_t, int, NodeDev *, const int *);
+void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
 void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, int,
                   FactorParams);
@@ -36,9 +38,10 @@ void launch_tpp(hipStream_t, int, NodeDev *, const int *, FactorParams);
 void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
 void launch_fwd(hipStream_t, int, const SolveNode *, const int *, const int *, double *, int,
-                int);
+                int, const int *, int);
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
-void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int);
+void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
+                const int *, int);
 
 void launch_residual(hipStream_t, int, const int64_t *, const int *, const int64_t *,
                      const double *, const double *, const double *, double *, int, int);
@@ -184,6 +187,9 @@ struct Numeric {
   // solve
   SolveNode *d_sn = nullptr;
   int *d_slots_all = nullptr;             // 0..nnodes-1 (slots in level order)
+  int *d_gemv_tab = nullptr;              // per level: nlvl+1 GEMV workgroup offsets
+  std::vector<int> gemv_tab_off, gemv_total;  // per level: offset into d_gemv_tab, total
+  std::vector<int> gemv_count;            // per slot
   std::vector<SolveNode> sn_h;
   double *d_X = nullptr;
   size_t d_X_cap = 0;
@@ -212,6 +218,7 @@ struct Numeric {
     if (d_scratch_n) (void)hipFree(d_scratch_n);
     if (d_sn) (void)hipFree(d_sn);
     if (d_slots_all) (void)hipFree(d_slots_all);
+    if (d_gemv_tab) (void)hipFree(d_gemv_tab);
     if (d_X) (void)hipFree(d_X);
     if (d_x) (void)hipFree(d_x);
     if (d_aval) (void)hipFree(d_aval);
@@ -492,7 +499,9 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       if (D.nsteps > 1) multi = true;
       size_t lb = (size_t)D.ldl * D.n * sizeof(double);
       D.L = (double *)palloc(lb);
+      D.pmap = (int *)palloc(std::max(D.m, 1) * sizeof(int));
       D.D = (double *)palloc(2 * (size_t)D.n * sizeof(double));
+      D.Dd = (double *)palloc(3 * (size_t)std::max(D.n, 1) * sizeof(double));
       D.perm = (int *)palloc(std::max(D.n, 1) * sizeof(int));
       D.perm0 = (int *)palloc(std::max(D.n, 1) * sizeof(int));
       D.ipos = (int *)palloc(std::max(D.n, 1) * sizeof(int));
@@ -501,11 +510,14 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       contrib_of[sl] = {D.C, cb};
       size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
       if (D.nsteps > 1) wb += (size_t)2 * kBlk * D.m * sizeof(double);
+      size_t tabn = (size_t)(cdiv(D.m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
+      wb += tabn * sizeof(int);
       char *w = (char *)N.pool.alloc(wb);
       work_of[sl] = {w, wb};
       D.bak = (double *)w;
       D.dbak = D.bak + (size_t)D.ldbak * kBlk;
       D.swp = (D.nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
+      D.ctab = (int *)(D.dbak + kBlk * kBlk + ((D.nsteps > 1) ? (size_t)2 * kBlk * D.m : 0));
     }
     HIPCHK(hipMemcpyAsync(N.d_nodes + s0, N.nodes.data() + s0, nl * sizeof(NodeDev),
                           hipMemcpyHostToDevice, st));
@@ -523,8 +535,16 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       tab.push_back(acc);
       return std::make_pair(base, acc);
     };
-    auto asm_t = add_off([](const NodeDev &D) { return cdiv(D.m, 32); });
-    auto app_t = add_off([](const NodeDev &D) { return std::max(1, cdiv(D.m, kThreads)); });
+    auto asm_t = add_off([](const NodeDev &D) {
+      int T = cdiv(D.m, 64);
+      return T * (T + 1) / 2;
+    });
+    std::vector<std::pair<int, int>> app_t(maxsteps);
+    for (int stp = 0; stp < maxsteps; ++stp)
+      app_t[stp] = add_off([stp](const NodeDev &D) {
+        if (stp >= D.nsteps) return 0;
+        return stp + std::max(1, cdiv(D.m, kThreads));  // stp chunks of L-left rows
+      });
     auto swp_t = add_off([](const NodeDev &D) {
       return D.nsteps > 1 ? std::max(1, cdiv((int64_t)2 * kBlk * D.m, kThreads * 8)) : 0;
     });
@@ -532,16 +552,15 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     for (int stp = 0; stp < maxsteps; ++stp) {
       upd_t[stp] = add_off([stp](const NodeDev &D) {
         if (stp >= D.nsteps) return 0;
-        int nt = D.m - std::min(D.n, (stp + 1) * kBlk);
-        nt = std::max(nt, kBlk);
-        int T = cdiv(nt, kBlk);
-        return std::max(1, std::min(T * (T + 1) / 2, 4096));
+        int t0 = std::min(D.n, (stp + 1) * kBlk);
+        int TI = cdiv(D.m - t0, kBlk), TJ = cdiv(D.n - t0, kBlk);
+        int nt = TJ * TI - TJ * (TJ - 1) / 2;
+        return std::max(1, std::min(nt, 4096));
       });
     }
     auto tpp_t = add_off([](const NodeDev &D) {
-      int nt = D.m - D.n;
-      if (nt <= 0) return 1;
-      int T = cdiv(nt, kBlk);
+      if (!D.is_root || D.m <= D.n) return 0;
+      int T = cdiv(D.m - D.n, kBlk);
       return std::max(1, std::min(T * (T + 1) / 2, 4096));
     });
     ensure_lvl(N, tab.size());
@@ -549,15 +568,16 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
                           hipMemcpyHostToDevice, st));
     const int *d_slots = N.d_lvl;
     auto off = [&](std::pair<int, int> t) { return (const int *)(N.d_lvl + t.first); };
-    launch_assemble(st, asm_t.second, N.d_nodes, d_slots, off(asm_t), nl, S.d_child, d_aval,
-                    S.d_asrc, S.d_arow, d_scaling);
+    launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
+    if (opt.timing) tev.emplace_back(timed_event(st));
+    launch_gather(st, asm_t.second, N.d_nodes, d_slots, off(asm_t), nl, S.d_child, d_aval,
+                  S.d_asrc, S.d_arow, d_scaling);
+    if (opt.timing) tev.emplace_back(timed_event(st));
     for (int stp = 0; stp < maxsteps; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
-      launch_apply(st, app_t.second, N.d_nodes, d_slots, off(app_t), nl, stp, fp);
-      if (opt.timing) tev.emplace_back(timed_event(st));
+      launch_apply(st, app_t[stp].second, N.d_nodes, d_slots, off(app_t[stp]), nl, stp, fp);
       launch_update(st, upd_t[stp].second, N.d_nodes, d_slots, off(upd_t[stp]), nl, stp, 0,
                     fp);
-      if (opt.timing) tev.emplace_back(timed_event(st));
       if (multi) {
         launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 0);
         launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 1);
@@ -565,9 +585,9 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     }
     launch_finalize(st, nl, N.d_nodes, d_slots);
     launch_tpp(st, nl, N.d_nodes, d_slots, fp);
-    if (opt.timing) tev.emplace_back(timed_event(st));
-    launch_update(st, tpp_t.second, N.d_nodes, d_slots, off(tpp_t), nl, 0, 1, fp);
-    if (opt.timing) tev.emplace_back(timed_event(st));
+    launch_dprep(st, nl, N.d_nodes, d_slots);
+    if (tpp_t.second > 0)
+      launch_update(st, tpp_t.second, N.d_nodes, d_slots, off(tpp_t), nl, 0, 1, fp);
     launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
     HIPCHK(hipGetLastError());
     // read back the level's results (nelim, flag)
@@ -641,37 +661,32 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   o.num_levels = nlev;
   o.factor_ms = ms;
   o.pool_peak_bytes = (int64_t)N.pool.peak();
-  o.update_ms = 0.0;
-  o.update_launches = 0;
+  o.kernel_ms = 0.0;
+  o.kernel_launches = 0;
   for (size_t k = 0; k + 1 < tev.size(); k += 2) {
     float e = 0.f;
     HIPCHK(hipEventElapsedTime(&e, tev[k], tev[k + 1]));
-    o.update_ms += e;
-    o.update_launches++;
+    o.kernel_ms += e;
+    o.kernel_launches++;
   }
   for (auto e : tev) (void)hipEventDestroy(e);
-  // algorithmic flops of the Schur updates: a rank-K update of the lower
-  // triangle of an N x N trailing matrix costs K*N*(N+1) flops (2 per FMA).
-  // Blocks are assumed to pass in full (the count is exact without delays).
-  double uf = 0.0;
+  // algorithmic work of the gather kernels: each child's Schur complement
+  // C_c - L21 D L21^T on the lower triangle (K*N*(N+1) flops, K = nelim,
+  // N = m-n), and HBM bytes = parent fronts written once (lower part) +
+  // children's contribution blocks (lower) and L21 panels read once.
+  double gf = 0.0, gb = 0.0;
   for (int sl = 0; sl < nn; ++sl) {
     const NodeDev &D = N.nodes[sl];
-    if (fp.pivot_tpp) {
-      double K = D.nelim, Nn = D.m - D.n;
-      uf += K * Nn * (Nn + 1);
-      continue;
+    double m = D.m, n = D.n;
+    gb += 8.0 * (n * (n + 1) / 2 + (m - n) * n + (m - n) * (m - n + 1) / 2);  // written
+    if (D.parent >= 0) {
+      double K = D.nelim, Nc = m - n;
+      gf += K * Nc * (Nc + 1);
+      gb += 8.0 * (Nc * (Nc + 1) / 2 + Nc * K + (n - D.nelim) * m);  // read
     }
-    int p = 0;
-    for (int stp = 0; stp < D.nsteps && p < D.nelim_app; ++stp) {
-      double K = std::min(kBlk, D.nelim_app - p);
-      double Nn = D.m - (p + K);
-      uf += K * Nn * (Nn + 1);
-      p += (int)K;
-    }
-    double K2 = D.nelim - D.nelim_app, N2 = D.m - D.n;
-    uf += K2 * N2 * (N2 + 1);
   }
-  o.update_flops = uf;
+  o.kernel_flops = gf;
+  o.kernel_bytes = gb;
   // solve descriptors
   N.sn_h.assign(nn, SolveNode{});
   std::vector<int> doff(nn, 0);
@@ -710,6 +725,31 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
                      hipMemcpyHostToDevice));
   }
+  // solve GEMV workgroup tables (depend on nelim)
+  {
+    const int nlev2 = (int)S.lvl_ptr.size() - 1;
+    std::vector<int> tab;
+    N.gemv_tab_off.assign(nlev2, 0);
+    N.gemv_total.assign(nlev2, 0);
+    N.gemv_count.assign(nn, 0);
+    for (int L = 0; L < nlev2; ++L) {
+      N.gemv_tab_off[L] = (int)tab.size();
+      int acc = 0;
+      for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) {
+        tab.push_back(acc);
+        const NodeDev &D = N.nodes[sl];
+        int c = (D.m > D.nelim) ? std::min(64, cdiv(D.m - D.nelim, kThreads)) : 0;
+        N.gemv_count[sl] = c;
+        acc += c;
+      }
+      tab.push_back(acc);
+      N.gemv_total[L] = acc;
+    }
+    if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
+    HIPCHK(hipMalloc(&N.d_gemv_tab, std::max<size_t>(tab.size(), 1) * sizeof(int)));
+    HIPCHK(hipMemcpy(N.d_gemv_tab, tab.data(), tab.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
   N.solve_nrhs_cap = 0;  // X pointers need refreshing
 }
 
@@ -717,7 +757,9 @@ static void ensure_solve_ws(Numeric &N, int nrhs) {
   Symbolic &S = *N.symb;
   if (nrhs <= N.solve_nrhs_cap) return;
   size_t tot = 0;
-  for (int sl = 0; sl < S.nnodes; ++sl) tot += (size_t)N.nodes[sl].m * nrhs;
+  for (int sl = 0; sl < S.nnodes; ++sl)
+    tot += (size_t)N.nodes[sl].m * nrhs +
+           (size_t)N.gemv_count[sl] * nrhs * std::max(N.nodes[sl].nelim, 1);
   if (tot > N.d_X_cap) {
     if (N.d_X) HIPCHK(hipFree(N.d_X));
     HIPCHK(hipMalloc(&N.d_X, std::max(tot, size_t(1)) * sizeof(double)));
@@ -727,6 +769,8 @@ static void ensure_solve_ws(Numeric &N, int nrhs) {
   for (int sl = 0; sl < S.nnodes; ++sl) {
     N.sn_h[sl].X = N.d_X + offx;
     offx += (size_t)N.nodes[sl].m * nrhs;
+    N.sn_h[sl].P = N.d_X + offx;
+    offx += (size_t)N.gemv_count[sl] * nrhs * std::max(N.nodes[sl].nelim, 1);
   }
   HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), S.nnodes * sizeof(SolveNode),
                    hipMemcpyHostToDevice));
@@ -745,12 +789,12 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   if (fwd)
     for (int L = 0; L < nlev; ++L)
       launch_fwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L],
-                 S.d_child, d_x, ldx, nrhs);
+                 S.d_child, d_x, ldx, nrhs, N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
   if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs);
   if (bwd)
     for (int L = nlev - 1; L >= 0; --L)
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
-                 ldx, nrhs);
+                 ldx, nrhs, N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
   HIPCHK(hipGetLastError());
 }
 

@@ -1,13 +1,19 @@
 // Forward / diagonal / backward solve kernels over the resident factors.
 //
-// The forward solve is a multifrontal extend-add of vectors: each front keeps
-// its trailing part X_s[nelim:m] (the update passed to the parent), the
-// parent gathers its children's updates in child order (deterministic), and
-// eliminated values are written to x by variable.  The backward solve reads x
-// by variable, so fronts of one level never write the same entry.
-// Dense parts are swept in 64-column blocks so L is read once per RHS
-// (reference semantics: src/ssids/cpu/NumericSubtree.hxx:286-406,
-// src/ssids/cpu/kernels/ldlt_app.cxx:2537-2589).
+// Forward (levels leaves -> root), per front s with nelim = ne:
+//   k_fwd_tri : one workgroup per front: X = own b + children's updates
+//               (children in order: deterministic), then the unit-lower
+//               ne x ne triangle (64-column blocks staged in LDS); the
+//               eliminated values y go to x by variable.
+//   k_fwd_gemv: many workgroups per front: X[ne:m] -= L21 y (rows split
+//               across workgroups, up to 8 right-hand sides per pass so L21
+//               is read once per 8 RHS).  X[ne:m] is the update the parent
+//               gathers.
+// Backward (root -> leaves):
+//   k_bwd_gemv: many workgroups per front: partial sums of L21^T x[rows].
+//   k_bwd_tri : one workgroup per front: reduce the partials, L11^T solve.
+// Reference semantics: src/ssids/cpu/NumericSubtree.hxx:286-406 and
+// src/ssids/cpu/kernels/ldlt_app.cxx:2537-2589.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -15,6 +21,18 @@
 #include "dev.hpp"
 
 namespace ssids_hip {
+
+constexpr int kRhsChunk = 8;
+
+__device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
+  int lo = 0, hi = nlvl;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (wg_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // post-pivot parent position of child trailing index k
 __device__ inline int post_pos(const SolveNode &P, const SolveNode &c, int k) {
@@ -31,9 +49,44 @@ __global__ void k_ipos(NodeDev *nodes, const int *slots, int *scratch) {
   for (int j = threadIdx.x; j < N.n; j += blockDim.x) N.ipos[j] = scratch[N.perm0[j]];
 }
 
-__global__ __launch_bounds__(kThreads) void k_fwd(const SolveNode *sn, const int *slots,
-                                                  const int *child_list, double *x,
-                                                  int ldx, int nrhs) {
+// unit-lower triangle solve of X[0:ne] (one workgroup), 64-column blocks:
+// the diagonal block is staged in LDS and solved by one wave, the rows below
+// inside the triangle are updated by the whole workgroup.
+__device__ void tri_lower(const SolveNode &P, double *X, double (*lb)[kBlk + 1], double *yb) {
+  const int tid = threadIdx.x, ne = P.nelim;
+  for (int j0 = 0; j0 < ne; j0 += kBlk) {
+    const int j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+    for (int idx = tid; idx < wb * wb; idx += kThreads) {
+      int r = idx % wb, c = idx / wb;
+      lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      double v = (tid < wb) ? X[j0 + tid] : 0.0;
+      for (int j = 0; j < wb; ++j) {
+        double yj = __shfl(v, j);
+        if (tid > j && tid < wb) v -= lb[tid][j] * yj;
+      }
+      if (tid < wb) {
+        X[j0 + tid] = v;
+        yb[tid] = v;
+      }
+    }
+    __syncthreads();
+    for (int i = j1 + tid; i < ne; i += kThreads) {
+      double acc = X[i];
+      const double *lc = P.L + (size_t)j0 * P.ldl + i;
+      for (int j = 0; j < wb; ++j) acc -= lc[(size_t)j * P.ldl] * yb[j];
+      X[i] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const int *slots,
+                                                      const int *child_list, double *x,
+                                                      int ldx, int nrhs) {
+  __shared__ double lb[kBlk][kBlk + 1];
   __shared__ double yb[kBlk];
   const SolveNode &P = sn[slots[blockIdx.x]];
   const int tid = threadIdx.x, m = P.m, n = P.n, ne = P.nelim;
@@ -50,31 +103,47 @@ __global__ __launch_bounds__(kThreads) void k_fwd(const SolveNode *sn, const int
       for (int k = c.nelim + tid; k < c.m; k += kThreads) X[post_pos(P, c, k)] += Xc[k];
       __syncthreads();
     }
-    // blocked forward substitution with unit lower L (columns [0, ne))
-    for (int j0 = 0; j0 < ne; j0 += kBlk) {
-      const int j1 = min(j0 + kBlk, ne);
-      if (tid < 64) {
-        // one wave solves the diagonal block
-        double v = (j0 + tid < j1) ? X[j0 + tid] : 0.0;
-        for (int j = j0; j < j1; ++j) {
-          double yj = __shfl(v, j - j0);
-          if (tid > j - j0 && j0 + tid < j1) v -= P.L[(size_t)j * P.ldl + j0 + tid] * yj;
-        }
-        if (j0 + tid < j1) {
-          X[j0 + tid] = v;
-          yb[tid] = v;
-        }
-      }
-      __syncthreads();
-      for (int i = j1 + tid; i < m; i += kThreads) {
-        double acc = X[i];
-        for (int j = j0; j < j1; ++j) acc -= P.L[(size_t)j * P.ldl + i] * yb[j - j0];
-        X[i] = acc;
-      }
-      __syncthreads();
-    }
+    tri_lower(P, X, lb, yb);
     for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
     __syncthreads();
+  }
+}
+
+// X[ne:m, r] -= L[ne:m, 0:ne] X[0:ne, r] for rows of this workgroup
+__global__ __launch_bounds__(kThreads) void k_fwd_gemv(const SolveNode *sn, const int *slots,
+                                                       const int *wg_off, int nlvl, int nrhs) {
+  __shared__ double ys[kRhsChunk][kBlk];
+  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
+  const SolveNode &P = sn[slots[li]];
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  const int tid = threadIdx.x, m = P.m, ne = P.nelim;
+  for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
+    const int nr = min(kRhsChunk, nrhs - r0);
+    for (int i0 = ne + local * kThreads; i0 < m; i0 += count * kThreads) {
+      const int i = i0 + tid;
+      double acc[kRhsChunk];
+#pragma unroll
+      for (int q = 0; q < kRhsChunk; ++q) acc[q] = 0.0;
+      for (int j0 = 0; j0 < ne; j0 += kBlk) {
+        const int wb = min(kBlk, ne - j0);
+        __syncthreads();
+        for (int idx = tid; idx < kRhsChunk * kBlk; idx += kThreads) {
+          int q = idx / kBlk, j = idx % kBlk;
+          ys[q][j] = (q < nr && j < wb) ? P.X[(size_t)(r0 + q) * m + j0 + j] : 0.0;
+        }
+        __syncthreads();
+        if (i < m) {
+          const double *lc = P.L + (size_t)j0 * P.ldl + i;
+          for (int j = 0; j < wb; ++j) {
+            double l = lc[(size_t)j * P.ldl];
+#pragma unroll
+            for (int q = 0; q < kRhsChunk; ++q) acc[q] += l * ys[q][j];
+          }
+        }
+      }
+      if (i < m)
+        for (int q = 0; q < nr; ++q) P.X[(size_t)(r0 + q) * m + i] -= acc[q];
+    }
   }
 }
 
@@ -100,33 +169,84 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_bwd(const SolveNode *sn, const int *slots,
-                                                  double *x, int ldx, int nrhs) {
-  __shared__ double part[kThreads / 64][kBlk];
-  __shared__ double xb[kBlk];
-  const SolveNode &P = sn[slots[blockIdx.x]];
+// partial[wg][r][j] = sum over this workgroup's rows i >= ne of L[i,j] x[var(i), r]
+__global__ __launch_bounds__(kThreads) void k_bwd_gemv(const SolveNode *sn, const int *slots,
+                                                       const int *wg_off, int nlvl,
+                                                       const double *x, int ldx, int nrhs) {
+  __shared__ double red[kThreads / 64][kBlk];
+  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
+  const SolveNode &P = sn[slots[li]];
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = P.m, n = P.n, ne = P.nelim;
+  double *part = P.P;  // count x nrhs x ne partial sums
   for (int r = 0; r < nrhs; ++r) {
-    double *X = P.X + (size_t)r * m;
-    double *xr = x + (size_t)r * ldx;
-    for (int i = tid; i < m; i += kThreads)
-      X[i] = (i < n) ? xr[P.perm[i]] : xr[P.rows[i - P.nd]];
-    __syncthreads();
-    // backward sweep over 64-column blocks, last block first
-    const int nb = (ne + kBlk - 1) / kBlk;
-    for (int b = nb - 1; b >= 0; --b) {
-      const int j0 = b * kBlk, j1 = min(j0 + kBlk, ne);
-      const int w = j1 - j0;
-      // X[j] -= L[j1:m, j]^T X[j1:m] for j in block
+    const double *xr = x + (size_t)r * ldx;
+    for (int j0 = 0; j0 < ne; j0 += kBlk) {
+      const int wb = min(kBlk, ne - j0);
       double acc[kBlk];
 #pragma unroll
       for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
-      for (int i = j1 + tid; i < m; i += kThreads) {
-        double xi = X[i];
+      for (int i = ne + local * kThreads + tid; i < m; i += count * kThreads) {
+        double xi = (i < n) ? xr[P.perm[i]] : xr[P.rows[i - P.nd]];
+        const double *lr = P.L + (size_t)j0 * P.ldl + i;
 #pragma unroll
         for (int t = 0; t < kBlk; ++t)
-          if (t < w) acc[t] += P.L[(size_t)(j0 + t) * P.ldl + i] * xi;
+          if (t < wb) acc[t] += lr[(size_t)t * P.ldl] * xi;
+      }
+#pragma unroll
+      for (int t = 0; t < kBlk; ++t) {
+        double v = acc[t];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[wave][t] = v;
+      }
+      __syncthreads();
+      if (tid < wb) {
+        double v = 0.0;
+        for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
+        part[((size_t)local * nrhs + r) * ne + j0 + tid] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const int *slots,
+                                                      const int *wg_off, double *x, int ldx,
+                                                      int nrhs) {
+  __shared__ double lb[kBlk][kBlk + 1];
+  __shared__ double part[kThreads / 64][kBlk];
+  __shared__ double xb[kBlk];
+  const int li = blockIdx.x;
+  const SolveNode &P = sn[slots[li]];
+  const int count = wg_off[li + 1] - wg_off[li];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = P.m, ne = P.nelim;
+  const double *partial = P.P;
+  for (int r = 0; r < nrhs; ++r) {
+    double *X = P.X + (size_t)r * m;
+    double *xr = x + (size_t)r * ldx;
+    // X[j] = x[perm[j]] - sum of the gemv partials (rows >= ne)
+    for (int j = tid; j < ne; j += kThreads) {
+      double v = xr[P.perm[j]];
+      for (int g = 0; g < count; ++g) v -= partial[((size_t)g * nrhs + r) * ne + j];
+      X[j] = v;
+    }
+    __syncthreads();
+    // L11^T solve, last 64-column block first
+    const int nb = (ne + kBlk - 1) / kBlk;
+    for (int b = nb - 1; b >= 0; --b) {
+      const int j0 = b * kBlk, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+      // X[j] -= L[j1:ne, j]^T X[j1:ne] for j in the block
+      double acc[kBlk];
+#pragma unroll
+      for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
+      for (int i = j1 + tid; i < ne; i += kThreads) {
+        double xi = X[i];
+        const double *lr = P.L + (size_t)j0 * P.ldl + i;
+#pragma unroll
+        for (int t = 0; t < kBlk; ++t)
+          if (t < wb) acc[t] += lr[(size_t)t * P.ldl] * xi;
       }
 #pragma unroll
       for (int t = 0; t < kBlk; ++t) {
@@ -134,21 +254,24 @@ __global__ __launch_bounds__(kThreads) void k_bwd(const SolveNode *sn, const int
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) part[wave][t] = v;
       }
+      for (int idx = tid; idx < wb * wb; idx += kThreads) {
+        int rr = idx % wb, cc = idx / wb;
+        lb[rr][cc] = (rr > cc) ? P.L[(size_t)(j0 + cc) * P.ldl + j0 + rr] : 0.0;
+      }
       __syncthreads();
-      if (tid < w) {
+      if (tid < wb) {
         double v = X[j0 + tid];
         for (int q = 0; q < kThreads / 64; ++q) v -= part[q][tid];
         xb[tid] = v;
       }
       __syncthreads();
-      // unit upper solve L_bb^T within the block (one wave)
       if (tid < 64) {
-        double v = (tid < w) ? xb[tid] : 0.0;
-        for (int j = w - 1; j >= 0; --j) {
+        double v = (tid < wb) ? xb[tid] : 0.0;
+        for (int j = wb - 1; j >= 0; --j) {
           double yj = __shfl(v, j);
-          if (tid < j) v -= P.L[(size_t)(j0 + tid) * P.ldl + j0 + j] * yj;
+          if (tid < j) v -= lb[j][tid] * yj;
         }
-        if (tid < w) X[j0 + tid] = v;
+        if (tid < wb) X[j0 + tid] = v;
       }
       __syncthreads();
     }
@@ -162,10 +285,14 @@ void launch_ipos(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int
     hipLaunchKernelGGL(k_ipos, dim3(nlvl), dim3(256), 0, st, nodes, slots, scratch);
 }
 void launch_fwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                const int *child_list, double *x, int ldx, int nrhs) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_fwd, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
-                       ldx, nrhs);
+                const int *child_list, double *x, int ldx, int nrhs, const int *gemv_off,
+                int gemv_wg) {
+  if (nlvl <= 0) return;
+  hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
+                     ldx, nrhs);
+  if (gemv_wg > 0)
+    hipLaunchKernelGGL(k_fwd_gemv, dim3(gemv_wg), dim3(kThreads), 0, st, sn, slots, gemv_off,
+                       nlvl, nrhs);
 }
 void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, int ldx,
                    int nrhs) {
@@ -173,9 +300,13 @@ void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, i
     hipLaunchKernelGGL(k_dsolve, dim3(nslots), dim3(64), 0, st, sn, nslots, x, ldx, nrhs);
 }
 void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots, double *x,
-                int ldx, int nrhs) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_bwd, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, x, ldx, nrhs);
+                int ldx, int nrhs, const int *gemv_off, int gemv_wg) {
+  if (nlvl <= 0) return;
+  if (gemv_wg > 0)
+    hipLaunchKernelGGL(k_bwd_gemv, dim3(gemv_wg), dim3(kThreads), 0, st, sn, slots, gemv_off,
+                       nlvl, x, ldx, nrhs);
+  hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, gemv_off, x, ldx,
+                     nrhs);
 }
 
 }  // namespace ssids_hip
