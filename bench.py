@@ -67,32 +67,32 @@ def main():
     if not _lib.LIB_PATH.exists():
         from spral_amd import build
         build.build(verbose=False)
-    import torch
+    from spral_amd import devmem
     L = _lib.lib()
+    sync = devmem.synchronize
     g = args.grid
     m = matrices.laplace3d(g, diag=6.0 - 0.3)
     opts = ssids.SSIDSOptions(ordering=3)
     ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, grid=(g, g, g))
     assert ainf.flag >= 0, ainf
     flops = ak.num_flops
-    dev = torch.device("cuda:0")
-    d_val = torch.from_numpy(m.val).to(dev)
+    d_val = devmem.DeviceArray.from_host(m.val)
     rng = np.random.Generator(np.random.PCG64(20261202 + 2))
     x_true = rng.uniform(-1, 1, m.n)
     b = m.full @ x_true
-    b2 = torch.from_numpy(np.ascontiguousarray(b[ak.invp])).to(dev)
-    d_x = torch.empty_like(b2)
+    b2 = devmem.DeviceArray.from_host(b[ak.invp])
+    d_x = devmem.DeviceArray(m.n)
     copt = opts.to_c()
     st = _lib.Stats()
-    torch.cuda.synchronize()
-    num = L.spral_ssids_hip_factor_device(False, ak.symbolic, C.c_void_p(d_val.data_ptr()),
+    sync()
+    num = L.spral_ssids_hip_factor_device(False, ak.symbolic, d_val.ptr,
                                           m.val.size, None, C.byref(copt), C.byref(st))
     assert num, f"factor failed flag={st.flag}"
 
     def refactor(timing=0):
         copt.timing = timing
         s = _lib.Stats()
-        f = L.spral_ssids_hip_refactor_device(num, C.c_void_p(d_val.data_ptr()), None,
+        f = L.spral_ssids_hip_refactor_device(num, d_val.ptr, None,
                                               C.byref(copt), C.byref(s))
         assert f >= 0, f
         return s
@@ -102,43 +102,43 @@ def main():
 
     def solve():
         # FP64 solve with refinement to the 1e-14 backward-error bar
-        f = L.spral_ssids_hip_solve_refine_device(num, 1, C.c_void_p(b2.data_ptr()),
-                                                  C.c_void_p(d_x.data_ptr()), m.n, 3, 1e-14,
+        f = L.spral_ssids_hip_solve_refine_device(num, 1, b2.ptr,
+                                                  d_x.ptr, m.n, 3, 1e-14,
                                                   bwd_dev, C.byref(its), None)
         assert f == 0, f
 
     for _ in range(args.warmup):
         refactor()
         solve()
-    torch.cuda.synchronize()
+    sync()
     fac_ms, sol_ms = [], []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
         s = refactor()
-        torch.cuda.synchronize()
+        sync()
         t1 = time.perf_counter()
         solve()
-        torch.cuda.synchronize()
+        sync()
         t2 = time.perf_counter()
         fac_ms.append((t1 - t0) * 1e3)
         sol_ms.append((t2 - t1) * 1e3)
-    torch.cuda.synchronize()
+    sync()
     wall = time.perf_counter() - t_all
     # solve latency samples on the resident factors
     lat = []
     for _ in range(args.solves):
         t0 = time.perf_counter()
         solve()
-        torch.cuda.synchronize()
+        sync()
         lat.append((time.perf_counter() - t0) * 1e3)
-    x = d_x.cpu().numpy()
+    x = d_x.download()
     xo = np.empty_like(x)
     xo[ak.invp] = x
     bwd = float(matrices.bwd_error(m.full, xo, b)[0])
     # profiled factorization: events around every Schur-update launch
     sp = refactor(timing=1)
-    torch.cuda.synchronize()
+    sync()
     k_s = sp.kernel_ms * 1e-3
     tflops = sp.kernel_flops / k_s / 1e12
     gbs = sp.kernel_bytes / k_s / 1e9

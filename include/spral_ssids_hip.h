@@ -69,6 +69,8 @@ struct spral_ssids_hip_stats {
   double kernel_bytes;     /* algorithmic HBM bytes of those launches       */
   int kernel_launches;     /* number of gather launches timed               */
   int64_t pool_peak_bytes; /* peak device pool use                          */
+  int static_plan;         /* 1: ran on the static plan (no per-level host
+                              round trip), 0: level by level              */
 };
 
 /* Fill *opt with the reference defaults (src/ssids/datatypes.f90:188-284). */

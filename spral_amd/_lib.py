@@ -54,6 +54,7 @@ class Stats(C.Structure):
         ("kernel_bytes", C.c_double),
         ("kernel_launches", C.c_int),
         ("pool_peak_bytes", C.c_int64),
+        ("static_plan", C.c_int),
     ]
 
 

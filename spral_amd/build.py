@@ -1,7 +1,8 @@
 """Build libspral_ssids_hip.so in-tree with hipcc for gfx950.
 
-Every .hip and .cpp file under spral_amd/csrc is compiled with hipcc (host
-files too, so one toolchain produces the whole C-ABI library).  Objects are
+Every .hip and .cpp file under spral_amd/csrc is compiled with hipcc for
+gfx950 (host files too: they share __host__ __device__ helpers with the
+kernels, and one toolchain produces the whole C-ABI library).  Objects are
 cached by source mtime under spral_amd/_build; the link always runs.
 """
 import os
@@ -41,9 +42,7 @@ def build(force: bool = False, verbose: bool = True) -> Path:
         if (not force and obj.exists()
                 and obj.stat().st_mtime >= max(src.stat().st_mtime, hdr_mtime)):
             continue
-        cmd = [HIPCC, *COMMON]
-        if src.suffix == ".hip":
-            cmd += DEVICE + ["-x", "hip"]
+        cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip"]
         cmd += ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(str(c) for c in cmd), flush=True)
@@ -69,7 +68,7 @@ def build_probe() -> None:
     """Compile the test-only MFMA layout probe (tests/hip/libprobe.so)."""
     src = HERE.parent / "tests" / "hip" / "probe.hip"
     if src.exists():
-        _run([HIPCC, "-O2", "-fPIC", "-shared", *DEVICE, "-x", "hip", src,
+        _run([HIPCC, "-O2", "-fPIC", "-shared", *DEVICE, f"-I{CSRC}", "-x", "hip", src,
               "-o", src.with_name("libprobe.so")])
 
 

@@ -41,6 +41,8 @@ struct NodeDev {
   int doff;         // offset of this node's delayed columns in parent
   int nsteps;       // APP steps this node needs (ceil(n/kBlk))
   int is_root;
+  int ndcap;        // delayed columns the node's buffers can take (static plan)
+  int skip;         // 1: delays exceeded ndcap here or below; node not factored
   // factorization state (device-written)
   int p, q, w, npass;
   int nelim;
@@ -63,6 +65,75 @@ struct SolveNode {
   double *P;         // backward GEMV partial sums (gemv workgroups x nrhs x nelim)
   int m, n, nd, sn, nelim, ldl, doff, cbeg, cend;
 };
+
+__host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ inline int idiv_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Per-level launch tables.  A level whose fronts need at most S APP steps has
+// 2S+3 tables, each an exclusive prefix over the level's fronts of the
+// workgroups one launch gives each front (last entry = total):
+//   0          gather: lower 64x64 tiles of the front
+//   1..S       apply, step t-1: t-1 chunks of earlier rows + rows below
+//   S+1        failed-column swap
+//   S+2..2S+1  trailing panel update, step t-S-2
+//   2S+2       contribution update of a root's TPP pivots
+__host__ __device__ inline int level_tables(int S) { return 2 * S + 3; }
+__host__ __device__ inline int level_table_count(int t, int S, int m, int n, int nsteps,
+                                                 int is_root) {
+  if (t == 0) {
+    const int T = idiv_up(m, kBlk);
+    return T * (T + 1) / 2;
+  }
+  if (t <= S) {
+    const int stp = t - 1;
+    return stp < nsteps ? stp + imax(1, idiv_up(m, kThreads)) : 0;
+  }
+  if (t == S + 1) return nsteps > 1 ? imax(1, idiv_up(2LL * kBlk * m, kThreads * 8)) : 0;
+  if (t <= 2 * S + 1) {
+    const int stp = t - S - 2;
+    if (stp >= nsteps) return 0;
+    const int t0 = imin(n, (stp + 1) * kBlk);
+    const int TI = idiv_up(m - t0, kBlk), TJ = idiv_up(n - t0, kBlk);
+    const int nt = TJ * TI - TJ * (TJ - 1) / 2;
+    return imax(1, imin(nt, 4096));
+  }
+  if (!is_root || m <= n) return 0;
+  const int T = idiv_up(m - n, kBlk);
+  return imax(1, imin(T * (T + 1) / 2, 4096));
+}
+
+// Wave-wide maxima with DPP lane permutations inside each row of 16 lanes
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// then the four row results read as scalars.  Every lane of the wave must be
+// active; the result is wave-uniform.
+__device__ inline unsigned dpp_max_u32(unsigned v, int ctrl) {
+  unsigned t;
+  switch (ctrl) {
+    case 0: t = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false); break;
+    case 1: t = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false); break;
+    case 2: t = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false); break;
+    default: t = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xf, 0xf, false); break;
+  }
+  return t > v ? t : v;
+}
+__device__ inline unsigned wave_max_u32(unsigned v) {
+  v = dpp_max_u32(v, 0);
+  v = dpp_max_u32(v, 1);
+  v = dpp_max_u32(v, 2);
+  v = dpp_max_u32(v, 3);
+  const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)v, 0);
+  const unsigned r1 = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+  const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)v, 32);
+  const unsigned r3 = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+  const unsigned a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
+  return a > b ? a : b;
+}
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+  const unsigned hi = wave_max_u32((unsigned)(v >> 32));
+  const unsigned lo = wave_max_u32(((unsigned)(v >> 32) == hi) ? (unsigned)v : 0u);
+  return ((unsigned long long)hi << 32) | lo;
+}
 
 struct FactorParams {
   double u, small;

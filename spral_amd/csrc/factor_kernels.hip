@@ -31,7 +31,10 @@ namespace ssids_hip {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
+// front of workgroup b in a level table, or -1 past the table's total (grids
+// sized for the plan's capacities may be larger than the level needs)
 __device__ inline int find_node(const int *wg_off, int nlvl, int b) {
+  if (b >= wg_off[nlvl]) return -1;
   int lo = 0, hi = nlvl;  // wg_off[lo] <= b < wg_off[hi]
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
@@ -59,20 +62,36 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block factorization, one wavefront per front, block resident in
-// LDS.  Lane j updates column j of the symmetric w x w block; pivots are
-// tracked in a wave-uniform bit mask (no physical swaps), only live rows are
-// updated, and the L column of an eliminated pivot is stored in that pivot's
-// (now dead) column.  The elimination order lp[] permutes the block on write
-// back.  Pivot choice: complete pivoting over the live block with the rules
-// of src/ssids/cpu/kernels/block_ldlt.hxx:289-414 (max entry on the diagonal:
-// 1x1; else the 2x2 test, else the larger diagonal as 1x1); the a posteriori
-// test of k_apply then checks |l| <= 1/u over the full column.
+// Diagonal block factorization, one wavefront per front.  Lane j keeps column
+// j of the symmetric w x w block in registers (a[0..63]); pivots are tracked
+// in a wave-uniform bit mask (no physical swaps).  Per pivot step:
+//  * search: every live lane scans its live rows for max |a(i,j)| (rows in
+//    ascending order, first maximum kept); the wave takes the maximum value,
+//    then the smallest column holding it (ballot), then that column's row;
+//  * the pivot column(s) are broadcast through LDS by the lanes owning them;
+//  * rank-1 / rank-2 update of every column in registers;
+//  * the L column(s) of the step go to LDS, indexed by elimination step.
+// Pivot rules of src/ssids/cpu/kernels/block_ldlt.hxx:289-414: maximum on the
+// diagonal -> 1x1; else the 2x2 test of block_ldlt.hxx:213-218, else the
+// larger diagonal as 1x1.  A maximum below `small` makes every remaining
+// index a zero pivot.  The a posteriori test of k_apply then checks
+// |l| <= 1/u over the full column.  The elimination order lp[] permutes the
+// block on write back.
 // ---------------------------------------------------------------------------
+// value held by lane 0 (wave-uniform by construction), as a scalar
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uni(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+
 __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slots, int step,
                                              FactorParams fp) {
-  __shared__ double t[kBlk][kBlk + 1];   // t[row][col], symmetric
-  __shared__ double pc1[kBlk], pc2[kBlk];
+  __shared__ double t[kBlk][kBlk + 1];  // load staging, then L by step: t[k][row]
+  __shared__ double cs[2][kBlk];        // broadcast pivot columns
   __shared__ double dinv[2 * kBlk + 2];
   __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
   const int s = lvl_slots[blockIdx.x];
@@ -98,83 +117,66 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
   if (w == 0) return;
   const int ldl = N.ldl;
   double *L = N.L;
-  for (int idx = lane; idx < w * w; idx += 64) {
-    int r = idx % w, c = idx / w;
+  for (int idx = lane; idx < kBlk * kBlk; idx += 64) {
+    int r = idx % kBlk, c = idx / kBlk;
     if (r >= c) {
-      double v = L[(size_t)(p + c) * ldl + p + r];
+      double v = (r < w) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
       t[r][c] = v;
       t[c][r] = v;
-      N.dbak[r + c * kBlk] = v;
-      N.dbak[c + r * kBlk] = v;
+      if (r < w) {
+        N.dbak[r + c * kBlk] = v;
+        N.dbak[c + r * kBlk] = v;
+      }
     }
   }
   N.colmax[lane] = 0ull;
   __syncthreads();
-  double dg = (lane < w) ? t[lane][lane] : 0.0;  // a(lane, lane), kept current
+  double a[kBlk];
+#pragma unroll
+  for (int i = 0; i < kBlk; ++i) a[i] = t[i][lane];
+  __syncthreads();
   unsigned long long elim = (w < 64) ? (~0ull << w) : 0ull;  // eliminated / absent
   int k = 0;
   while (k < w) {
-    // ---- complete pivoting: max |a(i,j)|, i >= j, over the live block
     const bool live = !((elim >> lane) & 1ull);
-    double bv = -1.0, bs = 0.0;
-    int br = 64;
-    if (live) {
-      for (int i = lane; i < w; ++i) {
+    // ---- search this column's live rows: four interleaved running maxima
+    // (independent chains), each keeping its first maximal row; then the
+    // largest, ties to the smaller row.  NaN never wins; an all-zero column
+    // keeps br = 64.
+    double bq[4] = {0.0, 0.0, 0.0, 0.0};
+    int rq[4] = {64, 64, 64, 64};
+#pragma unroll
+    for (int c = 0; c < kBlk / 8; ++c) {
+      if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
+#pragma unroll
+      for (int ii = 0; ii < 8; ++ii) {
+        const int i = 8 * c + ii;
         if ((elim >> i) & 1ull) continue;
-        double v = t[i][lane];
-        if (fabs(v) > bv) {  // rows ascend: first maximal row kept
-          bv = fabs(v);
-          bs = v;
-          br = i;
+        if (fabs(a[i]) > bq[ii & 3]) {
+          bq[ii & 3] = fabs(a[i]);
+          rq[ii & 3] = i;
         }
       }
     }
-    int bc = live ? lane : 64;
-    for (int off = 32; off > 0; off >>= 1) {
-      double ov = __shfl_xor(bv, off), os = __shfl_xor(bs, off);
-      int orr = __shfl_xor(br, off), oc = __shfl_xor(bc, off);
-      bool take = (ov > bv) || (ov == bv && (oc < bc || (oc == bc && orr < br)));
-      if (take) {
-        bv = ov;
-        bs = os;
-        br = orr;
-        bc = oc;
+    double bv = bq[0];
+    int br = rq[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+      if (bq[q] > bv || (bq[q] == bv && rq[q] < br)) {
+        bv = bq[q];
+        br = rq[q];
       }
-    }
-    int size, p1, p2 = -1;
-    double d11 = 0.0, d21 = 0.0, d22 = 0.0;
-    double x = 0.0, y = 0.0;
-    if (!(bv >= fp.small)) {
-      size = 0;  // everything left is (near) zero
-      p1 = __ffsll((long long)~elim) - 1;
-    } else if (br == bc) {
-      size = 1;
-      p1 = br;
-    } else {
-      // 2x2 test of src/ssids/cpu/kernels/block_ldlt.hxx:213-218 on (bc, br)
-      const double a11 = __shfl(dg, bc), a22 = __shfl(dg, br), a21 = bs;
-      const double detscale = 1.0 / fabs(a21);
-      const double detpiv = (a11 * detscale) * a22 - fabs(a21);
-      if (fabs(detpiv) >= fabs(a21) / 2) {
-        size = 2;
-        p1 = bc;
-        p2 = br;
-        d11 = (a22 * detscale) / detpiv;
-        d21 = (-a21 * detscale) / detpiv;
-        d22 = (a11 * detscale) / detpiv;
-      } else {
-        size = 1;
-        p1 = (fabs(a11) > fabs(a22)) ? bc : br;
-      }
-    }
-    x = t[lane][p1];
-    if (size == 2) y = t[lane][p2];
-    if (size == 0) {
+    // ---- wave arg-max: value, then smallest column, then (per column) row
+    const unsigned long long key = live ? (unsigned long long)__double_as_longlong(bv) : 0ull;
+    const unsigned long long kmax = wave_max_u64(key);
+    const unsigned long long hit = __ballot(live && key == kmax);
+    const double vmax = uni(__longlong_as_double((long long)kmax));
+    if (!(vmax >= fp.small) || hit == 0ull) {
       // zero pivots for every remaining index, in increasing index order
       int kk = k;
       for (int i = 0; i < w; ++i) {
         if ((elim >> i) & 1ull) continue;
-        if (live) t[lane][i] = 0.0;  // zero L column of pivot i
+        t[kk][lane] = 0.0;
         if (lane == 0) {
           lp[kk] = i;
           kind[kk] = 0;
@@ -187,27 +189,55 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
         N.nzero += w - k;
         if (!fp.action) N.flag = -5;
       }
-      __syncthreads();
       k = w;
       break;
     }
-    const double c1 = x;
-    const double c2 = (size == 2) ? y : 0.0;
-    pc1[lane] = c1;
-    pc2[lane] = c2;
+    const int bc = uni(__ffsll((long long)hit) - 1);
+    const int brow = __builtin_amdgcn_readlane(br, bc);
+    // ---- owners broadcast their columns: cs[0] <- column bc, cs[1] <- column brow
+    if (lane == bc || lane == brow) {
+      double *dst = cs[lane == bc ? 0 : 1];
+#pragma unroll
+      for (int i = 0; i < kBlk; ++i) dst[i] = a[i];
+    }
     __syncthreads();
-    if (size == 1) {
-      const double di = 1.0 / __shfl(dg, p1);
-      elim |= 1ull << p1;
-      const bool upd = !((elim >> lane) & 1ull);  // live and not the pivot
-      const double lj = c1 * di;                 // L(lane, k)
-      if (upd) {
-        for (int i = 0; i < w; ++i)
-          if (!((elim >> i) & 1ull)) t[i][lane] -= pc1[i] * lj;
-        dg -= c1 * lj;
+    int size, p1, p2 = -1;
+    const double *c1p, *c2p = cs[1];
+    double d11 = 0.0, d21 = 0.0, d22 = 0.0;
+    if (brow == bc) {
+      size = 1;
+      p1 = bc;
+      c1p = cs[0];
+    } else {
+      const double a11 = uni(cs[0][bc]), a22 = uni(cs[1][brow]), a21 = uni(cs[0][brow]);
+      const double detscale = 1.0 / fabs(a21);
+      const double detpiv = (a11 * detscale) * a22 - fabs(a21);
+      if (fabs(detpiv) >= fabs(a21) / 2) {
+        size = 2;
+        p1 = bc;
+        p2 = brow;
+        c1p = cs[0];
+        d11 = (a22 * detscale) / detpiv;
+        d21 = (-a21 * detscale) / detpiv;
+        d22 = (a11 * detscale) / detpiv;
+      } else {
+        size = 1;
+        const bool first = fabs(a11) > fabs(a22);
+        p1 = first ? bc : brow;
+        c1p = first ? cs[0] : cs[1];
       }
-      __syncthreads();
-      if (upd) t[lane][p1] = lj;  // L column stored in the dead column p1
+    }
+    if (size == 1) {
+      const double di = 1.0 / c1p[p1];
+      elim |= 1ull << p1;
+      const double lj = c1p[lane] * di;  // L(lane, k)
+#pragma unroll
+      for (int c = 0; c < kBlk / 8; ++c) {
+        if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
+#pragma unroll
+        for (int ii = 0; ii < 8; ++ii) a[8 * c + ii] -= c1p[8 * c + ii] * lj;
+      }
+      t[k][lane] = lj;
       if (lane == 0) {
         dinv[2 * k] = di;
         dinv[2 * k + 1] = 0.0;
@@ -216,18 +246,17 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
       }
     } else {
       elim |= (1ull << p1) | (1ull << p2);
-      const bool upd = !((elim >> lane) & 1ull);
-      const double l1 = d11 * c1 + d21 * c2, l2 = d21 * c1 + d22 * c2;
-      if (upd) {
-        for (int i = 0; i < w; ++i)
-          if (!((elim >> i) & 1ull)) t[i][lane] -= pc1[i] * l1 + pc2[i] * l2;
-        dg -= c1 * l1 + c2 * l2;
+      const double x = c1p[lane], y = c2p[lane];
+      const double l1 = d11 * x + d21 * y, l2 = d21 * x + d22 * y;
+#pragma unroll
+      for (int c = 0; c < kBlk / 8; ++c) {
+        if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
+#pragma unroll
+        for (int ii = 0; ii < 8; ++ii)
+          a[8 * c + ii] -= c1p[8 * c + ii] * l1 + c2p[8 * c + ii] * l2;
       }
-      __syncthreads();
-      if (upd) {
-        t[lane][p1] = l1;
-        t[lane][p2] = l2;
-      }
+      t[k][lane] = l1;
+      t[k + 1][lane] = l2;
       if (lane == 0) {
         dinv[2 * k] = d11;
         dinv[2 * k + 1] = d21;
@@ -253,7 +282,7 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
     double v;
     if (nr == kk) v = (kind[kk] == 0) ? 0.0 : 1.0;
     else if (nr == kk + 1 && kind[kk] == 2) v = 0.0;
-    else v = t[r][lp[kk]];
+    else v = t[kk][r];
     L[(size_t)(p + kk) * ldl + p + nr] = v;
   }
   for (int j = lane; j < w; j += 64) {
@@ -278,6 +307,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
   __shared__ int lps[kBlk];
   __shared__ double cmax[kThreads / 64][kBlk];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
   NodeDev &N = nodes[lvl_slots[li]];
   if (step >= N.nsteps) return;
   const int p = N.p, w = N.w;
@@ -455,6 +485,7 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
   __shared__ double As[kTile][kBlk + 1];
   __shared__ double Bs[kTile][kBlk + 1];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
   NodeDev &N = nodes[lvl_slots[li]];
   const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -598,14 +629,14 @@ __global__ void k_dprep(NodeDev *nodes, const int *lvl_slots) {
 // Schur update of the children's contribution blocks).  A workgroup owns a
 // 64x64 tile of one parent front and builds it in LDS:
 //   for each child in order, the child's trailing index ranges that map into
-//   the tile are found by binary search on the monotone child->parent map and
+//   the tile come from the k_maptab tables and their parent positions are
 //   staged in LDS; failed (delayed) columns are copied from the child panel;
-//   contribution columns are computed as C_c - L21 (L21 D)^T on MFMA
-//   (K = the child's nelim, in 64-wide chunks) and added through the maps;
+//   contribution columns are added as C_c (read down its columns) minus
+//   L21 (D L21)^T on MFMA (K = the child's nelim; operands straight from
+//   global memory in the fragment layout), both through the maps;
 //   then the tile's A entries are added and the tile is stored once.
 // ---------------------------------------------------------------------------
 constexpr int kGT = 64;
-constexpr int kGK = 32;  // K chunk of the contribution GEMM
 
 // Per level, for every child of the level's fronts: the parent position of
 // each trailing index (pmap) and, for each 64-row tile of the parent, the
@@ -613,6 +644,7 @@ constexpr int kGK = 32;  // K chunk of the contribution GEMM
 __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *lvl_slots,
                                                      const int *child_list) {
   const NodeDev &P = nodes[lvl_slots[blockIdx.x]];
+  if (P.skip) return;
   const int T = (P.m + kGT - 1) / kGT;
   int doff = 0;
   for (int ci = P.cbeg; ci < P.cend; ++ci) {
@@ -641,10 +673,9 @@ __global__ __launch_bounds__(kThreads) void k_gather(
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
     const double *scaling) {
   __shared__ double acc[kGT][kGT + 1];
-  __shared__ double As[kGT][kGK + 1];
-  __shared__ double Bs[kGT][kGK + 1];
   __shared__ int mr[kGT], mc[kGT];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
   const int s = lvl_slots[li];
   NodeDev &P = nodes[s];
   const int m = P.m, n = P.n, nd = P.nd;
@@ -681,12 +712,8 @@ __global__ __launch_bounds__(kThreads) void k_gather(
   }
   __syncthreads();
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  int doff = 0;
   for (int ci = P.cbeg; ci < P.cend; ++ci) {
     const NodeDev &c = nodes[child_list[ci]];
-    const int cdoff = doff;
-    doff += c.n - c.nelim;
-    (void)cdoff;
     const int kb = c.nelim, ke = c.m;
     if (ke <= kb) continue;
     const int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
@@ -707,66 +734,64 @@ __global__ __launch_bounds__(kThreads) void k_gather(
         if (kr >= k) acc[mr[a]][mc[b]] += c.L[(size_t)k * c.ldl + kr];
       }
     }
-    // contribution columns: C_c - L21 (L21 D)^T
+    // contribution columns: C_c - L21 (D L21)^T
     const int kc0 = max(k0, c.n);
     const int rr0 = max(kr0, c.n);
     if (k1 > kc0 && kr1 > rr0) {
       const int nrow = kr1 - rr0, ncol = k1 - kc0;
+      // C_c, lower part, read down its columns (coalesced)
+      for (int idx = tid; idx < nrow * ncol; idx += kThreads) {
+        const int a = idx % nrow, b = idx / nrow;
+        const int kr = rr0 + a, k = kc0 + b;
+        if (kr >= k) acc[mr[kr - kr0]][mc[k - k0]] += c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
+      }
+      // L21 (D L21)^T on MFMA: each wave a 32x32 quadrant, operands loaded in
+      // the 16x16x4 fragment layout (lane i+16k holds row i, pivot k)
       const int K = c.nelim;
+      const int ri = lane & 15, kq = lane >> 4;
+      const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
+      const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
+      const double *La = c.L + rr0, *Lb = c.L + kc0;
       double4_t g[2][2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      for (int t0 = 0; t0 < K; t0 += kGK) {
-        const int kn = min(kGK, K - t0);
-        const int Kp = (kn + 3) & ~3;
-        __syncthreads();
-        for (int idx = tid; idx < kGT * Kp; idx += kThreads) {
-          int i = idx % kGT, kk = idx / kGT;
-          double av = 0.0, bv = 0.0;
-          if (kk < kn) {
-            int t = t0 + kk;
-            if (i < nrow) av = c.L[(size_t)t * c.ldl + rr0 + i];
-            if (i < ncol) {
-              int row = kc0 + i;
-              double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
-              int off = (int)c.Dd[3 * t + 2];
-              bv = c.L[(size_t)t * c.ldl + row] * e;
-              if (off != 0) bv += c.L[(size_t)(t + off) * c.ldl + row] * f;
-            }
-          }
-          As[i][kk] = av;
-          Bs[i][kk] = bv;
-        }
-        __syncthreads();
-        for (int kk = 0; kk < Kp; kk += 4) {
-          const int kl = kk + (lane >> 4);
-#pragma unroll
-          for (int bi = 0; bi < 2; ++bi) {
-            double av = As[wr + bi * 16 + (lane & 15)][kl];
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj) {
-              double bv = Bs[wc + bj * 16 + (lane & 15)][kl];
-              g[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, g[bi][bj], 0, 0, 0);
-            }
+#pragma unroll 4
+      for (int t0 = 0; t0 < K; t0 += 4) {
+        const int t = t0 + kq;
+        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+        if (t < K) {
+          const size_t o = (size_t)t * c.ldl;
+          if (va0) a0 = La[o + ra0];
+          if (va1) a1 = La[o + ra1];
+          const double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
+          const int off = (int)c.Dd[3 * t + 2];
+          if (vb0) b0 = Lb[o + cb0] * e;
+          if (vb1) b1 = Lb[o + cb1] * e;
+          if (off != 0) {
+            const size_t op = (size_t)(t + off) * c.ldl;
+            if (vb0) b0 += Lb[op + cb0] * f;
+            if (vb1) b1 += Lb[op + cb1] * f;
           }
         }
+        g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, g[0][0], 0, 0, 0);
+        g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, g[0][1], 0, 0, 0);
+        g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, g[1][0], 0, 0, 0);
+        g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, g[1][1], 0, 0, 0);
       }
+      __syncthreads();  // the C_c pass has finished with acc
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
           for (int rg = 0; rg < 4; ++rg) {
-            int i = wr + bi * 16 + (lane >> 4) + 4 * rg;
-            int j = wc + bj * 16 + (lane & 15);
+            const int i = wr + bi * 16 + kq + 4 * rg;
+            const int j = wc + bj * 16 + ri;
             if (i < nrow && j < ncol) {
-              int kr = rr0 + i, k = kc0 + j;
-              if (kr >= k) {
-                double cv = c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
-                acc[mr[kr - kr0]][mc[k - k0]] += cv - g[bi][bj][rg];
-              }
+              const int kr = rr0 + i, k = kc0 + j;
+              if (kr >= k) acc[mr[kr - kr0]][mc[k - k0]] -= g[bi][bj][rg];
             }
           }
     }
@@ -829,6 +854,7 @@ __global__ __launch_bounds__(kThreads) void k_swap(NodeDev *nodes, const int *lv
                                                    const int *wg_off, int nlvl, int step,
                                                    int phase) {
   const int li = find_node(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
   NodeDev &N = nodes[lvl_slots[li]];
   if (step >= N.nsteps || N.w == 0) return;
   const int ns = swp_count(N);
@@ -1141,8 +1167,105 @@ __global__ void k_stats(const NodeDev *nodes, int nslots, int *counters) {
 }
 
 // ---------------------------------------------------------------------------
+// Level set-up for the static plan (one workgroup): sizes each front of the
+// level from its children's results -- nd = sum of the children's failed
+// columns, m = sm + nd, n = sn + nd -- and writes the level's launch tables
+// (exclusive prefix sums of level_table_count).  A front whose delays exceed
+// the capacity its buffers were planned with, or with a skipped child, is
+// skipped (m = n = 0, no work) and counted in *overflow when it is the first
+// on its path; the host then refactors with the per-level schedule.
+// Thread tid owns fronts tid, tid+1024, ... in both phases, so every front's
+// sizes are read back by the thread that wrote them.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_level_setup(NodeDev *nodes, int s0, int nl,
+                                                      const int *child_list, int pivot_tpp,
+                                                      int S, int *tab, int *overflow) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int li = tid; li < nl; li += 1024) {
+    NodeDev &D = nodes[s0 + li];
+    int nd = 0, poison = 0;
+    for (int k = D.cbeg; k < D.cend; ++k) {
+      const NodeDev &c = nodes[child_list[k]];
+      nd += c.n - c.nelim;
+      poison |= c.skip;
+    }
+    if (poison || nd > D.ndcap) {
+      D.skip = 1;
+      D.nd = 0;
+      D.m = 0;
+      D.n = 0;
+      D.nsteps = 0;
+      if (!poison) atomicAdd(overflow, 1);
+    } else {
+      D.skip = 0;
+      D.nd = nd;
+      D.m = D.sm + nd;
+      D.n = D.sn + nd;
+      D.nsteps = pivot_tpp ? 0 : (D.n + kBlk - 1) / kBlk;
+    }
+  }
+  const int ntab = level_tables(S);
+  for (int t = 0; t < ntab; ++t) {
+    int *out = tab + (size_t)t * (nl + 1);
+    int carry = 0;
+    for (int base = 0; base < nl; base += 1024) {
+      const int li = base + tid;
+      int v = 0;
+      if (li < nl) {
+        const NodeDev &D = nodes[s0 + li];
+        v = level_table_count(t, S, D.m, D.n, D.nsteps, D.is_root);
+      }
+      int x = v;  // inclusive scan within the wave
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) wsum[wv] = x;
+      __syncthreads();
+      int wpre = 0, tot = 0;
+      for (int w = 0; w < 16; ++w) {
+        const int sw = wsum[w];
+        if (w < wv) wpre += sw;
+        tot += sw;
+      }
+      if (li < nl) out[li] = carry + wpre + x - v;
+      carry += tot;
+      __syncthreads();
+    }
+    if (tid == 0) out[nl] = carry;
+  }
+}
+
+// the per-front results the host needs after a statically planned factor
+__global__ void k_pack_nodes(const NodeDev *nodes, int nn, int *out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nn) return;
+  const NodeDev &D = nodes[s];
+  int *o = out + 8 * (size_t)s;
+  o[0] = D.nd;
+  o[1] = D.m;
+  o[2] = D.n;
+  o[3] = D.nelim;
+  o[4] = D.nelim_app;
+  o[5] = D.flag;
+  o[6] = D.skip;
+  o[7] = D.nsteps;
+}
+
+// ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+void launch_level_setup(hipStream_t st, NodeDev *nodes, int s0, int nl, const int *child_list,
+                        int pivot_tpp, int S, int *tab, int *overflow) {
+  if (nl > 0)
+    hipLaunchKernelGGL(k_level_setup, dim3(1), dim3(1024), 0, st, nodes, s0, nl, child_list,
+                       pivot_tpp, S, tab, overflow);
+}
+void launch_pack_nodes(hipStream_t st, const NodeDev *nodes, int nn, int *out) {
+  if (nn > 0)
+    hipLaunchKernelGGL(k_pack_nodes, dim3((nn + 255) / 256), dim3(256), 0, st, nodes, nn, out);
+}
 void launch_gather(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, const int *child_list, const double *aval,
                    const int64_t *asrc, const int *arow, const double *scaling) {

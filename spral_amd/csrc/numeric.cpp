@@ -37,6 +37,8 @@ void launch_finalize(hipStream_t, int, NodeDev *, const int *);
 void launch_tpp(hipStream_t, int, NodeDev *, const int *, FactorParams);
 void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
+void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
+void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
 void launch_fwd(hipStream_t, int, const SolveNode *, const int *, const int *, double *, int,
                 int, const int *, int);
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
@@ -173,10 +175,41 @@ struct Symbolic {
   }
 };
 
+// Static factor plan: every front's buffers placed in the pool up front for a
+// delay capacity ndcap per front, so a factorization runs as one stream of
+// launches with no host round trip (device kernels size each level from the
+// children's results).  Temporaries (contribution blocks, workspaces) share
+// pool memory by lifetime: a child's block is reused by later levels once its
+// parent has been assembled, which stream order makes safe.
+struct Plan {
+  bool valid = false;
+  int pivot_tpp = -1;
+  std::vector<NodeDev> tmpl;               // slot-indexed static fields
+  NodeDev *d_tmpl = nullptr;
+  struct Level {
+    int maxsteps = 0;
+    bool multi = false;
+    size_t tab_off = 0;
+    std::vector<int> cap;                   // capacity workgroups per table
+  };
+  std::vector<Level> lv;
+  int *d_tabs = nullptr;
+  size_t tabs_cap = 0;
+  int *d_pack = nullptr;                   // 8 ints per front
+  std::vector<std::pair<void *, size_t>> allocs;
+  ~Plan() {
+    if (d_tmpl) (void)hipFree(d_tmpl);
+    if (d_tabs) (void)hipFree(d_tabs);
+    if (d_pack) (void)hipFree(d_pack);
+  }
+};
+
 struct Numeric {
   Symbolic *symb = nullptr;
   hipStream_t stream = nullptr;
   Pool pool;
+  Plan plan;
+  std::vector<int> nd_seen;               // per slot: most delays seen in a factor
   std::vector<NodeDev> nodes;             // slot-indexed host mirror
   NodeDev *d_nodes = nullptr;
   int *d_lvl = nullptr;                   // per-level scratch (slots + offsets)
@@ -408,6 +441,338 @@ static void ensure_lvl(Numeric &N, size_t ints) {
 
 static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+static hipEvent_t timed_event(hipStream_t s) {
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(e, s));
+  return e;
+}
+
+// Static fields of a front (everything but the delay-dependent sizes).
+static void node_static(const Symbolic &S, int sl, NodeDev &D) {
+  const int i = S.node_of[sl];
+  D.sm = S.sm[i];
+  D.sn = S.sn[i];
+  D.parent = S.parent[i] >= 0 ? S.slot[S.parent[i]] : -1;
+  D.cbeg = S.cbeg[i];
+  D.cend = S.cend[i];
+  D.is_root = (S.parent[i] < 0) ? 1 : 0;
+  D.rows = S.d_rows + S.row_off[i];
+  D.rmap = S.d_rmap + S.rmap_off[i];
+  D.acol = S.d_acol + S.acol_off[i];
+}
+
+// Buffers of a front with room for m rows / n fully-summed columns.
+// Persistent factors come from `keep`; the contribution block and the
+// workspace are returned through cb/wk for the caller to release.
+template <class Keep>
+static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int m, int n,
+                         int nsteps, Keep keep, std::pair<void *, size_t> &cb,
+                         std::pair<void *, size_t> &wk) {
+  const int i = S.node_of[sl];
+  D.ldl = std::max(m, 1);
+  D.ldc = D.sm - D.sn;
+  D.ldbak = std::max(m, 1);
+  D.L = (double *)keep((size_t)D.ldl * n * sizeof(double));
+  D.pmap = (int *)keep(std::max(m, 1) * sizeof(int));
+  D.D = (double *)keep(2 * (size_t)std::max(n, 1) * sizeof(double));
+  D.Dd = (double *)keep(3 * (size_t)std::max(n, 1) * sizeof(double));
+  D.perm = (int *)keep(std::max(n, 1) * sizeof(int));
+  D.perm0 = (int *)keep(std::max(n, 1) * sizeof(int));
+  D.ipos = (int *)keep(std::max(n, 1) * sizeof(int));
+  const size_t cbytes = (size_t)D.ldc * D.ldc * sizeof(double);
+  D.C = cbytes ? (double *)N.pool.alloc(cbytes) : nullptr;
+  cb = {D.C, cbytes};
+  size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
+  if (nsteps > 1) wb += (size_t)2 * kBlk * m * sizeof(double);
+  const size_t tabn = (size_t)(cdiv(m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
+  wb += tabn * sizeof(int);
+  char *w = (char *)N.pool.alloc(wb);
+  wk = {w, wb};
+  D.bak = (double *)w;
+  D.dbak = D.bak + (size_t)D.ldbak * kBlk;
+  D.swp = (nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
+  D.ctab = (int *)(D.dbak + kBlk * kBlk + ((nsteps > 1) ? (size_t)2 * kBlk * m : 0));
+}
+
+static void plan_release(Numeric &N) {
+  for (auto &a : N.plan.allocs) N.pool.release(a.first, a.second);
+  N.plan.allocs.clear();
+  N.plan.valid = false;
+}
+
+// Place every front for a delay capacity of
+//   ndcap = seen + ceil((multiplier - 1) (sn + seen)),
+// at most the children's fully-summed columns (seen = most delays observed
+// at this front in an earlier factorization, else 0).
+static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
+  Symbolic &S = *N.symb;
+  Plan &P = N.plan;
+  plan_release(N);
+  for (auto &p : N.persistent) N.pool.release(p.first, p.second);
+  N.persistent.clear();
+  const int nn = S.nnodes, nlev = (int)S.lvl_ptr.size() - 1;
+  if ((int)N.nd_seen.size() != nn) N.nd_seen.assign(nn, 0);
+  const double slack = std::max(0.0, multiplier - 1.0);
+  P.pivot_tpp = pivot_tpp;
+  P.tmpl.assign(nn, NodeDev{});
+  P.lv.assign(nlev, Plan::Level{});
+  std::vector<std::pair<void *, size_t>> cof(nn), wof(nn);
+  auto keep = [&](size_t b) {
+    void *p = N.pool.alloc(b);
+    P.allocs.push_back({p, b});
+    return p;
+  };
+  size_t tab_total = 0;
+  for (int L = 0; L < nlev; ++L) {
+    const int s0 = S.lvl_ptr[L], s1 = S.lvl_ptr[L + 1], nl = s1 - s0;
+    Plan::Level &lv = P.lv[L];
+    std::vector<int> mc(nl), nc(nl), sc(nl);
+    for (int sl = s0; sl < s1; ++sl) {
+      NodeDev &D = P.tmpl[sl];
+      node_static(S, sl, D);
+      const int i = S.node_of[sl];
+      int room = 0;  // most delays the children can hand up
+      for (int c : S.children[i]) {
+        const NodeDev &C = P.tmpl[S.slot[c]];
+        room += C.sn + C.ndcap;
+      }
+      const int seen = N.nd_seen[sl];
+      int cap = seen + (int)std::ceil(slack * (D.sn + seen));
+      D.ndcap = std::max(seen, std::min(cap, room));
+      const int m = D.sm + D.ndcap, n = D.sn + D.ndcap;
+      const int nsteps = pivot_tpp ? 0 : cdiv(n, kBlk);
+      mc[sl - s0] = m;
+      nc[sl - s0] = n;
+      sc[sl - s0] = nsteps;
+      lv.maxsteps = std::max(lv.maxsteps, nsteps);
+      if (nsteps > 1) lv.multi = true;
+      node_buffers(N, S, sl, D, m, n, nsteps, keep, cof[sl], wof[sl]);
+    }
+    const int ntab = level_tables(lv.maxsteps);
+    lv.cap.assign(ntab, 0);
+    for (int t = 0; t < ntab; ++t)
+      for (int li = 0; li < nl; ++li)
+        lv.cap[t] += level_table_count(t, lv.maxsteps, mc[li], nc[li], sc[li],
+                                       P.tmpl[s0 + li].is_root);
+    lv.tab_off = tab_total;
+    tab_total += (size_t)ntab * (nl + 1);
+    // this level's fronts have consumed their children's blocks
+    for (int sl = s0; sl < s1; ++sl) {
+      for (int c : S.children[S.node_of[sl]]) {
+        const int cs = S.slot[c];
+        N.pool.release(cof[cs].first, cof[cs].second);
+        cof[cs] = {nullptr, 0};
+      }
+      N.pool.release(wof[sl].first, wof[sl].second);
+    }
+  }
+  for (int sl = 0; sl < nn; ++sl)  // roots keep their contribution block
+    if (cof[sl].first) P.allocs.push_back(cof[sl]);
+  if (!P.d_tmpl) HIPCHK(hipMalloc(&P.d_tmpl, std::max(nn, 1) * sizeof(NodeDev)));
+  HIPCHK(hipMemcpy(P.d_tmpl, P.tmpl.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice));
+  if (tab_total > P.tabs_cap) {
+    if (P.d_tabs) HIPCHK(hipFree(P.d_tabs));
+    HIPCHK(hipMalloc(&P.d_tabs, std::max<size_t>(tab_total, 1) * sizeof(int)));
+    P.tabs_cap = tab_total;
+  }
+  if (!P.d_pack) HIPCHK(hipMalloc(&P.d_pack, 8 * (size_t)std::max(nn, 1) * sizeof(int)));
+  P.valid = true;
+}
+
+// One factorization on the static plan: all levels launched back to back, one
+// read-back at the end.  Returns false when some front's delays exceeded its
+// capacity (its results and its ancestors' are then void).
+static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_hip_options &opt,
+                          std::vector<hipEvent_t> &tev) {
+  Symbolic &S = *N.symb;
+  Plan &P = N.plan;
+  const int nn = S.nnodes, nlev = (int)S.lvl_ptr.size() - 1;
+  hipStream_t st = N.stream;
+  HIPCHK(hipMemcpyAsync(N.d_nodes, P.d_tmpl, nn * sizeof(NodeDev), hipMemcpyDeviceToDevice,
+                        st));
+  int *overflow = N.d_counters + 8;
+  for (int L = 0; L < nlev; ++L) {
+    const int s0 = S.lvl_ptr[L], nl = S.lvl_ptr[L + 1] - s0;
+    const Plan::Level &lv = P.lv[L];
+    const int ms = lv.maxsteps;
+    int *tab = P.d_tabs + lv.tab_off;
+    auto toff = [&](int t) { return (const int *)(tab + (size_t)t * (nl + 1)); };
+    const int *d_slots = N.d_slots_all + s0;
+    launch_level_setup(st, N.d_nodes, s0, nl, S.d_child, fp.pivot_tpp, ms, tab, overflow);
+    launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
+    if (opt.timing) tev.emplace_back(timed_event(st));
+    launch_gather(st, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
+                  S.d_asrc, S.d_arow, N.d_scal);
+    if (opt.timing) tev.emplace_back(timed_event(st));
+    for (int stp = 0; stp < ms; ++stp) {
+      launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      launch_apply(st, lv.cap[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
+      launch_update(st, lv.cap[ms + 2 + stp], N.d_nodes, d_slots, toff(ms + 2 + stp), nl, stp,
+                    0, fp);
+      if (lv.multi) {
+        launch_swap(st, lv.cap[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 0);
+        launch_swap(st, lv.cap[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 1);
+      }
+    }
+    launch_finalize(st, nl, N.d_nodes, d_slots);
+    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
+    launch_dprep(st, nl, N.d_nodes, d_slots);
+    if (lv.cap[2 * ms + 2] > 0)
+      launch_update(st, lv.cap[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
+    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
+  }
+  HIPCHK(hipGetLastError());
+  launch_pack_nodes(st, N.d_nodes, nn, P.d_pack);
+  std::vector<int> pack(8 * (size_t)nn);
+  int ovf = 0;
+  HIPCHK(hipMemcpyAsync(pack.data(), P.d_pack, pack.size() * sizeof(int),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&ovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (ovf > 0) return false;
+  N.nodes = P.tmpl;
+  for (int sl = 0; sl < nn; ++sl) {
+    NodeDev &D = N.nodes[sl];
+    const int *o = &pack[8 * (size_t)sl];
+    D.nd = o[0];
+    D.m = o[1];
+    D.n = o[2];
+    D.nelim = o[3];
+    D.nelim_app = o[4];
+    D.flag = o[5];
+    D.skip = o[6];
+    D.nsteps = o[7];
+  }
+  return true;
+}
+
+// One factorization level by level, sizing each level's fronts on the host
+// from the read-back of the levels below (exact sizes, any number of delays).
+static void factor_dynamic(Numeric &N, const FactorParams &fp,
+                           const spral_ssids_hip_options &opt, std::vector<hipEvent_t> &tev) {
+  Symbolic &S = *N.symb;
+  const int nn = S.nnodes, nlev = (int)S.lvl_ptr.size() - 1;
+  hipStream_t st = N.stream;
+  for (auto &p : N.persistent) N.pool.release(p.first, p.second);
+  N.persistent.clear();
+  N.nodes.assign(nn, NodeDev{});
+  std::vector<std::pair<void *, size_t>> contrib_of(nn), work_of(nn);
+  auto keep = [&](size_t bytes) {
+    void *p = N.pool.alloc(bytes);
+    N.persistent.push_back({p, bytes});
+    return p;
+  };
+  for (int L = 0; L < nlev; ++L) {
+    const int s0 = S.lvl_ptr[L], s1 = S.lvl_ptr[L + 1], nl = s1 - s0;
+    int maxsteps = 0;
+    bool multi = false;
+    for (int sl = s0; sl < s1; ++sl) {
+      const int i = S.node_of[sl];
+      NodeDev &D = N.nodes[sl];
+      D = NodeDev{};
+      node_static(S, sl, D);
+      int nd = 0;
+      for (int c : S.children[i]) {
+        const NodeDev &C = N.nodes[S.slot[c]];
+        nd += C.n - C.nelim;
+      }
+      D.nd = nd;
+      D.ndcap = nd;
+      D.m = D.sm + nd;
+      D.n = D.sn + nd;
+      D.nsteps = fp.pivot_tpp ? 0 : cdiv(D.n, kBlk);
+      maxsteps = std::max(maxsteps, D.nsteps);
+      if (D.nsteps > 1) multi = true;
+      node_buffers(N, S, sl, D, D.m, D.n, D.nsteps, keep, contrib_of[sl], work_of[sl]);
+    }
+    HIPCHK(hipMemcpyAsync(N.d_nodes + s0, N.nodes.data() + s0, nl * sizeof(NodeDev),
+                          hipMemcpyHostToDevice, st));
+    // per-level launch tables (exact sizes): slots | tables of level_table_count
+    const int ntab = level_tables(maxsteps);
+    std::vector<int> tab;
+    tab.reserve(nl + (size_t)ntab * (nl + 1));
+    for (int sl = s0; sl < s1; ++sl) tab.push_back(sl);
+    std::vector<int> tot(ntab, 0);
+    for (int t = 0; t < ntab; ++t) {
+      int acc = 0;
+      for (int li = 0; li < nl; ++li) {
+        tab.push_back(acc);
+        const NodeDev &D = N.nodes[s0 + li];
+        acc += level_table_count(t, maxsteps, D.m, D.n, D.nsteps, D.is_root);
+      }
+      tab.push_back(acc);
+      tot[t] = acc;
+    }
+    ensure_lvl(N, tab.size());
+    HIPCHK(hipMemcpyAsync(N.d_lvl, tab.data(), tab.size() * sizeof(int),
+                          hipMemcpyHostToDevice, st));
+    const int *d_slots = N.d_lvl;
+    auto toff = [&](int t) { return (const int *)(N.d_lvl + nl + (size_t)t * (nl + 1)); };
+    const int ms = maxsteps;
+    launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
+    if (opt.timing) tev.emplace_back(timed_event(st));
+    launch_gather(st, tot[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval, S.d_asrc,
+                  S.d_arow, N.d_scal);
+    if (opt.timing) tev.emplace_back(timed_event(st));
+    for (int stp = 0; stp < ms; ++stp) {
+      launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      launch_apply(st, tot[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
+      launch_update(st, tot[ms + 2 + stp], N.d_nodes, d_slots, toff(ms + 2 + stp), nl, stp, 0,
+                    fp);
+      if (multi) {
+        launch_swap(st, tot[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 0);
+        launch_swap(st, tot[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 1);
+      }
+    }
+    launch_finalize(st, nl, N.d_nodes, d_slots);
+    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
+    launch_dprep(st, nl, N.d_nodes, d_slots);
+    if (tot[2 * ms + 2] > 0)
+      launch_update(st, tot[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
+    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
+    HIPCHK(hipGetLastError());
+    // read back the level's results (nelim, flag)
+    std::vector<NodeDev> back(nl);
+    HIPCHK(hipMemcpyAsync(back.data(), N.d_nodes + s0, nl * sizeof(NodeDev),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int bad = 0;
+    for (int li = 0; li < nl; ++li) {
+      NodeDev &D = N.nodes[s0 + li];
+      D.nelim = back[li].nelim;
+      D.nelim_app = back[li].nelim_app;
+      D.flag = back[li].flag;
+      if (back[li].flag < 0) bad = back[li].flag;
+    }
+    // children contributions have been assembled: release them and the
+    // workspaces of this level
+    for (int sl = s0; sl < s1; ++sl) {
+      for (int c : S.children[S.node_of[sl]]) {
+        const int cs = S.slot[c];
+        N.pool.release(contrib_of[cs].first, contrib_of[cs].second);
+        contrib_of[cs] = {nullptr, 0};
+      }
+      N.pool.release(work_of[sl].first, work_of[sl].second);
+      work_of[sl] = {nullptr, 0};
+    }
+    if (bad < 0) {
+      N.stats.flag = bad;
+      break;
+    }
+  }
+  // roots keep their contribution block (exported by get_contrib)
+  for (int sl = 0; sl < nn; ++sl)
+    if (contrib_of[sl].first) N.persistent.push_back(contrib_of[sl]);
+  if ((int)N.nd_seen.size() != nn) N.nd_seen.assign(nn, 0);
+  for (int sl = 0; sl < nn; ++sl) N.nd_seen[sl] = std::max(N.nd_seen[sl], N.nodes[sl].nd);
+}
+
+static bool plan_disabled() {
+  const char *e = std::getenv("SPRAL_SSIDS_HIP_NO_PLAN");
+  return e && *e && *e != '0';
+}
+
 static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
                         const spral_ssids_hip_options &opt, int64_t naval) {
   Symbolic &S = *N.symb;
@@ -417,7 +782,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     HIPCHK(hipMalloc(&N.d_aval, std::max<int64_t>(naval, 1) * sizeof(double)));
     N.d_aval_cap = naval;
   }
-  if (naval > 0)
+  if (naval > 0 && d_aval != N.d_aval)
     HIPCHK(hipMemcpyAsync(N.d_aval, d_aval, naval * sizeof(double), hipMemcpyDeviceToDevice,
                           N.stream));
   if (d_scaling) {
@@ -437,193 +802,37 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   fp.action = opt.action;
   fp.tpp_failed = (opt.failed_pivot_method != 2);
   fp.pivot_tpp = (opt.pivot_method == 3);
-  // release factors of a previous factorization on this handle
-  for (auto &p : N.persistent) N.pool.release(p.first, p.second);
-  N.persistent.clear();
-  N.nodes.assign(nn, NodeDev{});
   if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
   if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
   if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
-  std::vector<int> nelim_h(nn, 0);  // slot indexed
-  const int nlev = (int)S.lvl_ptr.size() - 1;
-  std::vector<std::pair<void *, size_t>> contrib_of(nn);  // slot -> C buffer
-  std::vector<std::pair<void *, size_t>> work_of(nn);
-  int maxfront = 0;
-  std::vector<hipEvent_t> tev;  // pairs around Schur-update launches
-  auto timed_event = [](hipStream_t s) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipEventRecord(e, s));
-    return e;
-  };
+  if (!N.d_slots_all) {
+    std::vector<int> iota(std::max(nn, 1));
+    for (int i = 0; i < nn; ++i) iota[i] = i;
+    HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
+  std::vector<hipEvent_t> tev;  // pairs around the gather launches
   hipEvent_t ev0, ev1;
   HIPCHK(hipEventCreate(&ev0));
   HIPCHK(hipEventCreate(&ev1));
   HIPCHK(hipEventRecord(ev0, st));
-  auto palloc = [&](size_t bytes) {
-    void *p = N.pool.alloc(bytes);
-    N.persistent.push_back({p, bytes});
-    return p;
-  };
-  for (int L = 0; L < nlev; ++L) {
-    const int s0 = S.lvl_ptr[L], s1 = S.lvl_ptr[L + 1], nl = s1 - s0;
-    int maxsteps = 0;
-    bool multi = false;
-    for (int sl = s0; sl < s1; ++sl) {
-      int i = S.node_of[sl];
-      NodeDev &D = N.nodes[sl];
-      D = NodeDev{};
-      int nd = 0;
-      for (int c : S.children[i]) {
-        int cs = S.slot[c];
-        nd += N.nodes[cs].n - nelim_h[cs];
-      }
-      D.sm = S.sm[i];
-      D.sn = S.sn[i];
-      D.nd = nd;
-      D.m = D.sm + nd;
-      D.n = D.sn + nd;
-      D.ldl = D.m;
-      D.ldc = D.m - D.n;
-      D.ldbak = std::max(D.m, 1);
-      D.parent = S.parent[i] >= 0 ? S.slot[S.parent[i]] : -1;
-      D.cbeg = S.cbeg[i];
-      D.cend = S.cend[i];
-      D.is_root = (S.parent[i] < 0) ? 1 : 0;
-      D.nsteps = fp.pivot_tpp ? 0 : cdiv(D.n, kBlk);
-      D.rows = S.d_rows + S.row_off[i];
-      D.rmap = S.d_rmap + S.rmap_off[i];
-      D.acol = S.d_acol + S.acol_off[i];
-      maxfront = std::max(maxfront, D.m);
-      maxsteps = std::max(maxsteps, D.nsteps);
-      if (D.nsteps > 1) multi = true;
-      size_t lb = (size_t)D.ldl * D.n * sizeof(double);
-      D.L = (double *)palloc(lb);
-      D.pmap = (int *)palloc(std::max(D.m, 1) * sizeof(int));
-      D.D = (double *)palloc(2 * (size_t)D.n * sizeof(double));
-      D.Dd = (double *)palloc(3 * (size_t)std::max(D.n, 1) * sizeof(double));
-      D.perm = (int *)palloc(std::max(D.n, 1) * sizeof(int));
-      D.perm0 = (int *)palloc(std::max(D.n, 1) * sizeof(int));
-      D.ipos = (int *)palloc(std::max(D.n, 1) * sizeof(int));
-      size_t cb = (size_t)D.ldc * (D.m - D.n) * sizeof(double);
-      D.C = cb ? (double *)N.pool.alloc(cb) : nullptr;
-      contrib_of[sl] = {D.C, cb};
-      size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
-      if (D.nsteps > 1) wb += (size_t)2 * kBlk * D.m * sizeof(double);
-      size_t tabn = (size_t)(cdiv(D.m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
-      wb += tabn * sizeof(int);
-      char *w = (char *)N.pool.alloc(wb);
-      work_of[sl] = {w, wb};
-      D.bak = (double *)w;
-      D.dbak = D.bak + (size_t)D.ldbak * kBlk;
-      D.swp = (D.nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
-      D.ctab = (int *)(D.dbak + kBlk * kBlk + ((D.nsteps > 1) ? (size_t)2 * kBlk * D.m : 0));
-    }
-    HIPCHK(hipMemcpyAsync(N.d_nodes + s0, N.nodes.data() + s0, nl * sizeof(NodeDev),
-                          hipMemcpyHostToDevice, st));
-    // per-level launch tables: slots | asm | apply | swap | update(step..) | update tpp
-    std::vector<int> tab;
-    tab.reserve(nl * (5 + maxsteps) + 8);
-    for (int sl = s0; sl < s1; ++sl) tab.push_back(sl);
-    auto add_off = [&](auto count_fn) {
-      int base = (int)tab.size();
-      int acc = 0;
-      for (int li = 0; li < nl; ++li) {
-        tab.push_back(acc);
-        acc += count_fn(N.nodes[s0 + li]);
-      }
-      tab.push_back(acc);
-      return std::make_pair(base, acc);
-    };
-    auto asm_t = add_off([](const NodeDev &D) {
-      int T = cdiv(D.m, 64);
-      return T * (T + 1) / 2;
-    });
-    std::vector<std::pair<int, int>> app_t(maxsteps);
-    for (int stp = 0; stp < maxsteps; ++stp)
-      app_t[stp] = add_off([stp](const NodeDev &D) {
-        if (stp >= D.nsteps) return 0;
-        return stp + std::max(1, cdiv(D.m, kThreads));  // stp chunks of L-left rows
-      });
-    auto swp_t = add_off([](const NodeDev &D) {
-      return D.nsteps > 1 ? std::max(1, cdiv((int64_t)2 * kBlk * D.m, kThreads * 8)) : 0;
-    });
-    std::vector<std::pair<int, int>> upd_t(maxsteps);
-    for (int stp = 0; stp < maxsteps; ++stp) {
-      upd_t[stp] = add_off([stp](const NodeDev &D) {
-        if (stp >= D.nsteps) return 0;
-        int t0 = std::min(D.n, (stp + 1) * kBlk);
-        int TI = cdiv(D.m - t0, kBlk), TJ = cdiv(D.n - t0, kBlk);
-        int nt = TJ * TI - TJ * (TJ - 1) / 2;
-        return std::max(1, std::min(nt, 4096));
-      });
-    }
-    auto tpp_t = add_off([](const NodeDev &D) {
-      if (!D.is_root || D.m <= D.n) return 0;
-      int T = cdiv(D.m - D.n, kBlk);
-      return std::max(1, std::min(T * (T + 1) / 2, 4096));
-    });
-    ensure_lvl(N, tab.size());
-    HIPCHK(hipMemcpyAsync(N.d_lvl, tab.data(), tab.size() * sizeof(int),
-                          hipMemcpyHostToDevice, st));
-    const int *d_slots = N.d_lvl;
-    auto off = [&](std::pair<int, int> t) { return (const int *)(N.d_lvl + t.first); };
-    launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
-    if (opt.timing) tev.emplace_back(timed_event(st));
-    launch_gather(st, asm_t.second, N.d_nodes, d_slots, off(asm_t), nl, S.d_child, d_aval,
-                  S.d_asrc, S.d_arow, d_scaling);
-    if (opt.timing) tev.emplace_back(timed_event(st));
-    for (int stp = 0; stp < maxsteps; ++stp) {
-      launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
-      launch_apply(st, app_t[stp].second, N.d_nodes, d_slots, off(app_t[stp]), nl, stp, fp);
-      launch_update(st, upd_t[stp].second, N.d_nodes, d_slots, off(upd_t[stp]), nl, stp, 0,
-                    fp);
-      if (multi) {
-        launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 0);
-        launch_swap(st, swp_t.second, N.d_nodes, d_slots, off(swp_t), nl, stp, 1);
-      }
-    }
-    launch_finalize(st, nl, N.d_nodes, d_slots);
-    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
-    launch_dprep(st, nl, N.d_nodes, d_slots);
-    if (tpp_t.second > 0)
-      launch_update(st, tpp_t.second, N.d_nodes, d_slots, off(tpp_t), nl, 0, 1, fp);
-    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
-    HIPCHK(hipGetLastError());
-    // read back the level's results (nelim, flag)
-    std::vector<NodeDev> back(nl);
-    HIPCHK(hipMemcpyAsync(back.data(), N.d_nodes + s0, nl * sizeof(NodeDev),
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    int bad = 0;
-    for (int li = 0; li < nl; ++li) {
-      int sl = s0 + li;
-      nelim_h[sl] = back[li].nelim;
-      N.nodes[sl].nelim = back[li].nelim;
-      N.nodes[sl].nelim_app = back[li].nelim_app;
-      if (back[li].flag < 0) bad = back[li].flag;
-    }
-    // children contributions have been assembled: release them and the
-    // workspaces of this level
-    for (int sl = s0; sl < s1; ++sl) {
-      int i = S.node_of[sl];
-      for (int c : S.children[i]) {
-        int cs = S.slot[c];
-        N.pool.release(contrib_of[cs].first, contrib_of[cs].second);
-        contrib_of[cs] = {nullptr, 0};
-      }
-      N.pool.release(work_of[sl].first, work_of[sl].second);
-      work_of[sl] = {nullptr, 0};
-    }
-    if (bad < 0) {
-      N.stats.flag = bad;
-      break;
+  bool done = false;
+  if (!plan_disabled()) {
+    if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp)
+      plan_build(N, fp.pivot_tpp, opt.multiplier);
+    done = factor_static(N, fp, opt, tev);
+    if (!done) {
+      for (auto e : tev) (void)hipEventDestroy(e);
+      tev.clear();
+      plan_release(N);  // rebuilt with the delays seen, on the next factorization
     }
   }
-  // roots keep their contribution block (exported by get_contrib)
-  for (int sl = 0; sl < nn; ++sl)
-    if (contrib_of[sl].first) N.persistent.push_back(contrib_of[sl]);
-  HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
+  if (!done) factor_dynamic(N, fp, opt, tev);
+  const int nlev = (int)S.lvl_ptr.size() - 1;
+  int maxfront = 0;
+  for (int sl = 0; sl < nn; ++sl) maxfront = std::max(maxfront, N.nodes[sl].m);
   launch_stats(st, N.d_nodes, nn, N.d_counters);
   int cnt[16];
   HIPCHK(hipMemcpyAsync(cnt, N.d_counters, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -661,6 +870,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   o.num_levels = nlev;
   o.factor_ms = ms;
   o.pool_peak_bytes = (int64_t)N.pool.peak();
+  o.static_plan = done ? 1 : 0;
   o.kernel_ms = 0.0;
   o.kernel_launches = 0;
   for (size_t k = 0; k + 1 < tev.size(); k += 2) {

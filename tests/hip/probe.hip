@@ -1,7 +1,11 @@
-// Test-only probe of the v_mfma_f64_16x16x4_f64 operand/result layout
-// (in-bounds: one wave, 16x4 A, 4x16 B, 16x16 D).  Built into
-// tests/hip/libprobe.so; never linked by the product library.
+// Test-only probes of compiler builtins the kernels rely on (in-bounds: one
+// wave of 64 lanes, fixed-size buffers): the v_mfma_f64_16x16x4_f64
+// operand/result layout (16x4 A, 4x16 B, 16x16 D) and the DPP wave maximum
+// of the diagonal-block pivot search.  Built into tests/hip/libprobe.so;
+// never linked by the product library.
 #include <hip/hip_runtime.h>
+
+#include "dev.hpp"
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 __global__ void k_probe(const double *A, const double *B, double *out) {
@@ -25,6 +29,22 @@ extern "C" int probe_mfma_f64(const double *hA, const double *hB, double *hout) 
   hipError_t e = hipMemcpy(hout, O, 256 * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(A);
   (void)hipFree(B);
+  (void)hipFree(O);
+  return e == hipSuccess ? 0 : -2;
+}
+
+__global__ void k_probe_wave_max(const unsigned long long *in, unsigned long long *out) {
+  out[threadIdx.x] = ssids_hip::wave_max_u64(in[threadIdx.x]);  // 64 lanes, 64 outputs
+}
+
+extern "C" int probe_wave_max_u64(const unsigned long long *hin, unsigned long long *hout) {
+  unsigned long long *I, *O;
+  if (hipMalloc(&I, 64 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  if (hipMalloc(&O, 64 * sizeof(unsigned long long)) != hipSuccess) return -1;
+  (void)hipMemcpy(I, hin, 64 * sizeof(unsigned long long), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_probe_wave_max, dim3(1), dim3(64), 0, 0, I, O);
+  hipError_t e = hipMemcpy(hout, O, 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  (void)hipFree(I);
   (void)hipFree(O);
   return e == hipSuccess ? 0 : -2;
 }

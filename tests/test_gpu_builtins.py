@@ -1,0 +1,58 @@
+"""Lane layouts of the compiler builtins the kernels are written against,
+checked on the device through tests/hip/libprobe.so (test-only library)."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PROBE = Path(__file__).resolve().parent / "hip" / "libprobe.so"
+
+
+@pytest.fixture(scope="module")
+def probe(lib):
+    if not PROBE.exists():
+        from spral_amd import build
+        build.build_probe()
+    h = C.CDLL(str(PROBE))
+    h.probe_mfma_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    h.probe_wave_max_u64.argtypes = [C.c_void_p, C.c_void_p]
+    return h
+
+
+def test_mfma_f64_16x16x4_layout(probe):
+    """A[i][k] in lane i+16k, B[k][j] in lane j+16k, D[i][j] in lane
+    j+16(i%4), register i/4 -- the layout k_gather/k_update assume."""
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    out = np.zeros(256)
+    assert probe.probe_mfma_f64(A.ctypes.data, B.ctypes.data, out.ctypes.data) == 0
+    D = A @ B
+    got = np.empty((16, 16))
+    for i in range(16):
+        for j in range(16):
+            got[i, j] = out[(j + 16 * (i % 4)) * 4 + i // 4]
+    assert np.allclose(got, D, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dpp_wave_max_u64(probe, seed):
+    """The pivot search's wave maximum over 64-bit keys (non-negative
+    doubles' bit patterns), including ties and keys that differ only in the
+    low word, returns the true maximum in every lane."""
+    rng = np.random.default_rng(seed)
+    vals = np.abs(rng.standard_normal(64)) * 10.0 ** rng.integers(-5, 5, 64)
+    if seed % 2:
+        vals[rng.integers(0, 64, 5)] = vals.max()          # ties
+    if seed == 4:
+        base = vals.max()
+        vals[7] = np.nextafter(base, np.inf)                # differs in the low word
+    if seed == 5:
+        vals[:] = 0.0
+    keys = vals.view(np.uint64).copy()
+    out = np.zeros(64, np.uint64)
+    assert probe.probe_wave_max_u64(keys.ctypes.data, out.ctypes.data) == 0
+    assert np.all(out == keys.max())

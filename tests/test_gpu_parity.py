@@ -280,3 +280,82 @@ def test_scaling(lib, oracle_lib):
     ssids.ssids_solve_refine(xr, ak, fk, opts, maxit=5, tol=BWD_TOL)
     assert matrices.bwd_error(m.full, xr, b).max() <= BWD_TOL
     ssids.ssids_free(ak, fk)
+
+
+def _handle_factors(num, nnodes):
+    L = _lib.lib()
+    out = []
+    for node in range(nnodes):
+        info = np.zeros(6, np.int32)
+        L.spral_ssids_hip_node_factors(num, node, info.ctypes.data, None, None, None)
+        m, n, nd, ne, ld, sn = (int(v) for v in info)
+        Lm = np.zeros((max(n, 1), ld))
+        D = np.zeros(2 * max(n, 1))
+        perm = np.zeros(max(n, 1), np.int32)
+        L.spral_ssids_hip_node_factors(num, node, info.ctypes.data, Lm.ctypes.data,
+                                       D.ctypes.data, perm.ctypes.data)
+        Lf = np.tril(Lm[:n, :m].T)  # the panel's lower part (the rest is never written)
+        out.append((m, n, nd, ne, Lf, D[:2 * n].copy(), perm[:n].copy()))
+    return out
+
+
+@pytest.mark.parametrize("case", ["laplace", "delays"])
+def test_static_plan_matches_level_walk(lib, monkeypatch, case):
+    """The statically planned factorization (one stream of launches, fronts
+    sized on the device) gives bitwise the factors of the level-by-level
+    walk; a front whose delays exceed its planned capacity makes the
+    factorization fall back to the level walk, and the next one on the same
+    handle runs on a plan widened to the delays seen."""
+    import ctypes as C
+    from spral_amd import devmem
+    print(f"static plan case={case}", flush=True)
+    if case == "laplace":
+        m = matrices.laplace3d(12, diag=5.7)
+        ak, inf = analyse(m, grid=(12, 12, 12), use_gpu=True)
+    else:
+        m = matrices.random_indef(500, density=0.03, seed=5)
+        ak, inf = analyse(m, use_gpu=True)
+    assert inf.flag >= 0
+    L = _lib.lib()
+    opts = ssids.SSIDSOptions()
+    copt = opts.to_c()
+    copt.multiplier = 1.0  # no slack: any delay overflows the first plan
+    d_val = devmem.DeviceArray.from_host(m.val)
+    st = _lib.Stats()
+    num = L.spral_ssids_hip_factor_device(False, ak.symbolic, d_val.ptr,
+                                          m.val.size, None, C.byref(copt), C.byref(st))
+    assert num and st.flag >= 0
+    if case == "laplace":
+        assert st.num_delay == 0 and st.static_plan == 1
+    else:
+        assert st.num_delay > 0 and st.static_plan == 0
+    first = (st.num_neg, st.num_delay, st.num_two)
+
+    def refactor():
+        s = _lib.Stats()
+        f = L.spral_ssids_hip_refactor_device(num, d_val.ptr, None,
+                                              C.byref(copt), C.byref(s))
+        assert f >= 0, f
+        return s
+
+    s1 = refactor()
+    assert s1.static_plan == 1
+    assert (s1.num_neg, s1.num_delay, s1.num_two) == first
+    f_static = _handle_factors(num, ak.nnodes)
+    xt, b = rhs(m, 2, 11)
+    fk = ssids.FKeep(numeric=num, pos_def=False, scaling=None, inform=ssids.SSIDSInform())
+    x_static = gpu_solve(ak, fk, opts, b)
+    monkeypatch.setenv("SPRAL_SSIDS_HIP_NO_PLAN", "1")
+    s2 = refactor()
+    assert s2.static_plan == 0
+    assert (s2.num_neg, s2.num_delay, s2.num_two) == first
+    f_walk = _handle_factors(num, ak.nnodes)
+    x_walk = gpu_solve(ak, fk, opts, b)
+    for a, c in zip(f_static, f_walk):
+        assert a[:4] == c[:4]
+        for u, v in zip(a[4:], c[4:]):
+            assert np.array_equal(u, v)
+    assert np.array_equal(x_static, x_walk)
+    assert matrices.bwd_error(m.full, x_static, b).max() <= RAW_TOL
+    L.spral_ssids_hip_destroy_numeric(num)
+    ssids.ssids_free(ak)
