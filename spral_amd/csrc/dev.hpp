@@ -103,6 +103,15 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
   return imax(1, imin(T * (T + 1) / 2, 4096));
 }
 
+// value held by lane 0 (wave-uniform by construction), as a scalar
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double uni(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Wave-wide maxima with DPP lane permutations inside each row of 16 lanes
 // (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
 // then the four row results read as scalars.  Every lane of the wave must be

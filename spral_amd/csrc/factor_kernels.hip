@@ -9,7 +9,7 @@
 //               deterministic sums) through LDS-staged child->parent maps,
 //               computing the children's Schur complements on MFMA on the way.
 //  k_diag       one workgroup per front: factor the w<=64 column diagonal
-//               block with complete pivoting (1x1/2x2) in LDS
+//               block with complete pivoting (1x1/2x2) in registers
 //               (pivot rules of src/ssids/cpu/kernels/block_ldlt.hxx:213-414).
 //  k_apply      L = A Lbb^{-T} D^{-1} for the rows below the block, keeping a
 //               backup and the column max |l| (a posteriori test,
@@ -62,14 +62,16 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block factorization, one wavefront per front.  Lane j keeps column
-// j of the symmetric w x w block in registers (a[0..63]); pivots are tracked
-// in a wave-uniform bit mask (no physical swaps).  Per pivot step:
-//  * search: every live lane scans its live rows for max |a(i,j)| (rows in
-//    ascending order, first maximum kept); the wave takes the maximum value,
-//    then the smallest column holding it (ballot), then that column's row;
-//  * the pivot column(s) are broadcast through LDS by the lanes owning them;
-//  * rank-1 / rank-2 update of every column in registers;
+// Diagonal block factorization, NW waves per front.  Lane j of wave v keeps
+// rows [vR, vR+R) (R = 64/NW) of column j of the symmetric w x w block in
+// registers; pivots are tracked in a uniform bit mask (no physical swaps).
+// Per pivot step:
+//  * search: every lane scans its live rows for max |a(i,j)| (rows in
+//    ascending order, first maximum kept); each wave reduces to its
+//    (value, smallest column, that column's row) with DPP and a ballot, and
+//    the waves' candidates are merged through LDS in the same order;
+//  * the owners of the pivot column(s) broadcast them through LDS;
+//  * rank-1 / rank-2 update of every column's rows in registers;
 //  * the L column(s) of the step go to LDS, indexed by elimination step.
 // Pivot rules of src/ssids/cpu/kernels/block_ldlt.hxx:289-414: maximum on the
 // diagonal -> 1x1; else the 2x2 test of block_ldlt.hxx:213-218, else the
@@ -78,26 +80,22 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 // |l| <= 1/u over the full column.  The elimination order lp[] permutes the
 // block on write back.
 // ---------------------------------------------------------------------------
-// value held by lane 0 (wave-uniform by construction), as a scalar
-__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline double uni(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
-  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
+constexpr int kDiagWaves = 8;
 
-
-__global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slots, int step,
-                                             FactorParams fp) {
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl_slots,
+                                                  int step, FactorParams fp) {
+  constexpr int R = kBlk / NW;
   __shared__ double t[kBlk][kBlk + 1];  // load staging, then L by step: t[k][row]
   __shared__ double cs[2][kBlk];        // broadcast pivot columns
+  __shared__ unsigned long long ckey[2][NW];
+  __shared__ int ccol[2][NW], crow[2][NW];
   __shared__ double dinv[2 * kBlk + 2];
   __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
   const int s = lvl_slots[blockIdx.x];
   NodeDev &N = nodes[s];
   if (step >= N.nsteps) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r0 = wv * R;
   int p, q;
   if (step == 0) {
     p = 0;
@@ -107,8 +105,8 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
     q = N.q - (N.w - N.npass);
   }
   const int w = max(0, min(kBlk, q - p));
-  __syncthreads();  // all lanes have read N.p/N.npass before lane 0 updates them
-  if (lane == 0) {
+  __syncthreads();  // every thread has read N.p/N.npass before they change
+  if (tid == 0) {
     N.p = p;
     N.q = q;
     N.w = w;
@@ -117,10 +115,10 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
   if (w == 0) return;
   const int ldl = N.ldl;
   double *L = N.L;
-  for (int idx = lane; idx < kBlk * kBlk; idx += 64) {
-    int r = idx % kBlk, c = idx / kBlk;
+  for (int idx = tid; idx < kBlk * kBlk; idx += 64 * NW) {
+    const int r = idx % kBlk, c = idx / kBlk;
     if (r >= c) {
-      double v = (r < w) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
+      const double v = (r < w) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
       t[r][c] = v;
       t[c][r] = v;
       if (r < w) {
@@ -129,55 +127,64 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
       }
     }
   }
-  N.colmax[lane] = 0ull;
+  if (tid < kBlk) N.colmax[tid] = 0ull;
   __syncthreads();
-  double a[kBlk];
+  double a[R];
 #pragma unroll
-  for (int i = 0; i < kBlk; ++i) a[i] = t[i][lane];
+  for (int r = 0; r < R; ++r) a[r] = t[r0 + r][lane];
   __syncthreads();
   unsigned long long elim = (w < 64) ? (~0ull << w) : 0ull;  // eliminated / absent
-  int k = 0;
+  int k = 0, par = 0;
   while (k < w) {
     const bool live = !((elim >> lane) & 1ull);
-    // ---- search this column's live rows: four interleaved running maxima
-    // (independent chains), each keeping its first maximal row; then the
-    // largest, ties to the smaller row.  NaN never wins; an all-zero column
-    // keeps br = 64.
-    double bq[4] = {0.0, 0.0, 0.0, 0.0};
-    int rq[4] = {64, 64, 64, 64};
+    // ---- search this lane's live rows
+    double bv = 0.0;
+    int br = 64;
 #pragma unroll
-    for (int c = 0; c < kBlk / 8; ++c) {
-      if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
-#pragma unroll
-      for (int ii = 0; ii < 8; ++ii) {
-        const int i = 8 * c + ii;
-        if ((elim >> i) & 1ull) continue;
-        if (fabs(a[i]) > bq[ii & 3]) {
-          bq[ii & 3] = fabs(a[i]);
-          rq[ii & 3] = i;
-        }
+    for (int r = 0; r < R; ++r) {
+      if ((elim >> (r0 + r)) & 1ull) continue;
+      if (fabs(a[r]) > bv) {  // NaN never wins; all-zero keeps br = 64
+        bv = fabs(a[r]);
+        br = r0 + r;
       }
     }
-    double bv = bq[0];
-    int br = rq[0];
-#pragma unroll
-    for (int q = 1; q < 4; ++q)
-      if (bq[q] > bv || (bq[q] == bv && rq[q] < br)) {
-        bv = bq[q];
-        br = rq[q];
-      }
-    // ---- wave arg-max: value, then smallest column, then (per column) row
+    // ---- wave candidate: value, then smallest column, then its row
     const unsigned long long key = live ? (unsigned long long)__double_as_longlong(bv) : 0ull;
-    const unsigned long long kmax = wave_max_u64(key);
-    const unsigned long long hit = __ballot(live && key == kmax);
-    const double vmax = uni(__longlong_as_double((long long)kmax));
-    if (!(vmax >= fp.small) || hit == 0ull) {
+    const unsigned long long kw = wave_max_u64(key);
+    const unsigned long long hit = __ballot(live && key == kw);
+    const int cw = hit ? __ffsll((long long)hit) - 1 : 64;
+    const int rw = hit ? __builtin_amdgcn_readlane(br, cw) : 64;
+    if (lane == 0) {
+      ckey[par][wv] = kw;
+      ccol[par][wv] = cw;
+      crow[par][wv] = rw;
+    }
+    __syncthreads();
+    unsigned long long kmax = 0ull;
+    int bc = 64, brow = 64;
+#pragma unroll
+    for (int v = 0; v < NW; ++v) {
+      const unsigned long long kv = ckey[par][v];
+      const int cv = ccol[par][v], rv = crow[par][v];
+      if (kv > kmax || (kv == kmax && (cv < bc || (cv == bc && rv < brow)))) {
+        kmax = kv;
+        bc = cv;
+        brow = rv;
+      }
+    }
+    kmax = ((unsigned long long)(unsigned)uni((int)(kmax >> 32)) << 32) |
+           (unsigned)uni((int)(kmax & 0xffffffffull));
+    bc = uni(bc);
+    brow = uni(brow);
+    par ^= 1;
+    const double vmax = __longlong_as_double((long long)kmax);
+    if (!(vmax >= fp.small) || bc >= 64) {
       // zero pivots for every remaining index, in increasing index order
       int kk = k;
       for (int i = 0; i < w; ++i) {
         if ((elim >> i) & 1ull) continue;
-        t[kk][lane] = 0.0;
-        if (lane == 0) {
+        if (wv == 0) t[kk][lane] = 0.0;
+        if (tid == 0) {
           lp[kk] = i;
           kind[kk] = 0;
           dinv[2 * kk] = 0.0;
@@ -185,20 +192,18 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
         }
         ++kk;
       }
-      if (lane == 0) {
+      if (tid == 0) {
         N.nzero += w - k;
         if (!fp.action) N.flag = -5;
       }
       k = w;
       break;
     }
-    const int bc = uni(__ffsll((long long)hit) - 1);
-    const int brow = __builtin_amdgcn_readlane(br, bc);
     // ---- owners broadcast their columns: cs[0] <- column bc, cs[1] <- column brow
     if (lane == bc || lane == brow) {
-      double *dst = cs[lane == bc ? 0 : 1];
+      double *dst = cs[lane == bc ? 0 : 1] + r0;
 #pragma unroll
-      for (int i = 0; i < kBlk; ++i) dst[i] = a[i];
+      for (int r = 0; r < R; ++r) dst[r] = a[r];
     }
     __syncthreads();
     int size, p1, p2 = -1;
@@ -232,13 +237,9 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
       elim |= 1ull << p1;
       const double lj = c1p[lane] * di;  // L(lane, k)
 #pragma unroll
-      for (int c = 0; c < kBlk / 8; ++c) {
-        if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
-#pragma unroll
-        for (int ii = 0; ii < 8; ++ii) a[8 * c + ii] -= c1p[8 * c + ii] * lj;
-      }
-      t[k][lane] = lj;
-      if (lane == 0) {
+      for (int r = 0; r < R; ++r) a[r] -= c1p[r0 + r] * lj;
+      if (wv == 0) t[k][lane] = lj;
+      if (tid == 0) {
         dinv[2 * k] = di;
         dinv[2 * k + 1] = 0.0;
         lp[k] = p1;
@@ -249,15 +250,12 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
       const double x = c1p[lane], y = c2p[lane];
       const double l1 = d11 * x + d21 * y, l2 = d21 * x + d22 * y;
 #pragma unroll
-      for (int c = 0; c < kBlk / 8; ++c) {
-        if (((elim >> (8 * c)) & 0xffull) == 0xffull) continue;
-#pragma unroll
-        for (int ii = 0; ii < 8; ++ii)
-          a[8 * c + ii] -= c1p[8 * c + ii] * l1 + c2p[8 * c + ii] * l2;
+      for (int r = 0; r < R; ++r) a[r] -= c1p[r0 + r] * l1 + c2p[r0 + r] * l2;
+      if (wv == 0) {
+        t[k][lane] = l1;
+        t[k + 1][lane] = l2;
       }
-      t[k][lane] = l1;
-      t[k + 1][lane] = l2;
-      if (lane == 0) {
+      if (tid == 0) {
         dinv[2 * k] = d11;
         dinv[2 * k + 1] = d21;
         dinv[2 * k + 2] = INFINITY;
@@ -268,16 +266,15 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
         kind[k + 1] = 3;
       }
     }
-    __syncthreads();
     k += size;
   }
   __syncthreads();
   // ---- write back L_bb (rows permuted into elimination order), D^{-1}, lp
-  for (int i = lane; i < w; i += 64) inv_lp[lp[i]] = i;
+  for (int i = tid; i < w; i += 64 * NW) inv_lp[lp[i]] = i;
   __syncthreads();
-  for (int idx = lane; idx < w * w; idx += 64) {
-    int r = idx % w, kk = idx / w;  // original row r, elimination column kk
-    int nr = inv_lp[r];
+  for (int idx = tid; idx < w * w; idx += 64 * NW) {
+    const int r = idx % w, kk = idx / w;  // original row r, elimination column kk
+    const int nr = inv_lp[r];
     if (nr < kk) continue;
     double v;
     if (nr == kk) v = (kind[kk] == 0) ? 0.0 : 1.0;
@@ -285,15 +282,15 @@ __global__ __launch_bounds__(64) void k_diag(NodeDev *nodes, const int *lvl_slot
     else v = t[kk][r];
     L[(size_t)(p + kk) * ldl + p + nr] = v;
   }
-  for (int j = lane; j < w; j += 64) {
+  for (int j = tid; j < w; j += 64 * NW) {
     N.D[2 * (p + j)] = dinv[2 * j];
     N.D[2 * (p + j) + 1] = dinv[2 * j + 1];
     N.lp[j] = lp[j];
   }
   // permute the variable list (rows of earlier eliminated columns: k_apply)
-  int pv = (lane < w) ? N.perm[p + lp[lane]] : 0;
+  const int pv = (tid < w) ? N.perm[p + lp[tid]] : 0;
   __syncthreads();
-  if (lane < w) N.perm[p + lane] = pv;
+  if (tid < w) N.perm[p + tid] = pv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1284,7 +1281,8 @@ void launch_dprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_diag, dim3(nlvl), dim3(64), 0, st, nodes, slots, step, fp);
+    hipLaunchKernelGGL(k_diag<kDiagWaves>, dim3(nlvl), dim3(64 * kDiagWaves), 0, st, nodes,
+                       slots, step, fp);
 }
 void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                   const int *wg_off, int nlvl, int step, FactorParams fp) {
