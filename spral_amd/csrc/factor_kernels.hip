@@ -334,33 +334,37 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
   count -= nleft;
   for (int idx = tid; idx < w * w; idx += kThreads) {
     int r = idx % w, c = idx / w;
-    lbb[r][c] = (r > c) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
+    lbb[c][r] = (r > c) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;  // transposed: lbb[col][row]
   }
   for (int j = tid; j < 2 * w; j += kThreads) dv[j] = N.D[2 * p + j];
   if (tid == 0) { dv[2 * w] = 0.0; dv[2 * w + 1] = 0.0; }
   for (int j = tid; j < kBlk; j += kThreads) lps[j] = (j < w) ? N.lp[j] : 0;
   __syncthreads();
-  double mx[kBlk];
-#pragma unroll
-  for (int j = 0; j < kBlk; ++j) mx[j] = 0.0;
+  // one row per thread: the grid gives every front max(1, ceil(rows / kThreads))
+  // row workgroups
   const int r0 = p + w, m = N.m;
   const double small = fp.small;
-  for (int r = r0 + local * kThreads + tid; r < m; r += count * kThreads) {
-    double x[kBlk];
+  const int r = r0 + local * kThreads + tid;
+  const bool have = r < m;
+  double x[kBlk];
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) x[j] = 0.0;
+  if (have) {
 #pragma unroll
     for (int j = 0; j < kBlk; ++j)
       if (j < w) x[j] = L[(size_t)(p + lps[j]) * ldl + r];
 #pragma unroll
     for (int j = 0; j < kBlk; ++j)
       if (j < w) N.bak[(size_t)j * N.ldbak + (r - r0)] = x[j];
-    // x L_bb^T = a  (forward substitution)
+    // x L_bb^T = a  (forward substitution by columns: once x[t] is final it
+    // updates every later x[j], so the per-j chains interleave; each x[j]
+    // still accumulates in increasing t)
 #pragma unroll
-    for (int j = 1; j < kBlk; ++j) {
-      if (j < w) {
-        double acc = x[j];
+    for (int t = 0; t < kBlk - 1; ++t) {
+      if (t < w) {
 #pragma unroll
-        for (int t = 0; t < j; ++t) acc -= x[t] * lbb[j][t];
-        x[j] = acc;
+        for (int j = t + 1; j < kBlk; ++j)
+          if (j < w) x[j] -= x[t] * lbb[t][j];
       }
     }
     // x D^{-1}
@@ -382,18 +386,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
       }
     }
 #pragma unroll
-    for (int j = 0; j < kBlk; ++j) {
-      if (j < w) {
-        L[(size_t)(p + j) * ldl + r] = x[j];
-        double v = fabs(x[j]);
-        if (!(v <= mx[j])) mx[j] = v;  // NaN propagates as a failure
-      }
-    }
+    for (int j = 0; j < kBlk; ++j)
+      if (j < w) L[(size_t)(p + j) * ldl + r] = x[j];
   }
   // reduce column maxima: wave then workgroup, one atomic per column
 #pragma unroll
   for (int j = 0; j < kBlk; ++j) {
-    double v = mx[j];
+    double v = (have && j < w) ? fabs(x[j]) : 0.0;  // NaN propagates as a failure
     for (int off = 32; off > 0; off >>= 1) {
       double o = __shfl_xor(v, off);
       v = (o > v || isnan(o)) ? o : v;
@@ -668,7 +667,7 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
 __global__ __launch_bounds__(kThreads) void k_gather(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
-    const double *scaling) {
+    const int *acix, const double *scaling) {
   __shared__ double acc[kGT][kGT + 1];
   __shared__ int mr[kGT], mc[kGT];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
@@ -709,6 +708,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(
   }
   __syncthreads();
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int ri = lane & 15, kq = lane >> 4;
   for (int ci = P.cbeg; ci < P.cend; ++ci) {
     const NodeDev &c = nodes[child_list[ci]];
     const int kb = c.nelim, ke = c.m;
@@ -735,17 +735,30 @@ __global__ __launch_bounds__(kThreads) void k_gather(
     const int kc0 = max(k0, c.n);
     const int rr0 = max(kr0, c.n);
     if (k1 > kc0 && kr1 > rr0) {
-      const int nrow = kr1 - rr0, ncol = k1 - kc0;
-      // C_c, lower part, read down its columns (coalesced)
-      for (int idx = tid; idx < nrow * ncol; idx += kThreads) {
-        const int a = idx % nrow, b = idx / nrow;
-        const int kr = rr0 + a, k = kc0 + b;
-        if (kr >= k) acc[mr[kr - kr0]][mc[k - k0]] += c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
+      const int nrow = kr1 - rr0, ncol = k1 - kc0, nel = nrow * ncol;
+      // C_c, lower part, down its columns: all loads issued before any add
+      double cv[kGT * kGT / kThreads];
+#pragma unroll
+      for (int u = 0; u < kGT * kGT / kThreads; ++u) {
+        const int idx = tid + u * kThreads;
+        cv[u] = 0.0;
+        if (idx < nel) {
+          const int kr = rr0 + idx % nrow, k = kc0 + idx / nrow;
+          if (kr >= k) cv[u] = c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
+        }
       }
-      // L21 (D L21)^T on MFMA: each wave a 32x32 quadrant, operands loaded in
-      // the 16x16x4 fragment layout (lane i+16k holds row i, pivot k)
+#pragma unroll
+      for (int u = 0; u < kGT * kGT / kThreads; ++u) {
+        const int idx = tid + u * kThreads;
+        if (idx < nel) {
+          const int a = idx % nrow, b = idx / nrow;
+          if (rr0 + a >= kc0 + b) acc[mr[rr0 - kr0 + a]][mc[kc0 - k0 + b]] += cv[u];
+        }
+      }
+      // L21 (D L21)^T on MFMA: each wave a 32x32 quadrant, operands in the
+      // 16x16x4 fragment layout (lane i+16k holds row i, pivot k), loaded
+      // 16 pivots at a time before the MFMAs that use them
       const int K = c.nelim;
-      const int ri = lane & 15, kq = lane >> 4;
       const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
       const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
       const double *La = c.L + rr0, *Lb = c.L + kc0;
@@ -754,28 +767,38 @@ __global__ __launch_bounds__(kThreads) void k_gather(
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-      for (int t0 = 0; t0 < K; t0 += 4) {
-        const int t = t0 + kq;
-        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-        if (t < K) {
-          const size_t o = (size_t)t * c.ldl;
-          if (va0) a0 = La[o + ra0];
-          if (va1) a1 = La[o + ra1];
-          const double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
-          const int off = (int)c.Dd[3 * t + 2];
-          if (vb0) b0 = Lb[o + cb0] * e;
-          if (vb1) b1 = Lb[o + cb1] * e;
-          if (off != 0) {
-            const size_t op = (size_t)(t + off) * c.ldl;
-            if (vb0) b0 += Lb[op + cb0] * f;
-            if (vb1) b1 += Lb[op + cb1] * f;
+      constexpr int KC = 4;  // k-steps (of 4 pivots) per chunk
+      for (int t0 = 0; t0 < K; t0 += 4 * KC) {
+        double fa0[KC], fa1[KC], fb0[KC], fb1[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+          const int t = t0 + 4 * u + kq;
+          fa0[u] = fa1[u] = fb0[u] = fb1[u] = 0.0;
+          if (t < K) {
+            const size_t o = (size_t)t * c.ldl;
+            if (va0) fa0[u] = La[o + ra0];
+            if (va1) fa1[u] = La[o + ra1];
+            const double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
+            const int off = (int)c.Dd[3 * t + 2];
+            double x0 = vb0 ? Lb[o + cb0] : 0.0, x1 = vb1 ? Lb[o + cb1] : 0.0;
+            double y0 = 0.0, y1 = 0.0;
+            if (off != 0) {
+              const size_t op = (size_t)(t + off) * c.ldl;
+              if (vb0) y0 = Lb[op + cb0];
+              if (vb1) y1 = Lb[op + cb1];
+            }
+            fb0[u] = x0 * e + y0 * f;
+            fb1[u] = x1 * e + y1 * f;
           }
         }
-        g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, g[0][0], 0, 0, 0);
-        g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, g[0][1], 0, 0, 0);
-        g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, g[1][0], 0, 0, 0);
-        g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, g[1][1], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+          if (t0 + 4 * u >= K) break;
+          g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0[u], fb0[u], g[0][0], 0, 0, 0);
+          g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0[u], fb1[u], g[0][1], 0, 0, 0);
+          g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1[u], fb0[u], g[1][0], 0, 0, 0);
+          g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1[u], fb1[u], g[1][1], 0, 0, 0);
+        }
       }
       __syncthreads();  // the C_c pass has finished with acc
 #pragma unroll
@@ -794,21 +817,15 @@ __global__ __launch_bounds__(kThreads) void k_gather(
     }
     __syncthreads();
   }
-  // A entries of the tile's own fully-summed columns
+  // A entries of the tile's own fully-summed columns (entry -> column precomputed)
   {
-    int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
+    const int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
     if (cs1 > cs0) {
-      int64_t e0 = P.acol[cs0], e1 = P.acol[cs1];
+      const int64_t e0 = P.acol[cs0], e1 = P.acol[cs1];
       for (int64_t e = e0 + tid; e < e1; e += kThreads) {
-        int r = arow[e] + nd;
+        const int r = arow[e] + nd;
         if (r < r0 || r >= r0 + kGT) continue;
-        int lo = cs0, hi = cs1;
-        while (hi - lo > 1) {
-          int mid = (lo + hi) >> 1;
-          if (P.acol[mid] <= e) lo = mid;
-          else hi = mid;
-        }
-        int cc = lo;
+        const int cc = acix[e];
         double v = aval[asrc[e]];
         if (scaling) v *= scaling[P.rows[r - nd]] * scaling[P.rows[cc]];
         acc[r - r0][cc + nd - c0] += v;
@@ -1265,10 +1282,11 @@ void launch_pack_nodes(hipStream_t st, const NodeDev *nodes, int nn, int *out) {
 }
 void launch_gather(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, const int *child_list, const double *aval,
-                   const int64_t *asrc, const int *arow, const double *scaling) {
+                   const int64_t *asrc, const int *arow, const int *acix,
+                   const double *scaling) {
   if (nwg > 0)
     hipLaunchKernelGGL(k_gather, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,
-                       child_list, aval, asrc, arow, scaling);
+                       child_list, aval, asrc, arow, acix, scaling);
 }
 void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
                    const int *child_list) {

@@ -24,7 +24,7 @@ namespace ssids_hip {
 
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
 void launch_gather(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
-                   const double *, const int64_t *, const int *, const double *);
+                   const double *, const int64_t *, const int *, const int *, const double *);
 void launch_dprep(hipStream_t, int, NodeDev *, const int *);
 void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
@@ -39,8 +39,12 @@ void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
-void launch_fwd(hipStream_t, int, const SolveNode *, const int *, const int *, double *, int,
-                int, const int *, int);
+void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
+                    int, int);
+void launch_fwd_gemv(hipStream_t, int, const SolveNode *, const int *, int, const int *, int);
+void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
+void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
+int big_tri_threshold();
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
                 const int *, int);
@@ -164,7 +168,8 @@ struct Symbolic {
   int64_t naval_min = 0;                  // max A index referenced + 1
   int64_t num_flops = 0;
   // device
-  int *d_rows = nullptr, *d_rmap = nullptr, *d_arow = nullptr, *d_child = nullptr;
+  int *d_rows = nullptr, *d_rmap = nullptr, *d_arow = nullptr, *d_acix = nullptr;
+  int *d_child = nullptr;
   int64_t *d_asrc = nullptr, *d_acol = nullptr;
   // full symmetric matrix in elimination order (CSR) for refinement residuals
   int64_t *d_fptr = nullptr, *d_fsrc = nullptr;
@@ -223,6 +228,9 @@ struct Numeric {
   int *d_gemv_tab = nullptr;              // per level: nlvl+1 GEMV workgroup offsets
   std::vector<int> gemv_tab_off, gemv_total;  // per level: offset into d_gemv_tab, total
   std::vector<int> gemv_count;            // per slot
+  std::vector<std::vector<int>> big_of_level;  // slots whose triangle needs the launch sequence
+  double *d_tri_part = nullptr;           // partial sums of the big-front backward solve
+  size_t d_tri_part_cap = 0;
   std::vector<SolveNode> sn_h;
   double *d_X = nullptr;
   size_t d_X_cap = 0;
@@ -252,6 +260,7 @@ struct Numeric {
     if (d_sn) (void)hipFree(d_sn);
     if (d_slots_all) (void)hipFree(d_slots_all);
     if (d_gemv_tab) (void)hipFree(d_gemv_tab);
+    if (d_tri_part) (void)hipFree(d_tri_part);
     if (d_X) (void)hipFree(d_X);
     if (d_x) (void)hipFree(d_x);
     if (d_aval) (void)hipFree(d_aval);
@@ -317,7 +326,7 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   }
   // A entries sorted by column per node
   std::vector<int64_t> asrc;
-  std::vector<int> arow;
+  std::vector<int> arow, acix;  // front row and own-column index of each entry
   std::vector<int64_t> acol;
   int64_t amax = 0;
   for (int i = 0; i < nn; ++i) {
@@ -337,6 +346,7 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
       for (auto &pr : bycol[c]) {
         asrc.push_back(pr.first);
         arow.push_back(pr.second);
+        acix.push_back(c);
       }
     }
     acol.push_back((int64_t)asrc.size());
@@ -381,6 +391,7 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   S->d_rmap = dev_upload(rmap, S->owned);
   S->d_asrc = dev_upload(asrc, S->owned);
   S->d_arow = dev_upload(arow, S->owned);
+  S->d_acix = dev_upload(acix, S->owned);
   S->d_acol = dev_upload(acol, S->owned);
   S->d_child = dev_upload(S->child_list, S->owned);
   // full CSR (rows sorted by column) of the matrix in elimination order
@@ -503,8 +514,9 @@ static void plan_release(Numeric &N) {
 
 // Place every front for a delay capacity of
 //   ndcap = seen + ceil((multiplier - 1) (sn + seen)),
-// at most the children's fully-summed columns (seen = most delays observed
-// at this front in an earlier factorization, else 0).
+// at most the children's fully-summed columns and at most what fits in the
+// APP steps the front needs anyway (seen = most delays observed at this
+// front in an earlier factorization, else 0).
 static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
   Symbolic &S = *N.symb;
   Plan &P = N.plan;
@@ -539,6 +551,9 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
       }
       const int seen = N.nd_seen[sl];
       int cap = seen + (int)std::ceil(slack * (D.sn + seen));
+      // no slack beyond the last APP step the seen delays already need: an
+      // extra step would cost every factorization a round of empty launches
+      if (!pivot_tpp) cap = std::min(cap, cdiv(D.sn + seen, kBlk) * kBlk - D.sn);
       D.ndcap = std::max(seen, std::min(cap, room));
       const int m = D.sm + D.ndcap, n = D.sn + D.ndcap;
       const int nsteps = pivot_tpp ? 0 : cdiv(n, kBlk);
@@ -603,7 +618,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
     if (opt.timing) tev.emplace_back(timed_event(st));
     launch_gather(st, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
-                  S.d_asrc, S.d_arow, N.d_scal);
+                  S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
     if (opt.timing) tev.emplace_back(timed_event(st));
     for (int stp = 0; stp < ms; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
@@ -713,7 +728,7 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
     if (opt.timing) tev.emplace_back(timed_event(st));
     launch_gather(st, tot[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval, S.d_asrc,
-                  S.d_arow, N.d_scal);
+                  S.d_arow, S.d_acix, N.d_scal);
     if (opt.timing) tev.emplace_back(timed_event(st));
     for (int stp = 0; stp < ms; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
@@ -960,6 +975,10 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     HIPCHK(hipMemcpy(N.d_gemv_tab, tab.data(), tab.size() * sizeof(int),
                      hipMemcpyHostToDevice));
   }
+  N.big_of_level.assign(S.lvl_ptr.size() - 1, {});
+  for (int L = 0; L + 1 < (int)S.lvl_ptr.size(); ++L)
+    for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl)
+      if (N.nodes[sl].nelim > big_tri_threshold()) N.big_of_level[L].push_back(sl);
   N.solve_nrhs_cap = 0;  // X pointers need refreshing
 }
 
@@ -984,6 +1003,15 @@ static void ensure_solve_ws(Numeric &N, int nrhs) {
   }
   HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), S.nnodes * sizeof(SolveNode),
                    hipMemcpyHostToDevice));
+  size_t part = 0;
+  for (const auto &lv : N.big_of_level)
+    for (int sl : lv)
+      part = std::max(part, (size_t)(N.nodes[sl].nelim / kThreads + 1) * nrhs * kBlk);
+  if (part > N.d_tri_part_cap) {
+    if (N.d_tri_part) HIPCHK(hipFree(N.d_tri_part));
+    HIPCHK(hipMalloc(&N.d_tri_part, part * sizeof(double)));
+    N.d_tri_part_cap = part;
+  }
   N.solve_nrhs_cap = nrhs;
 }
 
@@ -997,14 +1025,22 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   bool fwd = (job == 0 || job == 1), bwd = (job == 0 || job == 3 || job == 4);
   bool diag = (job == 0 || job == 2 || job == 4);
   if (fwd)
-    for (int L = 0; L < nlev; ++L)
-      launch_fwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L],
-                 S.d_child, d_x, ldx, nrhs, N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
+    for (int L = 0; L < nlev; ++L) {
+      const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
+      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs);
+      for (int sl : N.big_of_level[L])
+        launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
+      launch_fwd_gemv(st, nl, N.d_sn, slots + S.lvl_ptr[L], nrhs,
+                      N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
+    }
   if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs);
   if (bwd)
-    for (int L = nlev - 1; L >= 0; --L)
+    for (int L = nlev - 1; L >= 0; --L) {
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
                  ldx, nrhs, N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
+      for (int sl : N.big_of_level[L])
+        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, N.d_tri_part);
+    }
   HIPCHK(hipGetLastError());
 }
 

@@ -12,10 +12,16 @@
 // Backward (root -> leaves):
 //   k_bwd_gemv: many workgroups per front: partial sums of L21^T x[rows].
 //   k_bwd_tri : one workgroup per front: reduce the partials, L11^T solve.
+// Fronts with more than kBigTri eliminated columns (the separators at the
+// top of the tree) solve their triangle with a launch sequence instead of one
+// workgroup: per 64-column block a one-workgroup diagonal solve and a
+// many-workgroup update of (forward) or partial sums over (backward) the
+// triangle's rows below the block.
 // Reference semantics: src/ssids/cpu/NumericSubtree.hxx:286-406 and
 // src/ssids/cpu/kernels/ldlt_app.cxx:2537-2589.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "dev.hpp"
@@ -23,6 +29,7 @@
 namespace ssids_hip {
 
 constexpr int kRhsChunk = 8;
+constexpr int kBigTri = 256;
 
 __device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
   int lo = 0, hi = nlvl;
@@ -103,8 +110,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
       for (int k = c.nelim + tid; k < c.m; k += kThreads) X[post_pos(P, c, k)] += Xc[k];
       __syncthreads();
     }
-    tri_lower(P, X, lb, yb);
-    for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
+    if (ne <= kBigTri) {
+      tri_lower(P, X, lb, yb);
+      for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
+    }
     __syncthreads();
   }
 }
@@ -233,6 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
       X[j] = v;
     }
     __syncthreads();
+    if (ne > kBigTri) continue;  // triangle solved by the big-front launches
     // L11^T solve, last 64-column block first
     const int nb = (ne + kBlk - 1) / kBlk;
     for (int b = nb - 1; b >= 0; --b) {
@@ -280,17 +290,176 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
   }
 }
 
+// ---- big fronts: forward -------------------------------------------------
+// X[j0:j1] <- L_bb^{-1} X[j0:j1] for every right-hand side (one workgroup,
+// one wave per right-hand side at a time)
+__global__ __launch_bounds__(kThreads) void k_tri_fdiag(const SolveNode *sn, int s, int j0,
+                                                        int nrhs) {
+  __shared__ double lb[kBlk][kBlk + 1];
+  const SolveNode &P = sn[s];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+  for (int idx = tid; idx < wb * wb; idx += kThreads) {
+    const int r = idx % wb, c = idx / wb;
+    lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
+  }
+  __syncthreads();
+  for (int r = wave; r < nrhs; r += kThreads / 64) {
+    double *X = P.X + (size_t)r * m;
+    double v = (lane < wb) ? X[j0 + lane] : 0.0;
+    for (int j = 0; j < wb; ++j) {
+      const double yj = __shfl(v, j);
+      if (lane > j && lane < wb) v -= lb[lane][j] * yj;
+    }
+    if (lane < wb) X[j0 + lane] = v;
+  }
+}
+
+// X[i] -= L[i, j0:j1] X[j0:j1] for the triangle's rows i in [j1, ne)
+__global__ __launch_bounds__(kThreads) void k_tri_frows(const SolveNode *sn, int s, int j0,
+                                                        int nrhs) {
+  __shared__ double ys[kRhsChunk][kBlk];
+  const SolveNode &P = sn[s];
+  const int tid = threadIdx.x, ne = P.nelim, m = P.m;
+  const int j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+  const int i = j1 + blockIdx.x * kThreads + tid;
+  for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
+    const int nr = min(kRhsChunk, nrhs - r0);
+    __syncthreads();
+    for (int idx = tid; idx < kRhsChunk * kBlk; idx += kThreads) {
+      const int q = idx / kBlk, j = idx % kBlk;
+      ys[q][j] = (q < nr && j < wb) ? P.X[(size_t)(r0 + q) * m + j0 + j] : 0.0;
+    }
+    __syncthreads();
+    if (i < ne) {
+      double acc[kRhsChunk];
+#pragma unroll
+      for (int q = 0; q < kRhsChunk; ++q) acc[q] = 0.0;
+      const double *lc = P.L + (size_t)j0 * P.ldl + i;
+      for (int j = 0; j < wb; ++j) {
+        const double l = lc[(size_t)j * P.ldl];
+#pragma unroll
+        for (int q = 0; q < kRhsChunk; ++q) acc[q] += l * ys[q][j];
+      }
+      for (int q = 0; q < nr; ++q) P.X[(size_t)(r0 + q) * m + i] -= acc[q];
+    }
+  }
+}
+
+// ---- big fronts: backward ------------------------------------------------
+// part[g][r][t] = sum over workgroup g's rows i in [j1, ne) of L[i, j0+t] X[i, r]
+__global__ __launch_bounds__(kThreads) void k_tri_brows(const SolveNode *sn, int s, int j0,
+                                                        int nrhs, double *part) {
+  __shared__ double red[kThreads / 64][kBlk];
+  const SolveNode &P = sn[s];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+  const int i = j1 + blockIdx.x * kThreads + tid;
+  for (int r = 0; r < nrhs; ++r) {
+    double acc[kBlk];
+#pragma unroll
+    for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
+    if (i < ne) {
+      const double xi = P.X[(size_t)r * m + i];
+      const double *lr = P.L + (size_t)j0 * P.ldl + i;
+#pragma unroll
+      for (int t = 0; t < kBlk; ++t)
+        if (t < wb) acc[t] = lr[(size_t)t * P.ldl] * xi;
+    }
+#pragma unroll
+    for (int t = 0; t < kBlk; ++t) {
+      double v = acc[t];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[wave][t] = v;
+    }
+    __syncthreads();
+    if (tid < kBlk) {
+      double v = 0.0;
+      for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
+      part[((size_t)blockIdx.x * nrhs + r) * kBlk + tid] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// X[j0:j1] -= sum of the partials, then X[j0:j1] <- L_bb^{-T} X[j0:j1]
+__global__ __launch_bounds__(kThreads) void k_tri_bdiag(const SolveNode *sn, int s, int j0,
+                                                        int ngroups, int nrhs,
+                                                        const double *part) {
+  __shared__ double lb[kBlk][kBlk + 1];
+  const SolveNode &P = sn[s];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
+  for (int idx = tid; idx < wb * wb; idx += kThreads) {
+    const int r = idx % wb, c = idx / wb;
+    lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
+  }
+  __syncthreads();
+  for (int r = wave; r < nrhs; r += kThreads / 64) {
+    double *X = P.X + (size_t)r * m;
+    double v = 0.0;
+    if (lane < wb) {
+      v = X[j0 + lane];
+      for (int g = 0; g < ngroups; ++g) v -= part[((size_t)g * nrhs + r) * kBlk + lane];
+    }
+    for (int j = wb - 1; j >= 0; --j) {
+      const double yj = __shfl(v, j);
+      if (lane < j) v -= lb[j][lane] * yj;
+    }
+    if (lane < wb) X[j0 + lane] = v;
+  }
+}
+
+// x[perm[i]] = X[i] for the front's eliminated variables
+__global__ void k_tri_scatter(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
+  const SolveNode &P = sn[s];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.nelim) return;
+  for (int r = 0; r < nrhs; ++r) x[(size_t)r * ldx + P.perm[i]] = P.X[(size_t)r * P.m + i];
+}
+
+void launch_big_fwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
+                    int nrhs) {
+  for (int j0 = 0; j0 < ne; j0 += kBlk) {
+    hipLaunchKernelGGL(k_tri_fdiag, dim3(1), dim3(kThreads), 0, st, sn, s, j0, nrhs);
+    const int rows = ne - std::min(j0 + kBlk, ne);
+    if (rows > 0)
+      hipLaunchKernelGGL(k_tri_frows, dim3((rows + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                         st, sn, s, j0, nrhs);
+  }
+  hipLaunchKernelGGL(k_tri_scatter, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
+                     nrhs);
+}
+// part: room for (ne / kThreads + 1) * nrhs * kBlk doubles
+void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
+                    int nrhs, double *part) {
+  const int nb = (ne + kBlk - 1) / kBlk;
+  for (int b = nb - 1; b >= 0; --b) {
+    const int j0 = b * kBlk, rows = ne - std::min(j0 + kBlk, ne);
+    const int ng = (rows + kThreads - 1) / kThreads;
+    if (ng > 0)
+      hipLaunchKernelGGL(k_tri_brows, dim3(ng), dim3(kThreads), 0, st, sn, s, j0, nrhs, part);
+    hipLaunchKernelGGL(k_tri_bdiag, dim3(1), dim3(kThreads), 0, st, sn, s, j0, ng, nrhs,
+                       (const double *)part);
+  }
+  hipLaunchKernelGGL(k_tri_scatter, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
+                     nrhs);
+}
+int big_tri_threshold() { return kBigTri; }
+
 void launch_ipos(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int *scratch) {
   if (nlvl > 0)
     hipLaunchKernelGGL(k_ipos, dim3(nlvl), dim3(256), 0, st, nodes, slots, scratch);
 }
-void launch_fwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                const int *child_list, double *x, int ldx, int nrhs, const int *gemv_off,
-                int gemv_wg) {
+void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
+                    const int *child_list, double *x, int ldx, int nrhs) {
   if (nlvl <= 0) return;
   hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
                      ldx, nrhs);
-  if (gemv_wg > 0)
+}
+void launch_fwd_gemv(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
+                     int nrhs, const int *gemv_off, int gemv_wg) {
+  if (nlvl > 0 && gemv_wg > 0)
     hipLaunchKernelGGL(k_fwd_gemv, dim3(gemv_wg), dim3(kThreads), 0, st, sn, slots, gemv_off,
                        nlvl, nrhs);
 }
