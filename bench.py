@@ -38,7 +38,25 @@ def parse():
     ap.add_argument("--cpu-grid", type=int, default=38,
                     help="grid of the bounded CPU-baseline sample (oracle)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+TRAFFIC = ROOT / "profiles" / "k_gather_laplace64_traffic.txt"
+
+
+def gather_traffic():
+    """HBM bytes per k_gather launch from the committed PMC summary (FETCH_SIZE
+    + WRITE_SIZE over the factorization's gather launches, scaled by the same
+    counters on a copy of known size), or None when no summary is committed."""
+    if not TRAFFIC.exists():
+        return None
+    for line in TRAFFIC.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(grid):
@@ -107,6 +125,13 @@ def main():
                                                   bwd_dev, C.byref(its), None)
         assert f == 0, f
 
+    if args.copy_mb > 0:
+        nb = args.copy_mb * (1 << 20) // 8
+        src, dst = devmem.DeviceArray(nb), devmem.DeviceArray(nb)
+        devmem.check(devmem.hip().hipMemcpy(dst.ptr, src.ptr, nb * 8, devmem.D2D))
+        sync()
+        src.free()
+        dst.free()
     for _ in range(args.warmup):
         refactor()
         solve()
@@ -150,7 +175,7 @@ def main():
     else:
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64_MFMA_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_MFMA_TFLOPS, 4)}
-    roof.update({"traffic": None,
+    roof.update({"traffic": gather_traffic(),
                  "kernel": "k_gather (extend-add + lazy Schur update, FP64 MFMA)",
                  "launches": sp.kernel_launches,
                  "avg_launch_us": round(sp.kernel_ms * 1e3 / max(1, sp.kernel_launches), 2),

@@ -721,43 +721,48 @@ __global__ __launch_bounds__(kThreads) void k_gather(
     for (int t = tid; t < kr1 - kr0; t += kThreads) mr[t] = c.pmap[kr0 + t - kb] - r0;
     for (int t = tid; t < k1 - k0; t += kThreads) mc[t] = c.pmap[k0 + t - kb] - c0;
     __syncthreads();
+    // thread (a, b0): row a of the tile's row range, columns b0, b0+4, ...
+    // (no index division; a wave reads 64 consecutive rows of one column)
+    const int ta = tid & 63, tb = tid >> 6;
     // failed (delayed) columns of the child: copy from its panel
     const int kf1 = min(k1, c.n);
     if (kf1 > k0) {
       const int nr = kr1 - kr0, ncf = kf1 - k0;
-      for (int idx = tid; idx < nr * ncf; idx += kThreads) {
-        int a = idx % nr, b = idx / nr;
-        int kr = kr0 + a, k = k0 + b;
-        if (kr >= k) acc[mr[a]][mc[b]] += c.L[(size_t)k * c.ldl + kr];
+      if (ta < nr) {
+        const int mra = mr[ta], kr = kr0 + ta;
+        for (int b = tb; b < ncf; b += kThreads / 64) {
+          const int k = k0 + b;
+          if (kr >= k) acc[mra][mc[b]] += c.L[(size_t)k * c.ldl + kr];
+        }
       }
     }
     // contribution columns: C_c - L21 (D L21)^T
     const int kc0 = max(k0, c.n);
     const int rr0 = max(kr0, c.n);
     if (k1 > kc0 && kr1 > rr0) {
-      const int nrow = kr1 - rr0, ncol = k1 - kc0, nel = nrow * ncol;
+      const int nrow = kr1 - rr0, ncol = k1 - kc0;
       // C_c, lower part, down its columns: all loads issued before any add
-      double cv[kGT * kGT / kThreads];
+      constexpr int NU = kGT / (kThreads / 64);
+      double cv[NU];
+      const bool rowok = ta < nrow;
+      const int kr = rr0 + ta;
+      const double *Cr = c.C + (kr - c.n);
 #pragma unroll
-      for (int u = 0; u < kGT * kGT / kThreads; ++u) {
-        const int idx = tid + u * kThreads;
-        cv[u] = 0.0;
-        if (idx < nel) {
-          const int kr = rr0 + idx % nrow, k = kc0 + idx / nrow;
-          if (kr >= k) cv[u] = c.C[(size_t)(k - c.n) * c.ldc + (kr - c.n)];
-        }
+      for (int u = 0; u < NU; ++u) {
+        const int b = tb + u * (kThreads / 64), k = kc0 + b;
+        cv[u] = (rowok && b < ncol && kr >= k) ? Cr[(size_t)(k - c.n) * c.ldc] : 0.0;
       }
+      if (rowok) {
+        const int mra = mr[rr0 - kr0 + ta];
 #pragma unroll
-      for (int u = 0; u < kGT * kGT / kThreads; ++u) {
-        const int idx = tid + u * kThreads;
-        if (idx < nel) {
-          const int a = idx % nrow, b = idx / nrow;
-          if (rr0 + a >= kc0 + b) acc[mr[rr0 - kr0 + a]][mc[kc0 - k0 + b]] += cv[u];
+        for (int u = 0; u < NU; ++u) {
+          const int b = tb + u * (kThreads / 64);
+          if (b < ncol && kr >= kc0 + b) acc[mra][mc[kc0 - k0 + b]] += cv[u];
         }
       }
       // L21 (D L21)^T on MFMA: each wave a 32x32 quadrant, operands in the
       // 16x16x4 fragment layout (lane i+16k holds row i, pivot k), loaded
-      // 16 pivots at a time before the MFMAs that use them
+      // 8 pivots at a time before the MFMAs that use them
       const int K = c.nelim;
       const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
       const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
@@ -767,7 +772,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      constexpr int KC = 4;  // k-steps (of 4 pivots) per chunk
+      constexpr int KC = 2;  // k-steps (of 4 pivots) per chunk
       for (int t0 = 0; t0 < K; t0 += 4 * KC) {
         double fa0[KC], fa1[KC], fb0[KC], fb1[KC];
 #pragma unroll
