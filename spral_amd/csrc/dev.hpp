@@ -21,6 +21,8 @@ constexpr int kThreads = 256;   // threads per workgroup in most kernels
 struct NodeDev {
   double *L;
   double *C;
+  double *LD;       // ldl x n: rows [n, m) of L21 D (the parent's gather operand);
+                    // allocated with C, absent for roots
   double *D;        // 2*n, D^{-1} as [d11, d21, (inf for 2x2 second), d22]
   double *Dd;       // 3*n: per pivot (D_kk, D_k,partner, partner offset 0/+1/-1)
   int *perm;        // n fully-summed variables (0-based elimination index)
@@ -63,6 +65,8 @@ struct SolveNode {
   const int *rmap;
   double *X;         // m x nrhs workspace (ld m)
   double *P;         // backward GEMV partial sums (gemv workgroups x nrhs x nelim)
+  const int *ctab;   // per child, per 64-row tile of this front: first child
+                     // trailing index whose position reaches the tile (stride T+1)
   int m, n, nd, sn, nelim, ldl, doff, cbeg, cend;
 };
 

@@ -620,6 +620,28 @@ __global__ void k_dprep(NodeDev *nodes, const int *lvl_slots) {
   }
 }
 
+// L21 D for the contribution rows [n, m) of a non-root front: the operand
+// the parent's gather multiplies with L21^T (one load per element there).
+// Workgroups (front, part): rows strided over the parts.
+constexpr int kLdParts = 8;
+__global__ __launch_bounds__(kThreads) void k_ldprep(NodeDev *nodes, const int *lvl_slots) {
+  const NodeDev &N = nodes[lvl_slots[blockIdx.x]];
+  if (N.skip || !N.LD) return;
+  const int ne = N.nelim, n = N.n, m = N.m, ldl = N.ldl;
+  const double *L = N.L, *Dd = N.Dd;
+  double *LD = N.LD;
+  for (int i = n + blockIdx.y * kThreads + threadIdx.x; i < m; i += kLdParts * kThreads) {
+#pragma unroll 8
+    for (int t = 0; t < ne; ++t) {
+      const double e = Dd[3 * t], f = Dd[3 * t + 1];
+      const int off = (int)Dd[3 * t + 2];
+      double v = L[(size_t)t * ldl + i] * e;
+      if (off != 0) v += L[(size_t)(t + off) * ldl + i] * f;
+      LD[(size_t)t * ldl + i] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Gather assembly of one level (replaces zero + scatter + extend-add + the
 // Schur update of the children's contribution blocks).  A workgroup owns a
@@ -760,50 +782,51 @@ __global__ __launch_bounds__(kThreads) void k_gather(
           if (b < ncol && kr >= kc0 + b) acc[mra][mc[kc0 - k0 + b]] += cv[u];
         }
       }
-      // L21 (D L21)^T on MFMA: each wave a 32x32 quadrant, operands in the
-      // 16x16x4 fragment layout (lane i+16k holds row i, pivot k), loaded
-      // 8 pivots at a time before the MFMAs that use them
+      // L21 (L21 D)^T on MFMA: each wave a 32x32 quadrant, operands in the
+      // 16x16x4 fragment layout (lane i+16k holds row i, pivot k) straight
+      // from L21 and the child's L21 D; two register buffers alternate so
+      // the loads of the next 16 pivots are in flight during the MFMAs
       const int K = c.nelim;
       const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
       const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
-      const double *La = c.L + rr0, *Lb = c.L + kc0;
+      const double *La = c.L + rr0, *Lb = c.LD + kc0;
       double4_t g[2][2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      constexpr int KC = 2;  // k-steps (of 4 pivots) per chunk
-      for (int t0 = 0; t0 < K; t0 += 4 * KC) {
-        double fa0[KC], fa1[KC], fb0[KC], fb1[KC];
+      constexpr int KC = 4;  // k-steps (of 4 pivots) per buffer
+      double xa0[KC], xa1[KC], xb0[KC], xb1[KC], ya0[KC], ya1[KC], yb0[KC], yb1[KC];
+      auto load = [&](int T0, double *A0, double *A1, double *B0, double *B1) {
 #pragma unroll
         for (int u = 0; u < KC; ++u) {
-          const int t = t0 + 4 * u + kq;
-          fa0[u] = fa1[u] = fb0[u] = fb1[u] = 0.0;
-          if (t < K) {
-            const size_t o = (size_t)t * c.ldl;
-            if (va0) fa0[u] = La[o + ra0];
-            if (va1) fa1[u] = La[o + ra1];
-            const double e = c.Dd[3 * t], f = c.Dd[3 * t + 1];
-            const int off = (int)c.Dd[3 * t + 2];
-            double x0 = vb0 ? Lb[o + cb0] : 0.0, x1 = vb1 ? Lb[o + cb1] : 0.0;
-            double y0 = 0.0, y1 = 0.0;
-            if (off != 0) {
-              const size_t op = (size_t)(t + off) * c.ldl;
-              if (vb0) y0 = Lb[op + cb0];
-              if (vb1) y1 = Lb[op + cb1];
-            }
-            fb0[u] = x0 * e + y0 * f;
-            fb1[u] = x1 * e + y1 * f;
-          }
+          const int t = T0 + 4 * u + kq;
+          const bool ok = t < K;
+          const size_t o = (size_t)t * c.ldl;
+          A0[u] = (ok && va0) ? La[o + ra0] : 0.0;
+          A1[u] = (ok && va1) ? La[o + ra1] : 0.0;
+          B0[u] = (ok && vb0) ? Lb[o + cb0] : 0.0;
+          B1[u] = (ok && vb1) ? Lb[o + cb1] : 0.0;
         }
+      };
+      auto mfma = [&](int T0, const double *A0, const double *A1, const double *B0,
+                      const double *B1) {
 #pragma unroll
         for (int u = 0; u < KC; ++u) {
-          if (t0 + 4 * u >= K) break;
-          g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0[u], fb0[u], g[0][0], 0, 0, 0);
-          g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa0[u], fb1[u], g[0][1], 0, 0, 0);
-          g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1[u], fb0[u], g[1][0], 0, 0, 0);
-          g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa1[u], fb1[u], g[1][1], 0, 0, 0);
+          if (T0 + 4 * u >= K) break;
+          g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B0[u], g[0][0], 0, 0, 0);
+          g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B1[u], g[0][1], 0, 0, 0);
+          g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B0[u], g[1][0], 0, 0, 0);
+          g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B1[u], g[1][1], 0, 0, 0);
         }
+      };
+      load(0, xa0, xa1, xb0, xb1);
+      for (int t0 = 0; t0 < K; t0 += 8 * KC) {
+        load(t0 + 4 * KC, ya0, ya1, yb0, yb1);
+        mfma(t0, xa0, xa1, xb0, xb1);
+        if (t0 + 4 * KC >= K) break;
+        load(t0 + 8 * KC, xa0, xa1, xb0, xb1);
+        mfma(t0 + 4 * KC, ya0, ya1, yb0, yb1);
       }
       __syncthreads();  // the C_c pass has finished with acc
 #pragma unroll
@@ -1300,6 +1323,10 @@ void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
 }
 void launch_dprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
   if (nlvl > 0) hipLaunchKernelGGL(k_dprep, dim3(nlvl), dim3(256), 0, st, nodes, slots);
+}
+void launch_ldprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_ldprep, dim3(nlvl, kLdParts), dim3(kThreads), 0, st, nodes, slots);
 }
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {

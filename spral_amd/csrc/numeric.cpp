@@ -26,6 +26,7 @@ namespace ssids_hip {
 void launch_gather(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
                    const double *, const int64_t *, const int *, const int *, const double *);
 void launch_dprep(hipStream_t, int, NodeDev *, const int *);
+void launch_ldprep(hipStream_t, int, NodeDev *, const int *);
 void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
 void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, int,
@@ -39,8 +40,9 @@ void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
-void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
-                    int, int);
+void launch_fwd_gather(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
+                       int, int, const int *, int);
+void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, double *, int, int);
 void launch_fwd_gemv(hipStream_t, int, const SolveNode *, const int *, int, const int *, int);
 void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
 void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
@@ -227,6 +229,7 @@ struct Numeric {
   int *d_slots_all = nullptr;             // 0..nnodes-1 (slots in level order)
   int *d_gemv_tab = nullptr;              // per level: nlvl+1 GEMV workgroup offsets
   std::vector<int> gemv_tab_off, gemv_total;  // per level: offset into d_gemv_tab, total
+  std::vector<int> fgat_tab_off, fgat_total;  // forward-gather tables (same array)
   std::vector<int> gemv_count;            // per slot
   std::vector<std::vector<int>> big_of_level;  // slots whose triangle needs the launch sequence
   double *d_tri_part = nullptr;           // partial sums of the big-front backward solve
@@ -491,19 +494,23 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   D.perm = (int *)keep(std::max(n, 1) * sizeof(int));
   D.perm0 = (int *)keep(std::max(n, 1) * sizeof(int));
   D.ipos = (int *)keep(std::max(n, 1) * sizeof(int));
-  const size_t cbytes = (size_t)D.ldc * D.ldc * sizeof(double);
-  D.C = cbytes ? (double *)N.pool.alloc(cbytes) : nullptr;
-  cb = {D.C, cbytes};
+  // contribution block and L21 D share one allocation (same lifetime)
+  const size_t cbytes = ((size_t)D.ldc * D.ldc * sizeof(double) + 255) & ~size_t(255);
+  const size_t ldbytes = D.is_root ? 0 : (size_t)D.ldl * n * sizeof(double);
+  char *cbase = (cbytes + ldbytes) ? (char *)N.pool.alloc(cbytes + ldbytes) : nullptr;
+  D.C = (D.ldc > 0) ? (double *)cbase : nullptr;
+  D.LD = ldbytes ? (double *)(cbase + cbytes) : nullptr;
+  cb = {cbase, cbytes + ldbytes};
   size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
   if (nsteps > 1) wb += (size_t)2 * kBlk * m * sizeof(double);
+  // child tile tables: built by the assembly, kept for the forward solve's gather
   const size_t tabn = (size_t)(cdiv(m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
-  wb += tabn * sizeof(int);
+  D.ctab = (int *)keep(tabn * sizeof(int));
   char *w = (char *)N.pool.alloc(wb);
   wk = {w, wb};
   D.bak = (double *)w;
   D.dbak = D.bak + (size_t)D.ldbak * kBlk;
   D.swp = (nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
-  D.ctab = (int *)(D.dbak + kBlk * kBlk + ((nsteps > 1) ? (size_t)2 * kBlk * m : 0));
 }
 
 static void plan_release(Numeric &N) {
@@ -633,6 +640,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     launch_finalize(st, nl, N.d_nodes, d_slots);
     launch_tpp(st, nl, N.d_nodes, d_slots, fp);
     launch_dprep(st, nl, N.d_nodes, d_slots);
+    launch_ldprep(st, nl, N.d_nodes, d_slots);
     if (lv.cap[2 * ms + 2] > 0)
       launch_update(st, lv.cap[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
     launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
@@ -743,6 +751,7 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
     launch_finalize(st, nl, N.d_nodes, d_slots);
     launch_tpp(st, nl, N.d_nodes, d_slots, fp);
     launch_dprep(st, nl, N.d_nodes, d_slots);
+    launch_ldprep(st, nl, N.d_nodes, d_slots);
     if (tot[2 * ms + 2] > 0)
       launch_update(st, tot[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
     launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
@@ -941,6 +950,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     Q.doff = doff[sl];
     Q.cbeg = D.cbeg;
     Q.cend = D.cend;
+    Q.ctab = D.ctab;
   }
   if (!N.d_sn) HIPCHK(hipMalloc(&N.d_sn, std::max(nn, 1) * sizeof(SolveNode)));
   if (!N.d_slots_all) {
@@ -969,6 +979,19 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       }
       tab.push_back(acc);
       N.gemv_total[L] = acc;
+    }
+    // forward-gather workgroups: one per 256 front positions
+    N.fgat_tab_off.assign(nlev2, 0);
+    N.fgat_total.assign(nlev2, 0);
+    for (int L = 0; L < nlev2; ++L) {
+      N.fgat_tab_off[L] = (int)tab.size();
+      int acc = 0;
+      for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) {
+        tab.push_back(acc);
+        acc += cdiv(N.nodes[sl].m, kThreads);
+      }
+      tab.push_back(acc);
+      N.fgat_total[L] = acc;
     }
     if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
     HIPCHK(hipMalloc(&N.d_gemv_tab, std::max<size_t>(tab.size(), 1) * sizeof(int)));
@@ -1027,7 +1050,9 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   if (fwd)
     for (int L = 0; L < nlev; ++L) {
       const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
-      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs);
+      launch_fwd_gather(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs,
+                        N.d_gemv_tab + N.fgat_tab_off[L], N.fgat_total[L]);
+      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], d_x, ldx, nrhs);
       for (int sl : N.big_of_level[L])
         launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
       launch_fwd_gemv(st, nl, N.d_sn, slots + S.lvl_ptr[L], nrhs,

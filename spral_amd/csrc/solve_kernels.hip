@@ -90,30 +90,52 @@ __device__ void tri_lower(const SolveNode &P, double *X, double (*lb)[kBlk + 1],
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const int *slots,
-                                                      const int *child_list, double *x,
-                                                      int ldx, int nrhs) {
-  __shared__ double lb[kBlk][kBlk + 1];
-  __shared__ double yb[kBlk];
-  const SolveNode &P = sn[slots[blockIdx.x]];
-  const int tid = threadIdx.x, m = P.m, n = P.n, ne = P.nelim;
+// X = own b + the children's updates.  Workgroup q of a front owns the
+// front positions (before pivoting) [256q, 256q+256): it sets them from b,
+// then adds each child's update in child order (the child's trailing indices
+// that land in the range come from the assembly's tile table), so every
+// entry is summed in a fixed order by one workgroup.
+__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveNode *sn, const int *slots,
+                                                         const int *child_list, double *x,
+                                                         int ldx, int nrhs, const int *wg_off,
+                                                         int nlvl) {
+  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
+  const SolveNode &P = sn[slots[li]];
+  const int tid = threadIdx.x, m = P.m, n = P.n, nd = P.nd;
+  const int q0 = (blockIdx.x - wg_off[li]) * kThreads, q1 = min(q0 + kThreads, m);
+  const int T = (m + 63) / 64;
+  const int I0 = q0 / 64, I1 = min(T, (q1 + 63) / 64);
+  const int pre = q0 + tid;
+  const int post = (pre < n) ? ((pre < q1) ? P.ipos[pre] : 0) : pre;
+  const int vrow = (pre >= nd && pre < n) ? P.rows[pre - nd] : -1;
   for (int r = 0; r < nrhs; ++r) {
     double *X = P.X + (size_t)r * m;
-    double *xr = x + (size_t)r * ldx;
-    for (int i = tid; i < m; i += kThreads) X[i] = 0.0;
-    __syncthreads();
-    for (int j = P.nd + tid; j < n; j += kThreads) X[P.ipos[j]] = xr[P.rows[j - P.nd]];
+    const double *xr = x + (size_t)r * ldx;
+    if (pre < q1) X[post] = (vrow >= 0) ? xr[vrow] : 0.0;
     __syncthreads();
     for (int ci = P.cbeg; ci < P.cend; ++ci) {
       const SolveNode &c = sn[child_list[ci]];
-      const double *Xc = c.X + (size_t)r * c.m;
-      for (int k = c.nelim + tid; k < c.m; k += kThreads) X[post_pos(P, c, k)] += Xc[k];
+      const int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
+      const int kb = tab[I0], ke = tab[I1];
+      const int k = kb + tid;  // at most 256 child indices land in the range
+      if (k < ke) X[post_pos(P, c, k)] += c.X[(size_t)r * c.m + k];
       __syncthreads();
     }
-    if (ne <= kBigTri) {
-      tri_lower(P, X, lb, yb);
-      for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
-    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const int *slots,
+                                                      double *x, int ldx, int nrhs) {
+  __shared__ double lb[kBlk][kBlk + 1];
+  __shared__ double yb[kBlk];
+  const SolveNode &P = sn[slots[blockIdx.x]];
+  const int tid = threadIdx.x, m = P.m, ne = P.nelim;
+  if (ne > kBigTri) return;  // triangle solved by the big-front launches
+  for (int r = 0; r < nrhs; ++r) {
+    double *X = P.X + (size_t)r * m;
+    double *xr = x + (size_t)r * ldx;
+    tri_lower(P, X, lb, yb);
+    for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
     __syncthreads();
   }
 }
@@ -238,6 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
     // X[j] = x[perm[j]] - sum of the gemv partials (rows >= ne)
     for (int j = tid; j < ne; j += kThreads) {
       double v = xr[P.perm[j]];
+#pragma unroll 8
       for (int g = 0; g < count; ++g) v -= partial[((size_t)g * nrhs + r) * ne + j];
       X[j] = v;
     }
@@ -451,11 +474,17 @@ void launch_ipos(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int
   if (nlvl > 0)
     hipLaunchKernelGGL(k_ipos, dim3(nlvl), dim3(256), 0, st, nodes, slots, scratch);
 }
+void launch_fwd_gather(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
+                       const int *child_list, double *x, int ldx, int nrhs, const int *wg_off,
+                       int nwg) {
+  if (nlvl > 0 && nwg > 0)
+    hipLaunchKernelGGL(k_fwd_gather, dim3(nwg), dim3(kThreads), 0, st, sn, slots, child_list,
+                       x, ldx, nrhs, wg_off, nlvl);
+}
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                    const int *child_list, double *x, int ldx, int nrhs) {
+                    double *x, int ldx, int nrhs) {
   if (nlvl <= 0) return;
-  hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
-                     ldx, nrhs);
+  hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, x, ldx, nrhs);
 }
 void launch_fwd_gemv(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                      int nrhs, const int *gemv_off, int gemv_wg) {
