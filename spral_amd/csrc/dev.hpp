@@ -33,6 +33,8 @@ struct NodeDev {
   const int64_t *acol;  // sn+1 offsets into the A-entry arrays
   double *bak;      // (m x kBlk) backup of the panel rows below the block
   double *dbak;     // kBlk x kBlk backup of the diagonal block
+  double *W;        // kBlk x kBlk row-major: inverse of the step's unit lower
+                    // diagonal block (elimination order), for k_apply
   double *swp;      // 2*kBlk x m gather buffer for the failed-column swap
   int *pmap;        // (m - nelim) parent positions of the trailing indices
   int *ctab;        // per child (stride T+1, T = 64-row tiles of this front):
@@ -78,7 +80,7 @@ __host__ __device__ inline int idiv_up(long long a, long long b) { return (int)(
 // 2S+3 tables, each an exclusive prefix over the level's fronts of the
 // workgroups one launch gives each front (last entry = total):
 //   0          gather: lower 64x64 tiles of the front
-//   1..S       apply, step t-1: t-1 chunks of earlier rows + rows below
+//   1..S       apply, step t-1: t-1 chunks of earlier rows + 64-row blocks below
 //   S+1        failed-column swap
 //   S+2..2S+1  trailing panel update, step t-S-2
 //   2S+2       contribution update of a root's TPP pivots
@@ -91,7 +93,7 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
   }
   if (t <= S) {
     const int stp = t - 1;
-    return stp < nsteps ? stp + imax(1, idiv_up(m, kThreads)) : 0;
+    return stp < nsteps ? stp + imax(1, idiv_up(m, kBlk)) : 0;
   }
   if (t == S + 1) return nsteps > 1 ? imax(1, idiv_up(2LL * kBlk * m, kThreads * 8)) : 0;
   if (t <= 2 * S + 1) {

@@ -92,6 +92,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   __shared__ int ccol[2][NW], crow[2][NW];
   __shared__ double dinv[2 * kBlk + 2];
   __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
+  __shared__ double mi[kBlk][kBlk + 1];  // inverse of L_bb (elimination order)
   const int s = lvl_slots[blockIdx.x];
   NodeDev &N = nodes[s];
   if (step >= N.nsteps) return;
@@ -287,6 +288,35 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
     N.D[2 * (p + j) + 1] = dinv[2 * j + 1];
     N.lp[j] = lp[j];
   }
+  // ---- M = L_bb^{-1} in elimination order for k_apply
+  // (L21 = A21[:, lp] M^T D^{-1}).  mi[k][i] = L_bb[i][k] (zero outside the
+  // strict lower w x w part), then forward substitution by columns: once
+  // m[k] is final it updates every later m[i]
+  for (int idx = tid; idx < kBlk * kBlk; idx += 64 * NW) {
+    const int i = idx & (kBlk - 1), k = idx / kBlk;
+    double v = 0.0;
+    if (i > k && i < w && !(kind[k] == 2 && i == k + 1)) v = t[k][lp[i]];
+    mi[k][i] = v;
+  }
+  __syncthreads();
+  {
+    // wave v: columns [8v, 8v+8) (NW = 8); lane = 8 * (column in wave) + phase,
+    // rows phase, phase+8, ... of that column in registers
+    static_assert(NW == 8, "inverse layout assumes 8 waves");
+    const int col = wv * 8 + (lane >> 3), ph = lane & 7;
+    double mm[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) mm[q] = (ph + 8 * q == col) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < kBlk - 1; ++k) {
+      const double mk = __shfl(mm[k >> 3], (lane & ~7) | (k & 7));
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (ph + 8 * q > k) mm[q] -= mi[k][ph + 8 * q] * mk;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) N.W[(ph + 8 * q) * kBlk + col] = mm[q];  // row-major
+  }
   // permute the variable list (rows of earlier eliminated columns: k_apply)
   const int pv = (tid < w) ? N.perm[p + lp[tid]] : 0;
   __syncthreads();
@@ -299,116 +329,121 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
 __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *lvl_slots,
                                                     const int *wg_off, int nlvl, int step,
                                                     FactorParams fp) {
-  __shared__ double lbb[kBlk][kBlk + 1];
+  __shared__ double ms[kBlk][kBlk + 1];  // M = L_bb^{-1}, then the left-rows chunk
+  __shared__ double xs[kBlk][kBlk + 1];  // A21[:, lp] M^T for this workgroup's 64 rows
   __shared__ double dv[2 * kBlk + 2];
   __shared__ int lps[kBlk];
-  __shared__ double cmax[kThreads / 64][kBlk];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
   if (li < 0) return;
   NodeDev &N = nodes[lvl_slots[li]];
   if (step >= N.nsteps) return;
   const int p = N.p, w = N.w;
   if (w == 0) return;
-  int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  int local = blockIdx.x - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ldl = N.ldl;
   double *L = N.L;
   // the first nleft workgroups permute rows p..p+w-1 of the eliminated
   // columns left of p into the block's elimination order (64-column chunks)
-  const int nleft = step * kBlk / kBlk;  // upper bound on ceil(p / 64)
+  const int nleft = step;  // upper bound on ceil(p / 64)
   if (local < nleft) {
     const int cb = local * kBlk;
     if (cb >= p) return;
     for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
       int r = idx % kBlk, cc = idx / kBlk;
-      if (r < w && cb + cc < p) lbb[r][cc] = L[(size_t)(cb + cc) * ldl + p + r];
+      if (r < w && cb + cc < p) ms[r][cc] = L[(size_t)(cb + cc) * ldl + p + r];
     }
     __syncthreads();
     for (int idx = tid; idx < kBlk * kBlk; idx += kThreads) {
       int r = idx % kBlk, cc = idx / kBlk;
-      if (r < w && cb + cc < p) L[(size_t)(cb + cc) * ldl + p + r] = lbb[N.lp[r]][cc];
+      if (r < w && cb + cc < p) L[(size_t)(cb + cc) * ldl + p + r] = ms[N.lp[r]][cc];
     }
     return;
   }
   local -= nleft;
-  count -= nleft;
-  for (int idx = tid; idx < w * w; idx += kThreads) {
-    int r = idx % w, c = idx / w;
-    lbb[c][r] = (r > c) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;  // transposed: lbb[col][row]
-  }
+  // ---- rows [rb, rb+64) below the block: X = A21[:, lp] M^T on MFMA
+  const int r0 = p + w, m = N.m;
+  const int rb = r0 + local * kBlk;
+  for (int idx = tid; idx < kBlk * kBlk; idx += kThreads)
+    ms[idx / kBlk][idx & (kBlk - 1)] = N.W[idx];
   for (int j = tid; j < 2 * w; j += kThreads) dv[j] = N.D[2 * p + j];
-  if (tid == 0) { dv[2 * w] = 0.0; dv[2 * w + 1] = 0.0; }
+  if (tid == 0) {
+    dv[2 * w] = 0.0;
+    dv[2 * w + 1] = 0.0;
+  }
   for (int j = tid; j < kBlk; j += kThreads) lps[j] = (j < w) ? N.lp[j] : 0;
   __syncthreads();
-  // one row per thread: the grid gives every front max(1, ceil(rows / kThreads))
-  // row workgroups
-  const int r0 = p + w, m = N.m;
-  const double small = fp.small;
-  const int r = r0 + local * kThreads + tid;
-  const bool have = r < m;
-  double x[kBlk];
+  const int ri = lane & 15, kq = lane >> 4;
+  const int r = rb + wave * 16 + ri;  // A row of this lane (fragment layout)
+  const bool rok = r < m;
+  double av[kBlk / 4];
 #pragma unroll
-  for (int j = 0; j < kBlk; ++j) x[j] = 0.0;
-  if (have) {
-#pragma unroll
-    for (int j = 0; j < kBlk; ++j)
-      if (j < w) x[j] = L[(size_t)(p + lps[j]) * ldl + r];
-#pragma unroll
-    for (int j = 0; j < kBlk; ++j)
-      if (j < w) N.bak[(size_t)j * N.ldbak + (r - r0)] = x[j];
-    // x L_bb^T = a  (forward substitution by columns: once x[t] is final it
-    // updates every later x[j], so the per-j chains interleave; each x[j]
-    // still accumulates in increasing t)
-#pragma unroll
-    for (int t = 0; t < kBlk - 1; ++t) {
-      if (t < w) {
-#pragma unroll
-        for (int j = t + 1; j < kBlk; ++j)
-          if (j < w) x[j] -= x[t] * lbb[t][j];
-      }
-    }
-    // x D^{-1}
-#pragma unroll
-    for (int j = 0; j < kBlk; ++j) {
-      if (j < w) {
-        bool two_first = (j + 1 < w) && isinf(dv[2 * j + 2]);
-        bool two_second = (j > 0) && isinf(dv[2 * j]);
-        if (two_first) {
-          double d11 = dv[2 * j], d21 = dv[2 * j + 1], d22 = dv[2 * j + 3];
-          double x1 = x[j], x2 = x[j + 1];
-          x[j] = d11 * x1 + d21 * x2;
-          x[j + 1] = d21 * x1 + d22 * x2;
-        } else if (!two_second) {
-          double d11 = dv[2 * j];
-          if (d11 == 0.0) x[j] = (fabs(x[j]) < small) ? 0.0 : INFINITY * x[j];
-          else x[j] *= d11;
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kBlk; ++j)
-      if (j < w) L[(size_t)(p + j) * ldl + r] = x[j];
+  for (int u = 0; u < kBlk / 4; ++u) {
+    const int k = 4 * u + kq;
+    av[u] = (rok && k < w) ? L[(size_t)(p + lps[k]) * ldl + r] : 0.0;
   }
-  // reduce column maxima: wave then workgroup, one atomic per column
 #pragma unroll
-  for (int j = 0; j < kBlk; ++j) {
-    double v = (have && j < w) ? fabs(x[j]) : 0.0;  // NaN propagates as a failure
-    for (int off = 32; off > 0; off >>= 1) {
-      double o = __shfl_xor(v, off);
-      v = (o > v || isnan(o)) ? o : v;
-    }
-    if (lane == 0) cmax[wave][j] = v;
+  for (int u = 0; u < kBlk / 4; ++u) {
+    const int k = 4 * u + kq;
+    if (rok && k < w) N.bak[(size_t)k * N.ldbak + (r - r0)] = av[u];
   }
+  double4_t acc[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) acc[g] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < kBlk / 4; ++u) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g)  // B[k][j] = M[j][k]
+      acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], ms[g * 16 + ri][4 * u + kq], acc[g],
+                                                    0, 0, 0);
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xs[wave * 16 + 4 * q + kq][g * 16 + ri] = acc[g][q];
   __syncthreads();
-  if (tid < w) {
-    double v = cmax[0][tid];
-    for (int i = 1; i < kThreads / 64; ++i) {
-      double o = cmax[i][tid];
+  // ---- x D^{-1}, store, column maxima (thread: row `row`, columns cb0 + 4u)
+  const int row = lane, cb0 = wave;
+  const int rr = rb + row;
+  const bool ok = rr < m;
+  const double small = fp.small;
+#pragma unroll
+  for (int u = 0; u < kBlk / 4; ++u) {
+    const int j = cb0 + 4 * u;
+    double y = 0.0;
+    if (j < w) {
+      const double xj = xs[row][j];
+      const bool two_first = (j + 1 < w) && isinf(dv[2 * j + 2]);
+      const bool two_second = (j > 0) && isinf(dv[2 * j]);
+      bool zero_piv = false;
+      if (two_first) {
+        y = dv[2 * j] * xj + dv[2 * j + 1] * xs[row][j + 1];
+      } else if (two_second) {
+        y = dv[2 * j - 1] * xs[row][j - 1] + dv[2 * j + 1] * xj;
+      } else {
+        const double d11 = dv[2 * j];
+        zero_piv = (d11 == 0.0);
+        if (zero_piv) y = (fabs(xj) < small) ? 0.0 : INFINITY * xj;
+        else y = xj * d11;
+      }
+      if (ok) {
+        L[(size_t)(p + j) * ldl + rr] = y;
+        // L21 D is the product before D^{-1} (zero for a zero pivot): the
+        // parent's gather operand, for the contribution rows
+        if (N.LD && rr >= N.n) N.LD[(size_t)(p + j) * ldl + rr] = zero_piv ? 0.0 : xj;
+      }
+    }
+    // column maximum over this wave's 64 rows (NaN propagates as a failure)
+    double v = (ok && j < w) ? fabs(y) : 0.0;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off);
       v = (o > v || isnan(o)) ? o : v;
     }
-    unsigned long long bits = isnan(v) ? 0x7ff0000000000001ull
-                                       : (unsigned long long)__double_as_longlong(v);
-    atomicMax(&N.colmax[tid], bits);
+    if (lane == 0 && j < w) {
+      const unsigned long long bits = isnan(v) ? 0x7ff0000000000001ull
+                                               : (unsigned long long)__double_as_longlong(v);
+      atomicMax(&N.colmax[j], bits);
+    }
   }
 }
 
@@ -622,17 +657,21 @@ __global__ void k_dprep(NodeDev *nodes, const int *lvl_slots) {
 
 // L21 D for the contribution rows [n, m) of a non-root front: the operand
 // the parent's gather multiplies with L21^T (one load per element there).
-// Workgroups (front, part): rows strided over the parts.
+// k_apply writes it for the APP pivots; this covers the pivots eliminated by
+// threshold partial pivoting afterwards.  Workgroups (front, part): rows
+// strided over the parts.
 constexpr int kLdParts = 8;
 __global__ __launch_bounds__(kThreads) void k_ldprep(NodeDev *nodes, const int *lvl_slots) {
   const NodeDev &N = nodes[lvl_slots[blockIdx.x]];
   if (N.skip || !N.LD) return;
   const int ne = N.nelim, n = N.n, m = N.m, ldl = N.ldl;
+  const int t0 = N.nelim_app;  // APP pivots: written by k_apply
+  if (t0 >= ne) return;
   const double *L = N.L, *Dd = N.Dd;
   double *LD = N.LD;
   for (int i = n + blockIdx.y * kThreads + threadIdx.x; i < m; i += kLdParts * kThreads) {
 #pragma unroll 8
-    for (int t = 0; t < ne; ++t) {
+    for (int t = t0; t < ne; ++t) {
       const double e = Dd[3 * t], f = Dd[3 * t + 1];
       const int off = (int)Dd[3 * t + 2];
       double v = L[(size_t)t * ldl + i] * e;

@@ -501,7 +501,7 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   D.C = (D.ldc > 0) ? (double *)cbase : nullptr;
   D.LD = ldbytes ? (double *)(cbase + cbytes) : nullptr;
   cb = {cbase, cbytes + ldbytes};
-  size_t wb = ((size_t)D.ldbak * kBlk + kBlk * kBlk) * sizeof(double);
+  size_t wb = ((size_t)D.ldbak * kBlk + 2 * kBlk * kBlk) * sizeof(double);
   if (nsteps > 1) wb += (size_t)2 * kBlk * m * sizeof(double);
   // child tile tables: built by the assembly, kept for the forward solve's gather
   const size_t tabn = (size_t)(cdiv(m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
@@ -510,7 +510,8 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   wk = {w, wb};
   D.bak = (double *)w;
   D.dbak = D.bak + (size_t)D.ldbak * kBlk;
-  D.swp = (nsteps > 1) ? D.dbak + kBlk * kBlk : nullptr;
+  D.W = D.dbak + kBlk * kBlk;
+  D.swp = (nsteps > 1) ? D.W + kBlk * kBlk : nullptr;
 }
 
 static void plan_release(Numeric &N) {
