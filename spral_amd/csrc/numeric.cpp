@@ -906,9 +906,10 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   }
   for (auto e : tev) (void)hipEventDestroy(e);
   // algorithmic work of the gather kernels: each child's Schur complement
-  // C_c - L21 D L21^T on the lower triangle (K*N*(N+1) flops, K = nelim,
+  // C_c - L21 (L21 D)^T on the lower triangle (K*N*(N+1) flops, K = nelim,
   // N = m-n), and HBM bytes = parent fronts written once (lower part) +
-  // children's contribution blocks (lower) and L21 panels read once.
+  // children's contribution blocks (lower), L21 and L21 D panels and
+  // delayed columns read once.
   double gf = 0.0, gb = 0.0;
   for (int sl = 0; sl < nn; ++sl) {
     const NodeDev &D = N.nodes[sl];
@@ -917,7 +918,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     if (D.parent >= 0) {
       double K = D.nelim, Nc = m - n;
       gf += K * Nc * (Nc + 1);
-      gb += 8.0 * (Nc * (Nc + 1) / 2 + Nc * K + (n - D.nelim) * m);  // read
+      gb += 8.0 * (Nc * (Nc + 1) / 2 + 2 * Nc * K + (n - D.nelim) * m);  // read
     }
   }
   o.kernel_flops = gf;
