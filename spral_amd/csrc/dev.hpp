@@ -72,6 +72,12 @@ struct SolveNode {
   int m, n, nd, sn, nelim, ldl, doff, cbeg, cend;
 };
 
+// Global-address-space views of device pointers read from descriptors: loads
+// and stores through them compile to global (not flat) memory instructions.
+typedef const double __attribute__((address_space(1))) gcdouble;
+typedef double __attribute__((address_space(1))) gdouble;
+typedef const int __attribute__((address_space(1))) gcint;
+
 __host__ __device__ inline int imin(int a, int b) { return a < b ? a : b; }
 __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ inline int idiv_up(long long a, long long b) { return (int)((a + b - 1) / b); }
@@ -107,6 +113,26 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
   if (!is_root || m <= n) return 0;
   const int T = idiv_up(m - n, kBlk);
   return imax(1, imin(T * (T + 1) / 2, 4096));
+}
+
+// Index li of the table segment holding workgroup b: off[li] <= b < off[li+1]
+// for a non-decreasing prefix table off[0..n]; -1 when b >= off[n].  Each
+// wave searches with 64 samples per round (ballot of off[sample] <= b), so a
+// table of up to 64^k entries costs k dependent loads.  Every lane of the
+// wave must be active; the result is wave-uniform.
+__device__ inline int table_find(const int *off, int n, int b) {
+  const int lane = threadIdx.x & 63;
+  if (b >= off[n]) return -1;
+  int lo = 0, hi = n;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int step = (hi - lo + 63) >> 6;
+    const int idx = lo + lane * step;
+    const bool le = idx < hi && off[idx] <= b;
+    const unsigned long long mask = __ballot(le);  // a prefix of the lanes; lane 0 set
+    lo += (63 - __clzll((long long)mask)) * step;
+    hi = (lo + step < hi) ? lo + step : hi;
+  }
+  return lo;
 }
 
 // value held by lane 0 (wave-uniform by construction), as a scalar

@@ -34,14 +34,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // front of workgroup b in a level table, or -1 past the table's total (grids
 // sized for the plan's capacities may be larger than the level needs)
 __device__ inline int find_node(const int *wg_off, int nlvl, int b) {
-  if (b >= wg_off[nlvl]) return -1;
-  int lo = 0, hi = nlvl;  // wg_off[lo] <= b < wg_off[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (wg_off[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+  return table_find(wg_off, nlvl, b);
 }
 
 // element (i,j), i>=j, of the front (panel for j<n, contribution block else)
@@ -735,7 +728,14 @@ __global__ __launch_bounds__(kThreads) void k_gather(
   if (li < 0) return;
   const int s = lvl_slots[li];
   NodeDev &P = nodes[s];
-  const int m = P.m, n = P.n, nd = P.nd;
+  // descriptor fields once, into registers (loads below never wait on them)
+  const int m = P.m, n = P.n, nd = P.nd, pldl = P.ldl, pldc = P.ldc;
+  const int cbeg = P.cbeg, cend = P.cend;
+  gdouble *PL = (gdouble *)P.L;
+  gdouble *PC = (gdouble *)P.C;
+  gcint *pctab = (gcint *)P.ctab;
+  gcint *prows = (gcint *)P.rows;
+  const int64_t *pacol = P.acol;
   const int T = (m + kGT - 1) / kGT;
   const int local = blockIdx.x - wg_off[li];
   if (local >= T * (T + 1) / 2) return;
@@ -749,11 +749,11 @@ __global__ __launch_bounds__(kThreads) void k_gather(
     for (int j = c0 + tid; j < min(c0 + kGT, n); j += kThreads) {
       int v;
       if (j >= nd) {
-        v = P.rows[j - nd];
+        v = prows[j - nd];
       } else {
         int doff = 0;
         v = -1;
-        for (int ci = P.cbeg; ci < P.cend; ++ci) {
+        for (int ci = cbeg; ci < cend; ++ci) {
           const NodeDev &c = nodes[child_list[ci]];
           int nf = c.n - c.nelim;
           if (j >= doff && j < doff + nf) {
@@ -767,52 +767,77 @@ __global__ __launch_bounds__(kThreads) void k_gather(
       P.perm0[j] = v;
     }
   }
+  // this tile's A entries (entry -> column precomputed), loaded up front
+  constexpr int kAE = 4;  // entries per thread held in registers
+  const int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
+  int64_t ae0 = 0, ae1 = 0;
+  if (cs1 > cs0) {
+    ae0 = pacol[cs0];
+    ae1 = pacol[cs1];
+  }
+  double av_[kAE];
+  int ar_[kAE], ac_[kAE];
+#pragma unroll
+  for (int u = 0; u < kAE; ++u) {
+    const int64_t e = ae0 + tid + (int64_t)u * kThreads;
+    ar_[u] = -1;
+    av_[u] = 0.0;
+    ac_[u] = 0;
+    if (e < ae1) {
+      ar_[u] = arow[e] + nd;
+      ac_[u] = acix[e];
+      av_[u] = aval[asrc[e]];
+    }
+  }
   __syncthreads();
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int ri = lane & 15, kq = lane >> 4;
-  for (int ci = P.cbeg; ci < P.cend; ++ci) {
+  for (int ci = cbeg; ci < cend; ++ci) {
     const NodeDev &c = nodes[child_list[ci]];
-    const int kb = c.nelim, ke = c.m;
-    if (ke <= kb) continue;
-    const int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
+    const int cn = c.n, cm = c.m, kb = c.nelim, cldl = c.ldl, cldc = c.ldc;
+    gcdouble *cL = (gcdouble *)c.L;
+    gcdouble *cLD = (gcdouble *)c.LD;
+    gcdouble *cC = (gcdouble *)c.C;
+    gcint *cpmap = (gcint *)c.pmap;
+    if (cm <= kb) continue;
+    gcint *tab = pctab + (size_t)(ci - cbeg) * (T + 1);
     const int kr0 = tab[I], kr1 = tab[I + 1];
-    if (kr1 <= kr0) continue;
     const int k0 = tab[J], k1 = tab[J + 1];
-    if (k1 <= k0 || kr1 - 1 < k0) continue;
-    for (int t = tid; t < kr1 - kr0; t += kThreads) mr[t] = c.pmap[kr0 + t - kb] - r0;
-    for (int t = tid; t < k1 - k0; t += kThreads) mc[t] = c.pmap[k0 + t - kb] - c0;
-    __syncthreads();
+    if (kr1 <= kr0 || k1 <= k0 || kr1 - 1 < k0) continue;
+    for (int t = tid; t < kr1 - kr0; t += kThreads) mr[t] = cpmap[kr0 + t - kb] - r0;
+    for (int t = tid; t < k1 - k0; t += kThreads) mc[t] = cpmap[k0 + t - kb] - c0;
     // thread (a, b0): row a of the tile's row range, columns b0, b0+4, ...
     // (no index division; a wave reads 64 consecutive rows of one column)
     const int ta = tid & 63, tb = tid >> 6;
+    const int kc0 = max(k0, cn);
+    const int rr0 = max(kr0, cn);
+    const int nrow = kr1 - rr0, ncol = k1 - kc0;
+    const bool contrib = (k1 > kc0 && kr1 > rr0);
+    // C_c, lower part, down its columns: all loads issued before any use
+    constexpr int NU = kGT / (kThreads / 64);
+    double cv[NU];
+    const bool rowok = contrib && ta < nrow;
+    const int kr = rr0 + ta;
+    gcdouble *Cr = cC + (kr - cn);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int b = tb + u * (kThreads / 64), k = kc0 + b;
+      cv[u] = (rowok && b < ncol && kr >= k) ? Cr[(size_t)(k - cn) * cldc] : 0.0;
+    }
+    __syncthreads();  // maps ready
     // failed (delayed) columns of the child: copy from its panel
-    const int kf1 = min(k1, c.n);
+    const int kf1 = min(k1, cn);
     if (kf1 > k0) {
       const int nr = kr1 - kr0, ncf = kf1 - k0;
       if (ta < nr) {
-        const int mra = mr[ta], kr = kr0 + ta;
+        const int mra = mr[ta], krr = kr0 + ta;
         for (int b = tb; b < ncf; b += kThreads / 64) {
           const int k = k0 + b;
-          if (kr >= k) acc[mra][mc[b]] += c.L[(size_t)k * c.ldl + kr];
+          if (krr >= k) acc[mra][mc[b]] += cL[(size_t)k * cldl + krr];
         }
       }
     }
-    // contribution columns: C_c - L21 (D L21)^T
-    const int kc0 = max(k0, c.n);
-    const int rr0 = max(kr0, c.n);
-    if (k1 > kc0 && kr1 > rr0) {
-      const int nrow = kr1 - rr0, ncol = k1 - kc0;
-      // C_c, lower part, down its columns: all loads issued before any add
-      constexpr int NU = kGT / (kThreads / 64);
-      double cv[NU];
-      const bool rowok = ta < nrow;
-      const int kr = rr0 + ta;
-      const double *Cr = c.C + (kr - c.n);
-#pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int b = tb + u * (kThreads / 64), k = kc0 + b;
-        cv[u] = (rowok && b < ncol && kr >= k) ? Cr[(size_t)(k - c.n) * c.ldc] : 0.0;
-      }
+    if (contrib) {
       if (rowok) {
         const int mra = mr[rr0 - kr0 + ta];
 #pragma unroll
@@ -825,10 +850,11 @@ __global__ __launch_bounds__(kThreads) void k_gather(
       // 16x16x4 fragment layout (lane i+16k holds row i, pivot k) straight
       // from L21 and the child's L21 D; two register buffers alternate so
       // the loads of the next 16 pivots are in flight during the MFMAs
-      const int K = c.nelim;
+      const int K = kb;
       const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
       const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
-      const double *La = c.L + rr0, *Lb = c.LD + kc0;
+      gcdouble *La = cL + rr0;
+      gcdouble *Lb = cLD + kc0;
       double4_t g[2][2];
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi)
@@ -841,7 +867,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(
         for (int u = 0; u < KC; ++u) {
           const int t = T0 + 4 * u + kq;
           const bool ok = t < K;
-          const size_t o = (size_t)t * c.ldl;
+          const size_t o = (size_t)t * cldl;
           A0[u] = (ok && va0) ? La[o + ra0] : 0.0;
           A1[u] = (ok && va1) ? La[o + ra1] : 0.0;
           B0[u] = (ok && vb0) ? Lb[o + cb0] : 0.0;
@@ -877,33 +903,40 @@ __global__ __launch_bounds__(kThreads) void k_gather(
             const int i = wr + bi * 16 + kq + 4 * rg;
             const int j = wc + bj * 16 + ri;
             if (i < nrow && j < ncol) {
-              const int kr = rr0 + i, k = kc0 + j;
-              if (kr >= k) acc[mr[kr - kr0]][mc[k - k0]] -= g[bi][bj][rg];
+              const int krr = rr0 + i, k = kc0 + j;
+              if (krr >= k) acc[mr[krr - kr0]][mc[k - k0]] -= g[bi][bj][rg];
             }
           }
     }
     __syncthreads();
   }
-  // A entries of the tile's own fully-summed columns (entry -> column precomputed)
-  {
-    const int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
-    if (cs1 > cs0) {
-      const int64_t e0 = P.acol[cs0], e1 = P.acol[cs1];
-      for (int64_t e = e0 + tid; e < e1; e += kThreads) {
-        const int r = arow[e] + nd;
-        if (r < r0 || r >= r0 + kGT) continue;
-        const int cc = acix[e];
-        double v = aval[asrc[e]];
-        if (scaling) v *= scaling[P.rows[r - nd]] * scaling[P.rows[cc]];
-        acc[r - r0][cc + nd - c0] += v;
-      }
-    }
+  // A entries
+#pragma unroll
+  for (int u = 0; u < kAE; ++u) {
+    const int r = ar_[u];
+    if (r < r0 || r >= r0 + kGT) continue;
+    const int cc = ac_[u];
+    double v = av_[u];
+    if (scaling) v *= scaling[prows[r - nd]] * scaling[prows[cc]];
+    acc[r - r0][cc + nd - c0] += v;
+  }
+  for (int64_t e = ae0 + tid + (int64_t)kAE * kThreads; e < ae1; e += kThreads) {
+    const int r = arow[e] + nd;
+    if (r < r0 || r >= r0 + kGT) continue;
+    const int cc = acix[e];
+    double v = aval[asrc[e]];
+    if (scaling) v *= scaling[prows[r - nd]] * scaling[prows[cc]];
+    acc[r - r0][cc + nd - c0] += v;
   }
   __syncthreads();
+  // store the tile once: panel columns to L, contribution columns to C
   for (int idx = tid; idx < kGT * kGT; idx += kThreads) {
-    int i = idx % kGT, j = idx / kGT;
-    int gr = r0 + i, gc = c0 + j;
-    if (gr < m && gc < m && gr >= gc) *front_at(P, gr, gc) = acc[i][j];
+    const int i = idx % kGT, j = idx / kGT;
+    const int gr = r0 + i, gc = c0 + j;
+    if (gr < m && gc < m && gr >= gc) {
+      if (gc < n) PL[(size_t)gc * pldl + gr] = acc[i][j];
+      else PC[(size_t)(gc - n) * pldc + (gr - n)] = acc[i][j];
+    }
   }
 }
 

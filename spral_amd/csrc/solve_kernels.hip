@@ -32,13 +32,7 @@ constexpr int kRhsChunk = 8;
 constexpr int kBigTri = 256;
 
 __device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
-  int lo = 0, hi = nlvl;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (wg_off[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+  return table_find(wg_off, nlvl, b);
 }
 
 // post-pivot parent position of child trailing index k

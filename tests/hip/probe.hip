@@ -48,3 +48,23 @@ extern "C" int probe_wave_max_u64(const unsigned long long *hin, unsigned long l
   (void)hipFree(O);
   return e == hipSuccess ? 0 : -2;
 }
+
+// table_find over a prefix table: every workgroup b of a 256-thread grid
+// records the segment the search returns (in-bounds: n + 1 table entries,
+// one output per workgroup)
+__global__ void k_probe_table_find(const int *off, int n, int *out) {
+  const int li = ssids_hip::table_find(off, n, blockIdx.x);
+  if (threadIdx.x == 0) out[blockIdx.x] = li;
+}
+
+extern "C" int probe_table_find(const int *hoff, int n, int nb, int *hout) {
+  int *O, *R;
+  if (hipMalloc(&O, (n + 1) * sizeof(int)) != hipSuccess) return -1;
+  if (hipMalloc(&R, nb * sizeof(int)) != hipSuccess) return -1;
+  (void)hipMemcpy(O, hoff, (n + 1) * sizeof(int), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_probe_table_find, dim3(nb), dim3(256), 0, 0, O, n, R);
+  hipError_t e = hipMemcpy(hout, R, nb * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(O);
+  (void)hipFree(R);
+  return e == hipSuccess ? 0 : -2;
+}

@@ -19,6 +19,7 @@ def probe(lib):
     h = C.CDLL(str(PROBE))
     h.probe_mfma_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     h.probe_wave_max_u64.argtypes = [C.c_void_p, C.c_void_p]
+    h.probe_table_find.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     return h
 
 
@@ -56,3 +57,20 @@ def test_dpp_wave_max_u64(probe, seed):
     out = np.zeros(64, np.uint64)
     assert probe.probe_wave_max_u64(keys.ctypes.data, out.ctypes.data) == 0
     assert np.all(out == keys.max())
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 3000])
+def test_table_find(probe, n):
+    """The wave-ballot search that maps a workgroup to its front in a level
+    table equals searchsorted, including empty segments and workgroups past
+    the total (-1)."""
+    rng = np.random.default_rng(n)
+    cnt = rng.integers(0, 5, n)
+    cnt[rng.random(n) < 0.3] = 0
+    off = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    nb = int(off[-1]) + 3
+    out = np.zeros(nb, np.int32)
+    assert probe.probe_table_find(off.ctypes.data, n, nb, out.ctypes.data) == 0
+    b = np.arange(nb)
+    exp = np.where(b < off[-1], np.searchsorted(off, b, side="right") - 1, -1)
+    assert np.array_equal(out, exp)
