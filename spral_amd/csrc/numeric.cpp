@@ -40,13 +40,15 @@ void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
-void launch_fwd_gather(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
-                       int, int, const int *, int);
-void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, double *, int, int);
-void launch_fwd_gemv(hipStream_t, int, const SolveNode *, const int *, int, const int *, int);
+void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
+                    int, int);
+void launch_fwd_rows(hipStream_t, int, const SolveNode *, const int *, const int *, int,
+                     const int *, int);
 void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
-void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
+void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
 int big_tri_threshold();
+int fwd_rows_per_wg();
+int bwd_rows_per_wg();
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
                 const int *, int);
@@ -227,13 +229,13 @@ struct Numeric {
   // solve
   SolveNode *d_sn = nullptr;
   int *d_slots_all = nullptr;             // 0..nnodes-1 (slots in level order)
-  int *d_gemv_tab = nullptr;              // per level: nlvl+1 GEMV workgroup offsets
-  std::vector<int> gemv_tab_off, gemv_total;  // per level: offset into d_gemv_tab, total
-  std::vector<int> fgat_tab_off, fgat_total;  // forward-gather tables (same array)
-  std::vector<int> gemv_count;            // per slot
+  // per level: nlvl+1 workgroup offsets of the solve's row kernels (forward:
+  // 64-row workgroups, backward: 256-row workgroups), one array
+  int *d_gemv_tab = nullptr;
+  std::vector<int> gemv_tab_off, gemv_total;  // forward: offset into d_gemv_tab, total
+  std::vector<int> bwd_tab_off, bwd_total;    // backward
+  std::vector<int> bwd_count;                 // per slot: backward row workgroups
   std::vector<std::vector<int>> big_of_level;  // slots whose triangle needs the launch sequence
-  double *d_tri_part = nullptr;           // partial sums of the big-front backward solve
-  size_t d_tri_part_cap = 0;
   std::vector<SolveNode> sn_h;
   double *d_X = nullptr;
   size_t d_X_cap = 0;
@@ -263,7 +265,6 @@ struct Numeric {
     if (d_sn) (void)hipFree(d_sn);
     if (d_slots_all) (void)hipFree(d_slots_all);
     if (d_gemv_tab) (void)hipFree(d_gemv_tab);
-    if (d_tri_part) (void)hipFree(d_tri_part);
     if (d_X) (void)hipFree(d_X);
     if (d_x) (void)hipFree(d_x);
     if (d_aval) (void)hipFree(d_aval);
@@ -962,38 +963,35 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
                      hipMemcpyHostToDevice));
   }
-  // solve GEMV workgroup tables (depend on nelim)
+  // solve row-workgroup tables (depend on nelim): every front gets at least
+  // one, which in the backward solve also reduces and solves the triangle
   {
     const int nlev2 = (int)S.lvl_ptr.size() - 1;
     std::vector<int> tab;
     N.gemv_tab_off.assign(nlev2, 0);
     N.gemv_total.assign(nlev2, 0);
-    N.gemv_count.assign(nn, 0);
+    N.bwd_tab_off.assign(nlev2, 0);
+    N.bwd_total.assign(nlev2, 0);
+    N.bwd_count.assign(nn, 0);
     for (int L = 0; L < nlev2; ++L) {
       N.gemv_tab_off[L] = (int)tab.size();
       int acc = 0;
       for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) {
         tab.push_back(acc);
-        const NodeDev &D = N.nodes[sl];
-        int c = (D.m > D.nelim) ? std::min(64, cdiv(D.m - D.nelim, kThreads)) : 0;
-        N.gemv_count[sl] = c;
-        acc += c;
+        acc += cdiv(N.nodes[sl].m - N.nodes[sl].nelim, fwd_rows_per_wg());
       }
       tab.push_back(acc);
       N.gemv_total[L] = acc;
-    }
-    // forward-gather workgroups: one per 256 front positions
-    N.fgat_tab_off.assign(nlev2, 0);
-    N.fgat_total.assign(nlev2, 0);
-    for (int L = 0; L < nlev2; ++L) {
-      N.fgat_tab_off[L] = (int)tab.size();
-      int acc = 0;
+      N.bwd_tab_off[L] = (int)tab.size();
+      acc = 0;
       for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) {
         tab.push_back(acc);
-        acc += cdiv(N.nodes[sl].m, kThreads);
+        const int c = cdiv(N.nodes[sl].m - N.nodes[sl].nelim, bwd_rows_per_wg());
+        N.bwd_count[sl] = c;
+        acc += c;
       }
       tab.push_back(acc);
-      N.fgat_total[L] = acc;
+      N.bwd_total[L] = acc;
     }
     if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
     HIPCHK(hipMalloc(&N.d_gemv_tab, std::max<size_t>(tab.size(), 1) * sizeof(int)));
@@ -1010,10 +1008,14 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
 static void ensure_solve_ws(Numeric &N, int nrhs) {
   Symbolic &S = *N.symb;
   if (nrhs <= N.solve_nrhs_cap) return;
+  // per front: X (m x nrhs), then the backward partial sums (row workgroups x
+  // nrhs x nelim), at least nrhs x nelim: big fronts keep their triangle's
+  // solution there
+  auto part_size = [&](int sl) {
+    return (size_t)std::max(N.bwd_count[sl], 1) * nrhs * std::max(N.nodes[sl].nelim, 1);
+  };
   size_t tot = 0;
-  for (int sl = 0; sl < S.nnodes; ++sl)
-    tot += (size_t)N.nodes[sl].m * nrhs +
-           (size_t)N.gemv_count[sl] * nrhs * std::max(N.nodes[sl].nelim, 1);
+  for (int sl = 0; sl < S.nnodes; ++sl) tot += (size_t)N.nodes[sl].m * nrhs + part_size(sl);
   if (tot > N.d_X_cap) {
     if (N.d_X) HIPCHK(hipFree(N.d_X));
     HIPCHK(hipMalloc(&N.d_X, std::max(tot, size_t(1)) * sizeof(double)));
@@ -1024,19 +1026,10 @@ static void ensure_solve_ws(Numeric &N, int nrhs) {
     N.sn_h[sl].X = N.d_X + offx;
     offx += (size_t)N.nodes[sl].m * nrhs;
     N.sn_h[sl].P = N.d_X + offx;
-    offx += (size_t)N.gemv_count[sl] * nrhs * std::max(N.nodes[sl].nelim, 1);
+    offx += part_size(sl);
   }
   HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), S.nnodes * sizeof(SolveNode),
                    hipMemcpyHostToDevice));
-  size_t part = 0;
-  for (const auto &lv : N.big_of_level)
-    for (int sl : lv)
-      part = std::max(part, (size_t)(N.nodes[sl].nelim / kThreads + 1) * nrhs * kBlk);
-  if (part > N.d_tri_part_cap) {
-    if (N.d_tri_part) HIPCHK(hipFree(N.d_tri_part));
-    HIPCHK(hipMalloc(&N.d_tri_part, part * sizeof(double)));
-    N.d_tri_part_cap = part;
-  }
   N.solve_nrhs_cap = nrhs;
 }
 
@@ -1052,21 +1045,19 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   if (fwd)
     for (int L = 0; L < nlev; ++L) {
       const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
-      launch_fwd_gather(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs,
-                        N.d_gemv_tab + N.fgat_tab_off[L], N.fgat_total[L]);
-      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], d_x, ldx, nrhs);
+      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs);
       for (int sl : N.big_of_level[L])
         launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
-      launch_fwd_gemv(st, nl, N.d_sn, slots + S.lvl_ptr[L], nrhs,
+      launch_fwd_rows(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, nrhs,
                       N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
     }
   if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs);
   if (bwd)
     for (int L = nlev - 1; L >= 0; --L) {
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
-                 ldx, nrhs, N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
+                 ldx, nrhs, N.d_gemv_tab + N.bwd_tab_off[L], N.bwd_total[L]);
       for (int sl : N.big_of_level[L])
-        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, N.d_tri_part);
+        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
     }
   HIPCHK(hipGetLastError());
 }

@@ -1,22 +1,29 @@
 // Forward / diagonal / backward solve kernels over the resident factors.
 //
-// Forward (levels leaves -> root), per front s with nelim = ne:
-//   k_fwd_tri : one workgroup per front: X = own b + children's updates
-//               (children in order: deterministic), then the unit-lower
-//               ne x ne triangle (64-column blocks staged in LDS); the
-//               eliminated values y go to x by variable.
-//   k_fwd_gemv: many workgroups per front: X[ne:m] -= L21 y (rows split
-//               across workgroups, up to 8 right-hand sides per pass so L21
-//               is read once per 8 RHS).  X[ne:m] is the update the parent
-//               gathers.
-// Backward (root -> leaves):
-//   k_bwd_gemv: many workgroups per front: partial sums of L21^T x[rows].
-//   k_bwd_tri : one workgroup per front: reduce the partials, L11^T solve.
-// Fronts with more than kBigTri eliminated columns (the separators at the
-// top of the tree) solve their triangle with a launch sequence instead of one
-// workgroup: per 64-column block a one-workgroup diagonal solve and a
-// many-workgroup update of (forward) or partial sums over (backward) the
-// triangle's rows below the block.
+// Positions of a front (after pivoting): [0, ne) eliminated, [ne, n) failed
+// fully-summed columns passed on to the parent, [n, m) the rows below.  X
+// (m x nrhs per front, ld m) holds the forward values of the front's positions.
+//
+// Forward (levels leaves -> root), two launches per level:
+//   k_fwd_tri : one workgroup per front: X[0:n] = own b + the children's
+//               updates that land on fully-summed positions, then the
+//               unit-lower triangle on X[0:ne] (fronts with ne <= kBigTri);
+//               the eliminated values go to x by variable.
+//   k_fwd_rows: 64-row workgroups over [ne, m): the children's updates that
+//               land on the rows (summed in LDS in a fixed order) minus
+//               L[rows, 0:ne] X[0:ne].  X[ne:m] is what the parent gathers.
+// Backward (root -> leaves), one launch per level:
+//   k_bwd     : 64-row workgroups over [ne, m) write partial sums of
+//               L[rows, 0:ne]^T x[rows]; the last workgroup of a front to
+//               finish (device-scope counter) reduces them in workgroup order
+//               and solves the triangle with L11^T.
+// Fronts with ne > kBigTri (separators at the top of the tree) solve their
+// triangle with one launch per 64-column block (k_big_fwd / k_big_bwd).
+//
+// Loads of L are issued in batches of 16 independent addresses (column indices
+// clamped into the block instead of guarded, the surplus products multiplied
+// by zero or discarded), and the 64-step diagonal-block substitutions keep the
+// block in registers and broadcast each solved value with a lane read.
 // Reference semantics: src/ssids/cpu/NumericSubtree.hxx:286-406 and
 // src/ssids/cpu/kernels/ldlt_app.cxx:2537-2589.
 #include <hip/hip_runtime.h>
@@ -28,17 +35,96 @@
 
 namespace ssids_hip {
 
+constexpr int kRows = 64;                  // rows per workgroup of the row kernels
+constexpr int kGroups = kThreads / kRows;  // column groups (one wave each)
+constexpr int kYChunk = 256;               // columns of X[0:ne] staged per pass
 constexpr int kRhsChunk = 8;
 constexpr int kBigTri = 256;
+constexpr int kTriCap = 1024;  // fully-summed positions gathered in LDS
 
 __device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
   return table_find(wg_off, nlvl, b);
 }
 
-// post-pivot parent position of child trailing index k
-__device__ inline int post_pos(const SolveNode &P, const SolveNode &c, int k) {
-  int pre = (k < c.n) ? c.doff + (k - c.nelim) : P.nd + c.rmap[k - c.n];
-  return (pre < P.n) ? P.ipos[pre] : pre;
+// v if keep, else +0.0.  The value passes through an empty asm first, so the
+// selection is not turned into a branch around the load that produced it
+// (which would serialize a batch of loads behind their uses).
+__device__ inline double masked(double v, bool keep) {
+  asm("" : "+v"(v));
+  return keep ? v : 0.0;
+}
+
+// value of v in lane j (j uniform)
+__device__ inline double bcast_lane(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a[t] for every t, spread over the lanes: the
+// result in a lane is the sum for t = lane >> (6 - log2 T).  Halving
+// exchanges (T - 1 shuffles) then butterflies over the remaining lane bits.
+template <int T>
+__device__ inline double wave_reduce_scatter(double (&a)[T], int lane) {
+  constexpr int k = (T >= 64) ? 6 : (T >= 32) ? 5 : (T >= 16) ? 4 : (T >= 8) ? 3
+                  : (T >= 4) ? 2 : (T >= 2) ? 1 : 0;
+#pragma unroll
+  for (int s = 0; s < k; ++s) {
+    const int mask = 32 >> s;
+    const bool up = (lane & mask) != 0;
+#pragma unroll
+    for (int i = 0; i < (T >> (s + 1)); ++i) {
+      const double send = up ? a[i] : a[i + (T >> (s + 1))];
+      const double keep = up ? a[i + (T >> (s + 1))] : a[i];
+      a[i] = keep + __shfl_xor(send, mask);
+    }
+  }
+  double v = a[0];
+#pragma unroll
+  for (int mask = 32 >> k; mask > 0; mask >>= 1) v += __shfl_xor(v, mask);
+  return v;
+}
+
+// One wave: acc[j0:j0+wb] <- L_bb^{-1} acc[j0:j0+wb] (unit lower, forward
+// substitution); lane i holds row i of the block.  Returns the lane's value
+// (0 for lanes >= wb).
+__device__ inline double fwd_diag_wave(gcdouble *PL, int ldl, int j0, int wb, double v,
+                                       int lane) {
+  double l[kBlk];
+  const int ri = j0 + imin(lane, wb - 1);
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) {
+    l[j] = masked(PL[(size_t)(j0 + imin(j, wb - 1)) * ldl + ri], j < lane && lane < wb);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // all loads issued before the chain
+  if (lane >= wb) v = 0.0;
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) v -= l[j] * bcast_lane(v, j);
+  return v;
+}
+
+// One wave: x_b <- L_bb^{-T} x_b (backward substitution); lane j reads
+// column j of the block from lb (lb[i][j] = L[j0+i, j0+j] for i > j, else 0).
+__device__ inline double bwd_diag_wave(const double (*lb)[kBlk + 1], double v, int lane) {
+  double c[kBlk];
+#pragma unroll
+  for (int i = 0; i < kBlk; ++i) c[i] = lb[i][lane];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int i = kBlk - 1; i >= 0; --i) v -= c[i] * bcast_lane(v, i);
+  return v;
+}
+
+// lb[i][j] = L[j0+i, j0+j] for j < i < wb, else 0 (whole workgroup)
+__device__ inline void stage_lower(gcdouble *PL, int ldl, int j0, int wb,
+                                   double (*lb)[kBlk + 1]) {
+#pragma unroll
+  for (int it = 0; it < kBlk * kBlk / kThreads; ++it) {
+    const int idx = threadIdx.x + it * kThreads, i = idx & (kBlk - 1), j = idx / kBlk;
+    lb[i][j] = masked(PL[(size_t)(j0 + imin(j, wb - 1)) * ldl + j0 + imin(i, wb - 1)],
+                      j < i && i < wb);
+  }
 }
 
 __global__ void k_ipos(NodeDev *nodes, const int *slots, int *scratch) {
@@ -50,125 +136,220 @@ __global__ void k_ipos(NodeDev *nodes, const int *slots, int *scratch) {
   for (int j = threadIdx.x; j < N.n; j += blockDim.x) N.ipos[j] = scratch[N.perm0[j]];
 }
 
-// unit-lower triangle solve of X[0:ne] (one workgroup), 64-column blocks:
-// the diagonal block is staged in LDS and solved by one wave, the rows below
-// inside the triangle are updated by the whole workgroup.
-__device__ void tri_lower(const SolveNode &P, double *X, double (*lb)[kBlk + 1], double *yb) {
-  const int tid = threadIdx.x, ne = P.nelim;
-  for (int j0 = 0; j0 < ne; j0 += kBlk) {
-    const int j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-    for (int idx = tid; idx < wb * wb; idx += kThreads) {
-      int r = idx % wb, c = idx / wb;
-      lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
-    }
-    __syncthreads();
-    if (tid < 64) {
-      double v = (tid < wb) ? X[j0 + tid] : 0.0;
-      for (int j = 0; j < wb; ++j) {
-        double yj = __shfl(v, j);
-        if (tid > j && tid < wb) v -= lb[tid][j] * yj;
-      }
-      if (tid < wb) {
-        X[j0 + tid] = v;
-        yb[tid] = v;
-      }
-    }
-    __syncthreads();
-    for (int i = j1 + tid; i < ne; i += kThreads) {
-      double acc = X[i];
-      const double *lc = P.L + (size_t)j0 * P.ldl + i;
-      for (int j = 0; j < wb; ++j) acc -= lc[(size_t)j * P.ldl] * yb[j];
-      X[i] = acc;
-    }
-    __syncthreads();
-  }
-}
-
-// X = own b + the children's updates.  Workgroup q of a front owns the
-// front positions (before pivoting) [256q, 256q+256): it sets them from b,
-// then adds each child's update in child order (the child's trailing indices
-// that land in the range come from the assembly's tile table), so every
-// entry is summed in a fixed order by one workgroup.
-__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveNode *sn, const int *slots,
-                                                         const int *child_list, double *x,
-                                                         int ldx, int nrhs, const int *wg_off,
-                                                         int nlvl) {
-  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
-  const SolveNode &P = sn[slots[li]];
-  const int tid = threadIdx.x, m = P.m, n = P.n, nd = P.nd;
-  const int q0 = (blockIdx.x - wg_off[li]) * kThreads, q1 = min(q0 + kThreads, m);
-  const int T = (m + 63) / 64;
-  const int I0 = q0 / 64, I1 = min(T, (q1 + 63) / 64);
-  const int pre = q0 + tid;
-  const int post = (pre < n) ? ((pre < q1) ? P.ipos[pre] : 0) : pre;
-  const int vrow = (pre >= nd && pre < n) ? P.rows[pre - nd] : -1;
-  for (int r = 0; r < nrhs; ++r) {
-    double *X = P.X + (size_t)r * m;
-    const double *xr = x + (size_t)r * ldx;
-    if (pre < q1) X[post] = (vrow >= 0) ? xr[vrow] : 0.0;
-    __syncthreads();
-    for (int ci = P.cbeg; ci < P.cend; ++ci) {
-      const SolveNode &c = sn[child_list[ci]];
-      const int *tab = P.ctab + (size_t)(ci - P.cbeg) * (T + 1);
-      const int kb = tab[I0], ke = tab[I1];
-      const int k = kb + tid;  // at most 256 child indices land in the range
-      if (k < ke) X[post_pos(P, c, k)] += c.X[(size_t)r * c.m + k];
-      __syncthreads();
-    }
-  }
-}
-
+// ---- forward --------------------------------------------------------------
+// One workgroup per front.  X[0:n] = own b + the children's updates on
+// fully-summed positions (children taken four at a time, a wave each; their
+// adds applied wave by wave, so every position is summed in a fixed order),
+// then the triangle X[0:ne] <- L11^{-1} X[0:ne] when ne <= kBigTri.
 __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const int *slots,
-                                                      double *x, int ldx, int nrhs) {
-  __shared__ double lb[kBlk][kBlk + 1];
+                                                      const int *child_list, double *x,
+                                                      int ldx, int nrhs) {
+  __shared__ double xs[kTriCap];
   __shared__ double yb[kBlk];
+  __shared__ int nchunk[kGroups];
   const SolveNode &P = sn[slots[blockIdx.x]];
-  const int tid = threadIdx.x, m = P.m, ne = P.nelim;
-  if (ne > kBigTri) return;  // triangle solved by the big-front launches
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gcint *Pipos = (gcint *)P.ipos;
+  gcint *Prows = (gcint *)P.rows;
+  gcint *Pctab = (gcint *)P.ctab;
+  gdouble *PX = (gdouble *)P.X;
+  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
+  const int cbeg = P.cbeg, cend = P.cend;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int T = idiv_up(m, 64), Tn = idiv_up(n, 64);
   for (int r = 0; r < nrhs; ++r) {
-    double *X = P.X + (size_t)r * m;
+    gdouble *X = PX + (size_t)r * m;
     double *xr = x + (size_t)r * ldx;
-    tri_lower(P, X, lb, yb);
-    for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
-    __syncthreads();
-  }
-}
-
-// X[ne:m, r] -= L[ne:m, 0:ne] X[0:ne, r] for rows of this workgroup
-__global__ __launch_bounds__(kThreads) void k_fwd_gemv(const SolveNode *sn, const int *slots,
-                                                       const int *wg_off, int nlvl, int nrhs) {
-  __shared__ double ys[kRhsChunk][kBlk];
-  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
-  const SolveNode &P = sn[slots[li]];
-  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
-  const int tid = threadIdx.x, m = P.m, ne = P.nelim;
-  for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
-    const int nr = min(kRhsChunk, nrhs - r0);
-    for (int i0 = ne + local * kThreads; i0 < m; i0 += count * kThreads) {
-      const int i = i0 + tid;
-      double acc[kRhsChunk];
-#pragma unroll
-      for (int q = 0; q < kRhsChunk; ++q) acc[q] = 0.0;
-      for (int j0 = 0; j0 < ne; j0 += kBlk) {
-        const int wb = min(kBlk, ne - j0);
-        __syncthreads();
-        for (int idx = tid; idx < kRhsChunk * kBlk; idx += kThreads) {
-          int q = idx / kBlk, j = idx % kBlk;
-          ys[q][j] = (q < nr && j < wb) ? P.X[(size_t)(r0 + q) * m + j0 + j] : 0.0;
-        }
-        __syncthreads();
-        if (i < m) {
-          const double *lc = P.L + (size_t)j0 * P.ldl + i;
-          for (int j = 0; j < wb; ++j) {
-            double l = lc[(size_t)j * P.ldl];
-#pragma unroll
-            for (int q = 0; q < kRhsChunk; ++q) acc[q] += l * ys[q][j];
+    double *acc = (n <= kTriCap) ? xs : (double *)X;
+    for (int p = tid; p < n; p += kThreads) {
+      const int vrow = (p >= nd) ? Prows[p - nd] : -1;
+      acc[Pipos[p]] = (vrow >= 0) ? xr[vrow] : 0.0;
+    }
+    for (int c0 = cbeg; c0 < cend; c0 += kGroups) {
+      const int ci = c0 + wave;
+      int kb = 0, ke = 0, cn = 0, cne = 0, cdoff = 0, cm = 0;
+      gcint *crmap = nullptr;
+      gcdouble *cX = nullptr;
+      if (ci < cend) {
+        const SolveNode &c = sn[child_list[ci]];
+        cn = c.n;
+        cne = c.nelim;
+        cdoff = c.doff;
+        cm = c.m;
+        crmap = (gcint *)c.rmap;
+        cX = (gcdouble *)c.X;
+        gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
+        kb = tab[0];
+        ke = tab[Tn];
+      }
+      if (lane == 0) nchunk[wave] = idiv_up(ke - kb, 64);
+      __syncthreads();
+      const int nch = imax(imax(nchunk[0], nchunk[1]), imax(nchunk[2], nchunk[3]));
+      __syncthreads();  // nchunk is rewritten by the next group
+      for (int it = 0; it < nch; ++it) {
+        const int k = kb + it * 64 + lane;
+        int pos = -1;
+        double val = 0.0;
+        if (k < ke) {
+          const int pre = (k < cn) ? cdoff + (k - cne) : nd + crmap[k - cn];
+          if (pre < n) {
+            pos = Pipos[pre];
+            val = cX[(size_t)r * cm + k];
           }
         }
+        for (int w = 0; w < kGroups; ++w) {
+          if (wave == w && pos >= 0) acc[pos] += val;
+          __syncthreads();
+        }
       }
-      if (i < m)
-        for (int q = 0; q < nr; ++q) P.X[(size_t)(r0 + q) * m + i] -= acc[q];
     }
+    __syncthreads();
+    if (ne <= kBigTri) {
+      for (int j0 = 0; j0 < ne; j0 += kBlk) {
+        const int j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+        if (wave == 0) {
+          const double v =
+              fwd_diag_wave(PL, ldl, j0, wb, (lane < wb) ? acc[j0 + lane] : 0.0, lane);
+          if (lane < wb) acc[j0 + lane] = v;
+          yb[lane] = v;
+        }
+        __syncthreads();
+        const int rows = ne - j1;  // rows of the triangle below the block
+        if (tid < rows) {
+          const int i = j1 + tid;
+          double s = 0.0;
+#pragma unroll
+          for (int u0 = 0; u0 < kBlk; u0 += 16) {
+            double lv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+              lv[u] = PL[(size_t)(j0 + imin(u0 + u, wb - 1)) * ldl + i];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) s += lv[u] * yb[u0 + u];
+          }
+          acc[i] -= s;
+        }
+        __syncthreads();
+      }
+      for (int p = tid; p < ne; p += kThreads) xr[Pperm[p]] = acc[p];
+    }
+    if (acc != (double *)X)
+      for (int p = tid; p < n; p += kThreads) X[p] = acc[p];
+    __syncthreads();
+  }
+}
+
+// 64-row workgroups over [ne, m) of every front of the level (RC right-hand
+// sides per pass):
+//   X[i] = (gathered fully-summed value, i < n) or (sum of the children's
+//          updates on row i, i >= n)  -  L[i, 0:ne] X[0:ne]
+// Thread (row rr, group g) takes columns g, g+4, ...; the four groups' sums
+// are added in group order.
+template <int RC>
+__global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, const int *slots,
+                                                       const int *child_list,
+                                                       const int *wg_off, int nlvl, int nrhs) {
+  __shared__ double xs[RC][kRows];
+  __shared__ double ys[RC][kYChunk];
+  __shared__ double red[kGroups][RC][kRows];
+  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
+  const SolveNode &P = sn[slots[li]];
+  const int ne = P.nelim, m = P.m;
+  const int r0 = ne + (blockIdx.x - wg_off[li]) * kRows, r1 = imin(m, r0 + kRows);
+  if (r0 >= r1) return;  // front without rows below
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pctab = (gcint *)P.ctab;
+  gdouble *PX = (gdouble *)P.X;
+  const int ldl = P.ldl, n = P.n, nd = P.nd, cbeg = P.cbeg, cend = P.cend;
+  const int tid = threadIdx.x, lane = tid & 63, rr = lane, g = tid >> 6;
+  const int row = imin(r0 + rr, r1 - 1);
+  const int T = idiv_up(m, 64);
+  const int lo = imax(r0, n), I0 = lo / 64, I1 = idiv_up(r1, 64);
+  for (int rc = 0; rc < nrhs; rc += RC) {
+    const int nr = imin(RC, nrhs - rc);
+    for (int idx = tid; idx < RC * kRows; idx += kThreads) {
+      const int q = idx / kRows, i = idx % kRows, rw = r0 + i;
+      xs[q][i] = (q < nr && rw < n && rw < r1) ? PX[(size_t)(rc + q) * m + rw] : 0.0;
+    }
+    __syncthreads();
+    if (lo < r1) {
+      // children four at a time (a wave each); a 64-row range holds at most
+      // 128 positions, so at most two indices per lane
+      for (int c0 = cbeg; c0 < cend; c0 += kGroups) {
+        const int ci = c0 + g;
+        int pos[2] = {-1, -1};
+        double val[2][RC];
+        if (ci < cend) {
+          const SolveNode &c = sn[child_list[ci]];
+          const int cn = c.n, cne = c.nelim, cdoff = c.doff, cm = c.m;
+          gcint *crmap = (gcint *)c.rmap;
+          gcdouble *cX = (gcdouble *)c.X;
+          gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
+          const int kb = tab[I0], ke = tab[I1];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int k = kb + lane + 64 * h;
+            if (k < ke) {
+              const int pre = (k < cn) ? cdoff + (k - cne) : nd + crmap[k - cn];
+              if (pre >= lo && pre < r1) {
+                pos[h] = pre - r0;
+#pragma unroll
+                for (int q = 0; q < RC; ++q)
+                  val[h][q] = (q < nr) ? cX[(size_t)(rc + q) * cm + k] : 0.0;
+              }
+            }
+          }
+        }
+        for (int w = 0; w < kGroups; ++w) {
+          if (g == w) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+              if (pos[h] >= 0) {
+#pragma unroll
+                for (int q = 0; q < RC; ++q) xs[q][pos[h]] += val[h][q];
+              }
+          }
+          __syncthreads();
+        }
+      }
+    }
+    double acc[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+    for (int j0 = 0; j0 < ne; j0 += kYChunk) {
+      const int jw = imin(kYChunk, ne - j0);
+      for (int idx = tid; idx < RC * kYChunk; idx += kThreads) {
+        const int q = idx / kYChunk, j = idx % kYChunk;
+        ys[q][j] = (q < nr && j < jw) ? PX[(size_t)(rc + q) * m + j0 + j] : 0.0;
+      }
+      __syncthreads();
+      for (int t0 = 0; t0 < kYChunk / kGroups && kGroups * t0 < jw; t0 += 16) {
+        double lv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int jl = g + kGroups * (t0 + u);
+          lv[u] = PL[(size_t)(j0 + imin(jl, jw - 1)) * ldl + row];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int jl = g + kGroups * (t0 + u);
+#pragma unroll
+          for (int q = 0; q < RC; ++q) acc[q] += lv[u] * ys[q][jl];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q) red[g][q][rr] = acc[q];
+    __syncthreads();
+    for (int idx = tid; idx < RC * kRows; idx += kThreads) {
+      const int q = idx / kRows, i = idx % kRows;
+      if (q < nr && r0 + i < r1)
+        PX[(size_t)(rc + q) * m + r0 + i] =
+            xs[q][i] - (((red[0][q][i] + red[1][q][i]) + red[2][q][i]) + red[3][q][i]);
+    }
+    __syncthreads();
   }
 }
 
@@ -176,328 +357,349 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
   const int s = blockIdx.x;
   if (s >= nslots) return;
   const SolveNode &P = sn[s];
+  gcint *Pperm = (gcint *)P.perm;
+  gcdouble *d = (gcdouble *)P.D;
   const int ne = P.nelim;
-  const double *d = P.D;
   for (int i = threadIdx.x; i < ne; i += blockDim.x) {
     bool first2 = (i + 1 < ne) && !isfinite(d[2 * i + 2]);
     bool second2 = !isfinite(d[2 * i]);
     for (int r = 0; r < nrhs; ++r) {
       double *xr = x + (size_t)r * ldx;
       if (first2) {
-        double x1 = xr[P.perm[i]], x2 = xr[P.perm[i + 1]];
-        xr[P.perm[i]] = d[2 * i] * x1 + d[2 * i + 1] * x2;
-        xr[P.perm[i + 1]] = d[2 * i + 1] * x1 + d[2 * i + 3] * x2;
+        double x1 = xr[Pperm[i]], x2 = xr[Pperm[i + 1]];
+        xr[Pperm[i]] = d[2 * i] * x1 + d[2 * i + 1] * x2;
+        xr[Pperm[i + 1]] = d[2 * i + 1] * x1 + d[2 * i + 3] * x2;
       } else if (!second2) {
-        xr[P.perm[i]] *= d[2 * i];
+        xr[Pperm[i]] *= d[2 * i];
       }
     }
   }
 }
 
-// partial[wg][r][j] = sum over this workgroup's rows i >= ne of L[i,j] x[var(i), r]
-__global__ __launch_bounds__(kThreads) void k_bwd_gemv(const SolveNode *sn, const int *slots,
+// ---- backward -------------------------------------------------------------
+// 256-row workgroups over [ne, m) of every front of the level (a row per
+// thread):  part[wg][r][j] = sum over the workgroup's rows i of
+// L[i, j] x[var(i), r].  Columns go 16 at a time: the wave's 64 row products
+// are summed by a reduce-scatter, then the four waves' sums in wave order.
+template <int RC>
+__global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, const int *slots,
                                                        const int *wg_off, int nlvl,
                                                        const double *x, int ldx, int nrhs) {
-  __shared__ double red[kThreads / 64][kBlk];
+  __shared__ double red[kGroups][RC][kBlk];
   const int li = find_node_s(wg_off, nlvl, blockIdx.x);
   const SolveNode &P = sn[slots[li]];
-  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gcint *Prows = (gcint *)P.rows;
+  gdouble *part = (gdouble *)P.P;  // count x nrhs x ne
+  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
+  const int local = blockIdx.x - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = P.m, n = P.n, ne = P.nelim;
-  double *part = P.P;  // count x nrhs x ne partial sums
-  for (int r = 0; r < nrhs; ++r) {
-    const double *xr = x + (size_t)r * ldx;
+  const int r0 = ne + local * kThreads, r1 = imin(m, r0 + kThreads);
+  const int i = r0 + tid, row = imin(i, r1 - 1);
+  const int var = (i < r1) ? ((i < n) ? Pperm[i] : Prows[i - nd]) : -1;
+  for (int rc = 0; rc < nrhs; rc += RC) {
+    const int nr = imin(RC, nrhs - rc);
+    double xv[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) xv[q] = (q < nr && var >= 0) ? x[(size_t)(rc + q) * ldx + var] : 0.0;
     for (int j0 = 0; j0 < ne; j0 += kBlk) {
-      const int wb = min(kBlk, ne - j0);
-      double acc[kBlk];
+      for (int u0 = 0; u0 < kBlk && j0 + u0 < ne; u0 += 16) {
+        double lv[16];
 #pragma unroll
-      for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
-      for (int i = ne + local * kThreads + tid; i < m; i += count * kThreads) {
-        double xi = (i < n) ? xr[P.perm[i]] : xr[P.rows[i - P.nd]];
-        const double *lr = P.L + (size_t)j0 * P.ldl + i;
+        for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(j0 + u0 + u, ne - 1) * ldl + row];
 #pragma unroll
-        for (int t = 0; t < kBlk; ++t)
-          if (t < wb) acc[t] += lr[(size_t)t * P.ldl] * xi;
-      }
+        for (int q = 0; q < RC; ++q) {
+          double a[16];
 #pragma unroll
-      for (int t = 0; t < kBlk; ++t) {
-        double v = acc[t];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) red[wave][t] = v;
+          for (int u = 0; u < 16; ++u) a[u] = lv[u] * xv[q];
+          const double s = wave_reduce_scatter<16>(a, lane);
+          if ((lane & 3) == 0) red[wave][q][u0 + (lane >> 2)] = s;
+        }
       }
       __syncthreads();
-      if (tid < wb) {
-        double v = 0.0;
-        for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
-        part[((size_t)local * nrhs + r) * ne + j0 + tid] = v;
+      for (int idx = tid; idx < RC * kBlk; idx += kThreads) {
+        const int q = idx / kBlk, c = idx % kBlk;
+        if (q < nr && j0 + c < ne)
+          part[((size_t)local * nrhs + rc + q) * ne + j0 + c] =
+              ((red[0][q][c] + red[1][q][c]) + red[2][q][c]) + red[3][q][c];
       }
       __syncthreads();
     }
   }
 }
 
+// One workgroup per front of the level: X[j] = x[perm[j]] - sum of the row
+// workgroups' partials (wave w sums workgroups w, w+4, ...; the waves' sums
+// are subtracted in wave order), then for ne <= kBigTri the triangle
+// L11^T X = X, 64-column blocks from the last, and x[perm[j]] = X[j].
 __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const int *slots,
                                                       const int *wg_off, double *x, int ldx,
                                                       int nrhs) {
+  __shared__ double xt[kBigTri];
   __shared__ double lb[kBlk][kBlk + 1];
-  __shared__ double part[kThreads / 64][kBlk];
-  __shared__ double xb[kBlk];
+  __shared__ double red[kGroups][kBlk];
+  __shared__ double pr[kGroups][kYChunk];
   const int li = blockIdx.x;
   const SolveNode &P = sn[slots[li]];
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gdouble *PX = (gdouble *)P.X;
+  gcdouble *part = (gcdouble *)P.P;
+  const int ldl = P.ldl, m = P.m, ne = P.nelim;
   const int count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m = P.m, ne = P.nelim;
-  const double *partial = P.P;
   for (int r = 0; r < nrhs; ++r) {
-    double *X = P.X + (size_t)r * m;
+    gdouble *X = PX + (size_t)r * m;
     double *xr = x + (size_t)r * ldx;
-    // X[j] = x[perm[j]] - sum of the gemv partials (rows >= ne)
-    for (int j = tid; j < ne; j += kThreads) {
-      double v = xr[P.perm[j]];
-#pragma unroll 8
-      for (int g = 0; g < count; ++g) v -= partial[((size_t)g * nrhs + r) * ne + j];
-      X[j] = v;
-    }
-    __syncthreads();
-    if (ne > kBigTri) continue;  // triangle solved by the big-front launches
-    // L11^T solve, last 64-column block first
-    const int nb = (ne + kBlk - 1) / kBlk;
-    for (int b = nb - 1; b >= 0; --b) {
-      const int j0 = b * kBlk, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-      // X[j] -= L[j1:ne, j]^T X[j1:ne] for j in the block
-      double acc[kBlk];
+    double *acc = (ne <= kBigTri) ? xt : (double *)X;
+    for (int jc = 0; jc < ne; jc += kYChunk) {
+      double s[kYChunk / 64];
 #pragma unroll
-      for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
-      for (int i = j1 + tid; i < ne; i += kThreads) {
-        double xi = X[i];
-        const double *lr = P.L + (size_t)j0 * P.ldl + i;
+      for (int k = 0; k < kYChunk / 64; ++k) s[k] = 0.0;
+#pragma unroll 4
+      for (int w = wave; w < count; w += kGroups) {
+        gcdouble *pw = part + ((size_t)w * nrhs + r) * ne;
 #pragma unroll
-        for (int t = 0; t < kBlk; ++t)
-          if (t < wb) acc[t] += lr[(size_t)t * P.ldl] * xi;
-      }
-#pragma unroll
-      for (int t = 0; t < kBlk; ++t) {
-        double v = acc[t];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) part[wave][t] = v;
-      }
-      for (int idx = tid; idx < wb * wb; idx += kThreads) {
-        int rr = idx % wb, cc = idx / wb;
-        lb[rr][cc] = (rr > cc) ? P.L[(size_t)(j0 + cc) * P.ldl + j0 + rr] : 0.0;
-      }
-      __syncthreads();
-      if (tid < wb) {
-        double v = X[j0 + tid];
-        for (int q = 0; q < kThreads / 64; ++q) v -= part[q][tid];
-        xb[tid] = v;
-      }
-      __syncthreads();
-      if (tid < 64) {
-        double v = (tid < wb) ? xb[tid] : 0.0;
-        for (int j = wb - 1; j >= 0; --j) {
-          double yj = __shfl(v, j);
-          if (tid < j) v -= lb[j][tid] * yj;
+        for (int k = 0; k < kYChunk / 64; ++k) {
+          const int j = jc + lane + 64 * k;
+          if (j < ne) s[k] += pw[j];
         }
-        if (tid < wb) X[j0 + tid] = v;
+      }
+#pragma unroll
+      for (int k = 0; k < kYChunk / 64; ++k) pr[wave][lane + 64 * k] = s[k];
+      __syncthreads();
+      const int j = jc + tid;
+      if (j < ne)
+        acc[j] = xr[Pperm[j]] - (((pr[0][tid] + pr[1][tid]) + pr[2][tid]) + pr[3][tid]);
+      __syncthreads();
+    }
+    if (ne > kBigTri) continue;  // the triangle is solved by the big-front launches
+    for (int b = idiv_up(ne, kBlk) - 1; b >= 0; --b) {
+      const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0, rows = ne - j1;
+      stage_lower(PL, ldl, j0, wb, lb);
+      // X[j] -= sum over the triangle's rows i in [j1, ne) of L[i, j] X[i]
+      double s = 0.0;
+      if (wave * 64 < rows) {
+        const int i = j1 + imin(tid, rows - 1);
+        const double xi = (tid < rows) ? xt[i] : 0.0;
+        double a[kBlk];
+#pragma unroll
+        for (int u = 0; u < kBlk; ++u)
+          a[u] = PL[(size_t)(j0 + imin(u, wb - 1)) * ldl + i] * xi;
+        s = wave_reduce_scatter<kBlk>(a, lane);
+      }
+      red[wave][lane] = s;
+      __syncthreads();
+      if (wave == 0) {
+        double v = 0.0;
+        if (lane < wb)
+          v = xt[j0 + lane] - (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
+        v = bwd_diag_wave(lb, v, lane);
+        if (lane < wb) xt[j0 + lane] = v;
       }
       __syncthreads();
     }
-    for (int i = tid; i < ne; i += kThreads) xr[P.perm[i]] = X[i];
+    for (int j = tid; j < ne; j += kThreads) xr[Pperm[j]] = xt[j];
     __syncthreads();
   }
 }
 
-// ---- big fronts: forward -------------------------------------------------
-// X[j0:j1] <- L_bb^{-1} X[j0:j1] for every right-hand side (one workgroup,
-// one wave per right-hand side at a time)
-__global__ __launch_bounds__(kThreads) void k_tri_fdiag(const SolveNode *sn, int s, int j0,
-                                                        int nrhs) {
-  __shared__ double lb[kBlk][kBlk + 1];
-  const SolveNode &P = sn[s];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-  for (int idx = tid; idx < wb * wb; idx += kThreads) {
-    const int r = idx % wb, c = idx / wb;
-    lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
-  }
-  __syncthreads();
-  for (int r = wave; r < nrhs; r += kThreads / 64) {
-    double *X = P.X + (size_t)r * m;
-    double v = (lane < wb) ? X[j0 + lane] : 0.0;
-    for (int j = 0; j < wb; ++j) {
-      const double yj = __shfl(v, j);
-      if (lane > j && lane < wb) v -= lb[lane][j] * yj;
-    }
-    if (lane < wb) X[j0 + lane] = v;
-  }
-}
-
-// X[i] -= L[i, j0:j1] X[j0:j1] for the triangle's rows i in [j1, ne)
-__global__ __launch_bounds__(kThreads) void k_tri_frows(const SolveNode *sn, int s, int j0,
-                                                        int nrhs) {
+// ---- big fronts -------------------------------------------------------------
+// Forward, block [j0, j1): every workgroup solves the diagonal block from
+// X[j0:j1] (as left by the previous block's launch); workgroup q then updates
+// the triangle's rows [j1 + 64q, ...) with the solution, and workgroup 0 keeps
+// the solution in Y (the front's partial-sum area, nrhs x ne).
+__global__ __launch_bounds__(kThreads) void k_big_fwd(const SolveNode *sn, int s, int j0,
+                                                      int nrhs) {
   __shared__ double ys[kRhsChunk][kBlk];
+  __shared__ double red[kGroups][kRhsChunk][kRows];
   const SolveNode &P = sn[s];
-  const int tid = threadIdx.x, ne = P.nelim, m = P.m;
-  const int j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-  const int i = j1 + blockIdx.x * kThreads + tid;
-  for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
-    const int nr = min(kRhsChunk, nrhs - r0);
-    __syncthreads();
-    for (int idx = tid; idx < kRhsChunk * kBlk; idx += kThreads) {
-      const int q = idx / kBlk, j = idx % kBlk;
-      ys[q][j] = (q < nr && j < wb) ? P.X[(size_t)(r0 + q) * m + j0 + j] : 0.0;
+  gcdouble *PL = (gcdouble *)P.L;
+  gdouble *PX = (gdouble *)P.X;
+  gdouble *Y = (gdouble *)P.P;
+  const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+  const int tid = threadIdx.x, lane = tid & 63, rr = lane, g = tid >> 6, wave = g;
+  const int r0 = j1 + blockIdx.x * kRows, r1 = imin(ne, r0 + kRows);
+  const int row = (r0 < r1) ? imin(r0 + rr, r1 - 1) : 0;
+  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
+    const int nr = imin(kRhsChunk, nrhs - rc);
+    for (int q = wave; q < kRhsChunk; q += kGroups) {
+      double v = 0.0;
+      if (q < nr) {
+        v = fwd_diag_wave(PL, ldl, j0, wb,
+                          (lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] : 0.0, lane);
+        if (blockIdx.x == 0 && lane < wb) Y[(size_t)(rc + q) * ne + j0 + lane] = v;
+      }
+      ys[q][lane] = v;
     }
     __syncthreads();
-    if (i < ne) {
+    if (r0 < r1) {
       double acc[kRhsChunk];
 #pragma unroll
       for (int q = 0; q < kRhsChunk; ++q) acc[q] = 0.0;
-      const double *lc = P.L + (size_t)j0 * P.ldl + i;
-      for (int j = 0; j < wb; ++j) {
-        const double l = lc[(size_t)j * P.ldl];
+      double lv[16];
 #pragma unroll
-        for (int q = 0; q < kRhsChunk; ++q) acc[q] += l * ys[q][j];
+      for (int u = 0; u < 16; ++u)
+        lv[u] = PL[(size_t)(j0 + imin(g + kGroups * u, wb - 1)) * ldl + row];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int q = 0; q < kRhsChunk; ++q) acc[q] += lv[u] * ys[q][g + kGroups * u];
+#pragma unroll
+      for (int q = 0; q < kRhsChunk; ++q) red[g][q][rr] = acc[q];
+    }
+    __syncthreads();
+    if (r0 < r1)
+      for (int idx = tid; idx < kRhsChunk * kRows; idx += kThreads) {
+        const int q = idx / kRows, i = idx % kRows;
+        if (q < nr && r0 + i < r1)
+          PX[(size_t)(rc + q) * m + r0 + i] -=
+              ((red[0][q][i] + red[1][q][i]) + red[2][q][i]) + red[3][q][i];
       }
-      for (int q = 0; q < nr; ++q) P.X[(size_t)(r0 + q) * m + i] -= acc[q];
-    }
+    __syncthreads();
   }
 }
 
-// ---- big fronts: backward ------------------------------------------------
-// part[g][r][t] = sum over workgroup g's rows i in [j1, ne) of L[i, j0+t] X[i, r]
-__global__ __launch_bounds__(kThreads) void k_tri_brows(const SolveNode *sn, int s, int j0,
-                                                        int nrhs, double *part) {
-  __shared__ double red[kThreads / 64][kBlk];
+// X[0:ne] = Y and x[perm[i]] = Y[i] after the last forward block
+__global__ void k_big_fwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
   const SolveNode &P = sn[s];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-  const int i = j1 + blockIdx.x * kThreads + tid;
+  gcint *Pperm = (gcint *)P.perm;
+  gdouble *PX = (gdouble *)P.X;
+  gcdouble *Y = (gcdouble *)P.P;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim, m = P.m;
+  if (i >= ne) return;
+  const int v = Pperm[i];
   for (int r = 0; r < nrhs; ++r) {
-    double acc[kBlk];
-#pragma unroll
-    for (int t = 0; t < kBlk; ++t) acc[t] = 0.0;
-    if (i < ne) {
-      const double xi = P.X[(size_t)r * m + i];
-      const double *lr = P.L + (size_t)j0 * P.ldl + i;
-#pragma unroll
-      for (int t = 0; t < kBlk; ++t)
-        if (t < wb) acc[t] = lr[(size_t)t * P.ldl] * xi;
-    }
-#pragma unroll
-    for (int t = 0; t < kBlk; ++t) {
-      double v = acc[t];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) red[wave][t] = v;
-    }
-    __syncthreads();
-    if (tid < kBlk) {
-      double v = 0.0;
-      for (int q = 0; q < kThreads / 64; ++q) v += red[q][tid];
-      part[((size_t)blockIdx.x * nrhs + r) * kBlk + tid] = v;
-    }
-    __syncthreads();
+    const double y = Y[(size_t)r * ne + i];
+    PX[(size_t)r * m + i] = y;
+    x[(size_t)r * ldx + v] = y;
   }
 }
 
-// X[j0:j1] -= sum of the partials, then X[j0:j1] <- L_bb^{-T} X[j0:j1]
-__global__ __launch_bounds__(kThreads) void k_tri_bdiag(const SolveNode *sn, int s, int j0,
-                                                        int ngroups, int nrhs,
-                                                        const double *part) {
+// Backward, block [j0, j1) (blocks from the last): every workgroup solves the
+// diagonal block with L_bb^T from X[j0:j1] (complete: every later block has
+// already pushed its contribution); workgroup q then pushes the solution into
+// the columns [64q, 64q + 64) of X[0:j0]:  X[c] -= L[j0:j1, c]^T x_b.
+// Workgroup 0 keeps the solution in Y (the front's partial-sum area).
+__global__ __launch_bounds__(kThreads) void k_big_bwd(const SolveNode *sn, int s, int j0,
+                                                      int nrhs) {
   __shared__ double lb[kBlk][kBlk + 1];
+  __shared__ double xb[kRhsChunk][kBlk];
   const SolveNode &P = sn[s];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ne = P.nelim, m = P.m, j1 = min(j0 + kBlk, ne), wb = j1 - j0;
-  for (int idx = tid; idx < wb * wb; idx += kThreads) {
-    const int r = idx % wb, c = idx / wb;
-    lb[r][c] = (r > c) ? P.L[(size_t)(j0 + c) * P.ldl + j0 + r] : 0.0;
-  }
+  gcdouble *PL = (gcdouble *)P.L;
+  gdouble *PX = (gdouble *)P.X;
+  gdouble *Y = (gdouble *)P.P;
+  const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6, wave = g;
+  const int c0 = blockIdx.x * kBlk, c1 = imin(j0, c0 + kBlk);  // columns of X to update
+  stage_lower(PL, ldl, j0, wb, lb);
   __syncthreads();
-  for (int r = wave; r < nrhs; r += kThreads / 64) {
-    double *X = P.X + (size_t)r * m;
-    double v = 0.0;
-    if (lane < wb) {
-      v = X[j0 + lane];
-      for (int g = 0; g < ngroups; ++g) v -= part[((size_t)g * nrhs + r) * kBlk + lane];
+  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
+    const int nr = imin(kRhsChunk, nrhs - rc);
+    for (int q = wave; q < kRhsChunk; q += kGroups) {
+      double v = 0.0;
+      if (q < nr) {
+        v = bwd_diag_wave(lb, (lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] : 0.0, lane);
+        if (lane >= wb) v = 0.0;
+        if (blockIdx.x == 0 && lane < wb) Y[(size_t)(rc + q) * ne + j0 + lane] = v;
+      }
+      xb[q][lane] = v;
     }
-    for (int j = wb - 1; j >= 0; --j) {
-      const double yj = __shfl(v, j);
-      if (lane < j) v -= lb[j][lane] * yj;
+    __syncthreads();
+    if (c0 < c1) {
+      // lane = row t of the block, wave g = columns c0 + g + 4u
+      double lv[16];
+      const int t = j0 + imin(lane, wb - 1);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(c0 + g + kGroups * u, c1 - 1) * ldl + t];
+      const int c = c0 + g + kGroups * (lane >> 2);
+      for (int q = 0; q < nr; ++q) {
+        double a[16];
+        const double xt = xb[q][lane];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] = lv[u] * xt;
+        const double sum = wave_reduce_scatter<16>(a, lane);
+        if ((lane & 3) == 0 && c < c1) PX[(size_t)(rc + q) * m + c] -= sum;
+      }
     }
-    if (lane < wb) X[j0 + lane] = v;
+    __syncthreads();
   }
 }
 
-// x[perm[i]] = X[i] for the front's eliminated variables
-__global__ void k_tri_scatter(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
+// x[perm[i]] = Y[i] after the first block
+__global__ void k_big_bwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
   const SolveNode &P = sn[s];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.nelim) return;
-  for (int r = 0; r < nrhs; ++r) x[(size_t)r * ldx + P.perm[i]] = P.X[(size_t)r * P.m + i];
+  gcint *Pperm = (gcint *)P.perm;
+  gcdouble *Y = (gcdouble *)P.P;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim;
+  if (i >= ne) return;
+  const int v = Pperm[i];
+  for (int r = 0; r < nrhs; ++r) x[(size_t)r * ldx + v] = Y[(size_t)r * ne + i];
 }
+
+// ---- launchers ----------------------------------------------------------------
+int big_tri_threshold() { return kBigTri; }
+int fwd_rows_per_wg() { return kRows; }
+int bwd_rows_per_wg() { return kThreads; }
 
 void launch_big_fwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
                     int nrhs) {
   for (int j0 = 0; j0 < ne; j0 += kBlk) {
-    hipLaunchKernelGGL(k_tri_fdiag, dim3(1), dim3(kThreads), 0, st, sn, s, j0, nrhs);
     const int rows = ne - std::min(j0 + kBlk, ne);
-    if (rows > 0)
-      hipLaunchKernelGGL(k_tri_frows, dim3((rows + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                         st, sn, s, j0, nrhs);
+    hipLaunchKernelGGL(k_big_fwd, dim3(std::max(1, (rows + kRows - 1) / kRows)),
+                       dim3(kThreads), 0, st, sn, s, j0, nrhs);
   }
-  hipLaunchKernelGGL(k_tri_scatter, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
+  hipLaunchKernelGGL(k_big_fwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
                      nrhs);
 }
-// part: room for (ne / kThreads + 1) * nrhs * kBlk doubles
 void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
-                    int nrhs, double *part) {
-  const int nb = (ne + kBlk - 1) / kBlk;
-  for (int b = nb - 1; b >= 0; --b) {
-    const int j0 = b * kBlk, rows = ne - std::min(j0 + kBlk, ne);
-    const int ng = (rows + kThreads - 1) / kThreads;
-    if (ng > 0)
-      hipLaunchKernelGGL(k_tri_brows, dim3(ng), dim3(kThreads), 0, st, sn, s, j0, nrhs, part);
-    hipLaunchKernelGGL(k_tri_bdiag, dim3(1), dim3(kThreads), 0, st, sn, s, j0, ng, nrhs,
-                       (const double *)part);
-  }
-  hipLaunchKernelGGL(k_tri_scatter, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
+                    int nrhs) {
+  for (int j0 = (ne - 1) / kBlk * kBlk; j0 >= 0; j0 -= kBlk)
+    hipLaunchKernelGGL(k_big_bwd, dim3(std::max(1, (j0 + kBlk - 1) / kBlk)), dim3(kThreads), 0,
+                       st, sn, s, j0, nrhs);
+  hipLaunchKernelGGL(k_big_bwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
                      nrhs);
 }
-int big_tri_threshold() { return kBigTri; }
 
 void launch_ipos(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int *scratch) {
   if (nlvl > 0)
     hipLaunchKernelGGL(k_ipos, dim3(nlvl), dim3(256), 0, st, nodes, slots, scratch);
 }
-void launch_fwd_gather(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                       const int *child_list, double *x, int ldx, int nrhs, const int *wg_off,
-                       int nwg) {
-  if (nlvl > 0 && nwg > 0)
-    hipLaunchKernelGGL(k_fwd_gather, dim3(nwg), dim3(kThreads), 0, st, sn, slots, child_list,
-                       x, ldx, nrhs, wg_off, nlvl);
-}
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                    double *x, int ldx, int nrhs) {
-  if (nlvl <= 0) return;
-  hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, x, ldx, nrhs);
+                    const int *child_list, double *x, int ldx, int nrhs) {
+  if (nlvl > 0)
+    hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
+                       ldx, nrhs);
 }
-void launch_fwd_gemv(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                     int nrhs, const int *gemv_off, int gemv_wg) {
-  if (nlvl > 0 && gemv_wg > 0)
-    hipLaunchKernelGGL(k_fwd_gemv, dim3(gemv_wg), dim3(kThreads), 0, st, sn, slots, gemv_off,
-                       nlvl, nrhs);
+void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
+                     const int *child_list, int nrhs, const int *wg_off, int nwg) {
+  if (nlvl <= 0 || nwg <= 0) return;
+  if (nrhs == 1)
+    hipLaunchKernelGGL(k_fwd_rows<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots, child_list,
+                       wg_off, nlvl, nrhs);
+  else
+    hipLaunchKernelGGL(k_fwd_rows<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
+                       child_list, wg_off, nlvl, nrhs);
 }
 void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, int ldx,
                    int nrhs) {
   if (nslots > 0)
     hipLaunchKernelGGL(k_dsolve, dim3(nslots), dim3(64), 0, st, sn, nslots, x, ldx, nrhs);
 }
+// rows_off / nrows: the level's 256-row workgroup table and its total
 void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots, double *x,
-                int ldx, int nrhs, const int *gemv_off, int gemv_wg) {
+                int ldx, int nrhs, const int *rows_off, int nrows) {
   if (nlvl <= 0) return;
-  if (gemv_wg > 0)
-    hipLaunchKernelGGL(k_bwd_gemv, dim3(gemv_wg), dim3(kThreads), 0, st, sn, slots, gemv_off,
-                       nlvl, x, ldx, nrhs);
-  hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, gemv_off, x, ldx,
+  if (nrows > 0) {
+    if (nrhs == 1)
+      hipLaunchKernelGGL(k_bwd_rows<1>, dim3(nrows), dim3(kThreads), 0, st, sn, slots, rows_off,
+                         nlvl, x, ldx, nrhs);
+    else
+      hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nrows), dim3(kThreads), 0, st, sn, slots,
+                         rows_off, nlvl, x, ldx, nrhs);
+  }
+  hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x, ldx,
                      nrhs);
 }
 
