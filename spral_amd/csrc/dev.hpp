@@ -135,6 +135,14 @@ __device__ inline int table_find(const int *off, int n, int b) {
   return lo;
 }
 
+// v if keep, else +0.0.  The value passes through an empty asm first, so the
+// selection is not turned into a branch around the load that produced it
+// (which would serialize a batch of loads behind their uses).
+__device__ inline double masked(double v, bool keep) {
+  asm("" : "+v"(v));
+  return keep ? v : 0.0;
+}
+
 // value held by lane 0 (wave-uniform by construction), as a scalar
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ inline double uni(double v) {

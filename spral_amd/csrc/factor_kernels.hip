@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   __shared__ double t[kBlk][kBlk + 1];  // load staging, then L by step: t[k][row]
   __shared__ double cs[2][kBlk];        // broadcast pivot columns
   __shared__ unsigned long long ckey[2][NW];
-  __shared__ int ccol[2][NW], crow[2][NW];
+  __shared__ int ccr[2][NW];  // candidate column << 7 | row
   __shared__ double dinv[2 * kBlk + 2];
   __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
   __shared__ double mi[kBlk][kBlk + 1];  // inverse of L_bb (elimination order)
@@ -109,15 +109,29 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   if (w == 0) return;
   const int ldl = N.ldl;
   double *L = N.L;
-  for (int idx = tid; idx < kBlk * kBlk; idx += 64 * NW) {
-    const int r = idx % kBlk, c = idx / kBlk;
-    if (r >= c) {
-      const double v = (r < w) ? L[(size_t)(p + c) * ldl + p + r] : 0.0;
-      t[r][c] = v;
-      t[c][r] = v;
-      if (r < w) {
-        N.dbak[r + c * kBlk] = v;
-        N.dbak[c + r * kBlk] = v;
+  {
+    // the w x w block (lower triangle mirrored), zero outside; every thread's
+    // loads are unconditional (indices clamped into the block) so they are
+    // all in flight together
+    gcdouble *Lg = (gcdouble *)L;
+    gdouble *dbak = (gdouble *)N.dbak;
+    double v[kBlk * kBlk / (64 * NW)];
+#pragma unroll
+    for (int it = 0; it < kBlk * kBlk / (64 * NW); ++it) {
+      const int idx = tid + it * 64 * NW, r = idx % kBlk, c = idx / kBlk;
+      v[it] = masked(Lg[(size_t)(p + imin(c, w - 1)) * ldl + p + imin(r, w - 1)],
+                     r >= c && r < w);
+    }
+#pragma unroll
+    for (int it = 0; it < kBlk * kBlk / (64 * NW); ++it) {
+      const int idx = tid + it * 64 * NW, r = idx % kBlk, c = idx / kBlk;
+      if (r >= c) {
+        t[r][c] = v[it];
+        t[c][r] = v[it];
+        if (r < w) {
+          dbak[r + c * kBlk] = v[it];
+          dbak[c + r * kBlk] = v[it];
+        }
       }
     }
   }
@@ -135,12 +149,11 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
     double bv = 0.0;
     int br = 64;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if ((elim >> (r0 + r)) & 1ull) continue;
-      if (fabs(a[r]) > bv) {  // NaN never wins; all-zero keeps br = 64
-        bv = fabs(a[r]);
-        br = r0 + r;
-      }
+    for (int r = 0; r < R; ++r) {  // NaN never wins; all-zero keeps br = 64
+      const double av = fabs(a[r]);
+      const bool take = !((elim >> (r0 + r)) & 1ull) && av > bv;
+      bv = take ? av : bv;
+      br = take ? r0 + r : br;
     }
     // ---- wave candidate: value, then smallest column, then its row
     const unsigned long long key = live ? (unsigned long long)__double_as_longlong(bv) : 0ull;
@@ -150,26 +163,27 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
     const int rw = hit ? __builtin_amdgcn_readlane(br, cw) : 64;
     if (lane == 0) {
       ckey[par][wv] = kw;
-      ccol[par][wv] = cw;
-      crow[par][wv] = rw;
+      ccr[par][wv] = (cw << 7) | rw;
     }
     __syncthreads();
-    unsigned long long kmax = 0ull;
-    int bc = 64, brow = 64;
+    // largest value; among the waves holding it, smallest column, then row
+    unsigned long long kv[NW];
+    int cr[NW];
 #pragma unroll
     for (int v = 0; v < NW; ++v) {
-      const unsigned long long kv = ckey[par][v];
-      const int cv = ccol[par][v], rv = crow[par][v];
-      if (kv > kmax || (kv == kmax && (cv < bc || (cv == bc && rv < brow)))) {
-        kmax = kv;
-        bc = cv;
-        brow = rv;
-      }
+      kv[v] = ckey[par][v];
+      cr[v] = ccr[par][v];
     }
+    unsigned long long kmax = kv[0];
+#pragma unroll
+    for (int v = 1; v < NW; ++v) kmax = (kv[v] > kmax) ? kv[v] : kmax;
+    int best = 1 << 30;  // above every encoding; kmax is some wave's key
+#pragma unroll
+    for (int v = 0; v < NW; ++v) best = (kv[v] == kmax && cr[v] < best) ? cr[v] : best;
     kmax = ((unsigned long long)(unsigned)uni((int)(kmax >> 32)) << 32) |
            (unsigned)uni((int)(kmax & 0xffffffffull));
-    bc = uni(bc);
-    brow = uni(brow);
+    best = uni(best);
+    int bc = best >> 7, brow = best & 127;
     par ^= 1;
     const double vmax = __longlong_as_double((long long)kmax);
     if (!(vmax >= fp.small) || bc >= 64) {

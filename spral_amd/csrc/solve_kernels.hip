@@ -46,14 +46,6 @@ __device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
   return table_find(wg_off, nlvl, b);
 }
 
-// v if keep, else +0.0.  The value passes through an empty asm first, so the
-// selection is not turned into a branch around the load that produced it
-// (which would serialize a batch of loads behind their uses).
-__device__ inline double masked(double v, bool keep) {
-  asm("" : "+v"(v));
-  return keep ? v : 0.0;
-}
-
 // value of v in lane j (j uniform)
 __device__ inline double bcast_lane(double v, int j) {
   const long long b = __double_as_longlong(v);
