@@ -125,7 +125,10 @@ void spral_ssids_hip_destroy_symbolic(void *symbolic);
  * (src/ssids/gpu/subtree.f90:304-460).  aval is the host array of lower
  * triangle values (as passed to ssids_factor), scaling may be NULL.
  * Returns the numeric handle (factors resident in HBM) or NULL with
- * stats->flag < 0.
+ * stats->flag < 0.  posdef = true factors A = L L^T as factor_node_posdef
+ * (src/ssids/cpu/factor.hxx:139-171): pivots in order, all 1x1; a pivot
+ * that is not positive fails with stats->flag = -6 (SSIDS_ERROR_NOT_POS_DEF).
+ * The factors are held as L D L^T with L unit lower (L_chol = L D^{1/2}).
  * ------------------------------------------------------------------------- */
 void *spral_ssids_hip_factor(bool posdef, void *symbolic, const double *aval,
                              int64_t naval, const double *scaling,
@@ -150,7 +153,10 @@ void spral_ssids_hip_destroy_numeric(void *numeric);
 /* ---------------------------------------------------------------------------
  * Solves with host x (replace gpu_numeric_subtree%solve_fwd/_diag/_diag_bwd/
  * _bwd, src/ssids/gpu/subtree.f90:555-738).  All RHS are solved together.
- * Return a flag (0 success).
+ * Return a flag (0 success).  For a positive-definite factorization fwd
+ * solves L_chol y = x, bwd and diag_bwd L_chol^T y = x, and diag does
+ * nothing, as the CPU subtree (src/ssids/cpu/NumericSubtree.hxx:342-343);
+ * ssids_solve itself rejects jobs 2 and 4 then (src/ssids/ssids.f90:1237).
  * ------------------------------------------------------------------------- */
 int spral_ssids_hip_solve_fwd(void *numeric, int nrhs, double *x, int ldx);
 int spral_ssids_hip_solve_diag(void *numeric, int nrhs, double *x, int ldx);
@@ -185,6 +191,13 @@ int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d
 int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order,
                                   double *d);
 int spral_ssids_hip_alter(void *numeric, const double *d);
+/* Positive-definite factorization: d[i] = diagonal of the Cholesky factor in
+ * pivot order; replaces gpu_numeric_subtree%enquire_posdef
+ * (src/ssids/gpu/subtree.f90:702-738, values as the CPU subtree's
+ * src/ssids/cpu/NumericSubtree.hxx:424-433).  Returns -13
+ * (SSIDS_ERROR_NOT_LLT) for an indefinite factorization; enquire_indef and
+ * alter return -14 (SSIDS_ERROR_NOT_LDLT) for a positive-definite one. */
+int spral_ssids_hip_enquire_posdef(const void *numeric, double *d);
 
 /* Contribution block of the subtree root for a parent subtree held elsewhere;
  * same out-parameters as spral_ssids_cpu_subtree_get_contrib_dbl

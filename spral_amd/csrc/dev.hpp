@@ -189,6 +189,7 @@ struct FactorParams {
   int action;
   int tpp_failed;   // failed_pivot_method == TPP
   int pivot_tpp;    // pivot_method == TPP (whole panel with TPP)
+  int posdef;       // A = L L^T: pivots in order, every one 1x1 and > 0
 };
 
 }  // namespace ssids_hip

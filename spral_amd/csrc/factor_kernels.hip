@@ -80,7 +80,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
                                                   int step, FactorParams fp) {
   constexpr int R = kBlk / NW;
   __shared__ double t[kBlk][kBlk + 1];  // load staging, then L by step: t[k][row]
-  __shared__ double cs[2][kBlk];        // broadcast pivot columns
+  __shared__ double cs2[2][2][kBlk];    // broadcast pivot columns (by step parity)
   __shared__ unsigned long long ckey[2][NW];
   __shared__ int ccr[2][NW];  // candidate column << 7 | row
   __shared__ double dinv[2 * kBlk + 2];
@@ -143,69 +143,89 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   __syncthreads();
   unsigned long long elim = (w < 64) ? (~0ull << w) : 0ull;  // eliminated / absent
   int k = 0, par = 0;
-  while (k < w) {
-    const bool live = !((elim >> lane) & 1ull);
-    // ---- search this lane's live rows
-    double bv = 0.0;
-    int br = 64;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {  // NaN never wins; all-zero keeps br = 64
-      const double av = fabs(a[r]);
-      const bool take = !((elim >> (r0 + r)) & 1ull) && av > bv;
-      bv = take ? av : bv;
-      br = take ? r0 + r : br;
-    }
-    // ---- wave candidate: value, then smallest column, then its row
-    const unsigned long long key = live ? (unsigned long long)__double_as_longlong(bv) : 0ull;
-    const unsigned long long kw = wave_max_u64(key);
-    const unsigned long long hit = __ballot(live && key == kw);
-    const int cw = hit ? __ffsll((long long)hit) - 1 : 64;
-    const int rw = hit ? __builtin_amdgcn_readlane(br, cw) : 64;
-    if (lane == 0) {
-      ckey[par][wv] = kw;
-      ccr[par][wv] = (cw << 7) | rw;
-    }
-    __syncthreads();
-    // largest value; among the waves holding it, smallest column, then row
-    unsigned long long kv[NW];
-    int cr[NW];
-#pragma unroll
-    for (int v = 0; v < NW; ++v) {
-      kv[v] = ckey[par][v];
-      cr[v] = ccr[par][v];
-    }
-    unsigned long long kmax = kv[0];
-#pragma unroll
-    for (int v = 1; v < NW; ++v) kmax = (kv[v] > kmax) ? kv[v] : kmax;
-    int best = 1 << 30;  // above every encoding; kmax is some wave's key
-#pragma unroll
-    for (int v = 0; v < NW; ++v) best = (kv[v] == kmax && cr[v] < best) ? cr[v] : best;
-    kmax = ((unsigned long long)(unsigned)uni((int)(kmax >> 32)) << 32) |
-           (unsigned)uni((int)(kmax & 0xffffffffull));
-    best = uni(best);
-    int bc = best >> 7, brow = best & 127;
-    par ^= 1;
-    const double vmax = __longlong_as_double((long long)kmax);
-    if (!(vmax >= fp.small) || bc >= 64) {
-      // zero pivots for every remaining index, in increasing index order
-      int kk = k;
-      for (int i = 0; i < w; ++i) {
-        if ((elim >> i) & 1ull) continue;
-        if (wv == 0) t[kk][lane] = 0.0;
-        if (tid == 0) {
-          lp[kk] = i;
-          kind[kk] = 0;
-          dinv[2 * kk] = 0.0;
-          dinv[2 * kk + 1] = 0.0;
-        }
-        ++kk;
-      }
+  // every remaining index becomes a zero pivot, in increasing index order;
+  // code -6: a non-positive pivot of the positive-definite factorization
+  auto zero_rest = [&](int code) {
+    int kk = k;
+    for (int i = 0; i < w; ++i) {
+      if ((elim >> i) & 1ull) continue;
+      if (wv == 0) t[kk][lane] = 0.0;
       if (tid == 0) {
+        lp[kk] = i;
+        kind[kk] = 0;
+        dinv[2 * kk] = 0.0;
+        dinv[2 * kk + 1] = 0.0;
+      }
+      ++kk;
+    }
+    if (tid == 0) {
+      if (code == -6) {
+        N.flag = -6;
+      } else {
         N.nzero += w - k;
         if (!fp.action) N.flag = -5;
       }
-      k = w;
-      break;
+    }
+    k = w;
+  };
+  int cpar = 0;
+  while (k < w) {
+    // parity-buffered: a step with one barrier (positive definite) must not
+    // overwrite the columns a slower wave still reads from the step before
+    double (*cs)[kBlk] = cs2[cpar];
+    cpar ^= 1;
+    int bc, brow;
+    if (fp.posdef) {
+      // Cholesky order: the lowest remaining index, always 1x1
+      bc = brow = __ffsll((long long)~elim) - 1;
+    } else {
+      const bool live = !((elim >> lane) & 1ull);
+      // ---- search this lane's live rows
+      double bv = 0.0;
+      int br = 64;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {  // NaN never wins; all-zero keeps br = 64
+        const double av = fabs(a[r]);
+        const bool take = !((elim >> (r0 + r)) & 1ull) && av > bv;
+        bv = take ? av : bv;
+        br = take ? r0 + r : br;
+      }
+      // ---- wave candidate: value, then smallest column, then its row
+      const unsigned long long key = live ? (unsigned long long)__double_as_longlong(bv) : 0ull;
+      const unsigned long long kw = wave_max_u64(key);
+      const unsigned long long hit = __ballot(live && key == kw);
+      const int cw = hit ? __ffsll((long long)hit) - 1 : 64;
+      const int rw = hit ? __builtin_amdgcn_readlane(br, cw) : 64;
+      if (lane == 0) {
+        ckey[par][wv] = kw;
+        ccr[par][wv] = (cw << 7) | rw;
+      }
+      __syncthreads();
+      // largest value; among the waves holding it, smallest column, then row
+      unsigned long long kv[NW];
+      int cr[NW];
+#pragma unroll
+      for (int v = 0; v < NW; ++v) {
+        kv[v] = ckey[par][v];
+        cr[v] = ccr[par][v];
+      }
+      unsigned long long kmax = kv[0];
+#pragma unroll
+      for (int v = 1; v < NW; ++v) kmax = (kv[v] > kmax) ? kv[v] : kmax;
+      int best = 1 << 30;  // above every encoding; kmax is some wave's key
+#pragma unroll
+      for (int v = 0; v < NW; ++v) best = (kv[v] == kmax && cr[v] < best) ? cr[v] : best;
+      kmax = ((unsigned long long)(unsigned)uni((int)(kmax >> 32)) << 32) |
+             (unsigned)uni((int)(kmax & 0xffffffffull));
+      best = uni(best);
+      bc = best >> 7;
+      brow = best & 127;
+      par ^= 1;
+      const double vmax = __longlong_as_double((long long)kmax);
+      if (!(vmax >= fp.small) || bc >= 64) {
+        zero_rest(-5);
+        break;
+      }
     }
     // ---- owners broadcast their columns: cs[0] <- column bc, cs[1] <- column brow
     if (lane == bc || lane == brow) {
@@ -214,6 +234,10 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
       for (int r = 0; r < R; ++r) dst[r] = a[r];
     }
     __syncthreads();
+    if (fp.posdef && !(uni(cs[0][bc]) > 0.0)) {
+      zero_rest(-6);
+      break;
+    }
     int size, p1, p2 = -1;
     const double *c1p, *c2p = cs[1];
     double d11 = 0.0, d21 = 0.0, d22 = 0.0;
@@ -533,7 +557,7 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
     p = N.p;
     w = N.w;
     if (w == 0) return;
-    int np = block_npass(N, fp.u);
+    int np = fp.posdef ? N.w : block_npass(N, fp.u);
     if (local == 0 && tid == 0) N.npass = np;
     a0 = p;
     K = np;
@@ -1291,7 +1315,8 @@ __global__ void k_stats(const NodeDev *nodes, int nslots, int *counters) {
   atomicAdd(&counters[2], zero);
   atomicAdd(&counters[3], N.n - N.nelim);
   atomicAdd(&counters[4], N.n - N.nelim_app);  // not_first_pass
-  atomicAdd(&counters[5], N.flag < 0 ? 1 : 0);
+  atomicAdd(&counters[5], N.flag == -5 ? 1 : 0);
+  atomicAdd(&counters[6], N.flag == -6 ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,7 +49,7 @@ void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int
 int big_tri_threshold();
 int fwd_rows_per_wg();
 int bwd_rows_per_wg();
-void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int);
+void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
                 const int *, int);
 
@@ -242,7 +242,7 @@ struct Numeric {
   double *d_x = nullptr;
   size_t d_x_cap = 0;
   int solve_nrhs_cap = 0;
-  bool posdef = false;
+  bool posdef = false;                    // A = L L^T (pivots in order, all > 0)
   spral_ssids_hip_stats stats{};
   double *d_aval = nullptr;               // copy of the factorized values
   size_t d_aval_cap = 0;
@@ -827,7 +827,8 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   fp.small = opt.small;
   fp.action = opt.action;
   fp.tpp_failed = (opt.failed_pivot_method != 2);
-  fp.pivot_tpp = (opt.pivot_method == 3);
+  fp.pivot_tpp = (opt.pivot_method == 3) && !N.posdef;
+  fp.posdef = N.posdef;
   if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
   if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
   if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
@@ -878,6 +879,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     o.not_second_pass = cnt[3];
     o.flag = (cnt[2] > 0) ? 7 : 0;  // SSIDS_WARNING_FACT_SINGULAR
     if (cnt[5] > 0) o.flag = -5;
+    if (cnt[6] > 0) o.flag = -6;  // SSIDS_ERROR_NOT_POS_DEF
   }
   int64_t nf = 0, fl = 0;
   int maxsn = 0;
@@ -1038,10 +1040,17 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   Symbolic &S = *N.symb;
   ensure_solve_ws(N, nrhs);
   const int nlev = (int)S.lvl_ptr.size() - 1;
+  // positive definite, as the CPU subtree (src/ssids/cpu/NumericSubtree.hxx:
+  // 342-343): the diagonal solve alone is a no-op, diag + bwd is bwd
+  if (N.posdef && job == 2) return;
+  if (N.posdef && job == 4) job = 3;
   // the slot list of all levels is simply 0..nnodes-1 in level order
   const int *slots = N.d_slots_all;
   bool fwd = (job == 0 || job == 1), bwd = (job == 0 || job == 3 || job == 4);
   bool diag = (job == 0 || job == 2 || job == 4);
+  // positive definite: L_chol = L D^{1/2}, so the forward (job 1) and
+  // backward (job 3) halves of L L^T carry D^{-1/2} on their inner side
+  const bool half_after_fwd = N.posdef && job == 1, half_before_bwd = N.posdef && job == 3;
   if (fwd)
     for (int L = 0; L < nlev; ++L) {
       const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
@@ -1051,7 +1060,9 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
       launch_fwd_rows(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, nrhs,
                       N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
     }
-  if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs);
+  if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 0);
+  if (half_after_fwd || half_before_bwd)
+    launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 1);
   if (bwd)
     for (int L = nlev - 1; L >= 0; --L) {
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
@@ -1205,16 +1216,13 @@ void *spral_ssids_hip_factor_device(bool posdef, void *symbolic, const double *d
   if (options) opt = *options;
   else spral_ssids_hip_default_options(&opt);
   std::memset(stats, 0, sizeof(*stats));
-  if (posdef) {
-    stats->flag = -98;  // LL^T path not provided by this backend
-    return nullptr;
-  }
   if (naval < S->naval_min) {
     stats->flag = -3;
     return nullptr;
   }
   auto *N = new Numeric();
   N->symb = S;
+  N->posdef = posdef;
   try {
     HIPCHK(hipSetDevice(S->device));
     HIPCHK(hipStreamCreate(&N->stream));
@@ -1362,9 +1370,33 @@ int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, doubl
   return rc;
 }
 
+int spral_ssids_hip_enquire_posdef(const void *numeric, double *d) {
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  if (!N->posdef) return -13;  // SSIDS_ERROR_NOT_LLT
+  try {
+    for (int i = 0; i < S.n; ++i) d[i] = 0.0;
+    double *dp = d;
+    // diagonal of L_chol = L D^{1/2}: sqrt(D_ii) = 1 / sqrt(D^{-1}_ii), in
+    // the pivot order of enquire_indef (nodes in postorder)
+    for (int i = 0; i < S.nnodes; ++i) {
+      const NodeDev &D = N->nodes[S.slot[i]];
+      const int ne = D.nelim;
+      if (ne == 0) continue;
+      std::vector<double> dh(2 * (size_t)ne);
+      HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
+      for (int k = 0; k < ne; ++k) *dp++ = 1.0 / std::sqrt(dh[2 * k]);
+    }
+    return 0;
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
 int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d) {
   const Numeric *N = static_cast<const Numeric *>(numeric);
   const Symbolic &S = *N->symb;
+  if (N->posdef) return -14;  // SSIDS_ERROR_NOT_LDLT
   try {
     if (piv_order)
       for (int i = 0; i < S.n; ++i) piv_order[i] = 0;
@@ -1415,6 +1447,7 @@ int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d
 int spral_ssids_hip_alter(void *numeric, const double *d) {
   Numeric *N = static_cast<Numeric *>(numeric);
   const Symbolic &S = *N->symb;
+  if (N->posdef) return -14;  // SSIDS_ERROR_NOT_LDLT
   try {
     const double *dp = d;
     for (int i = 0; i < S.nnodes; ++i) {

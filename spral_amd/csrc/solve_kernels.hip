@@ -345,7 +345,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, cons
   }
 }
 
-__global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, int nrhs) {
+// x = D^{-1} x over every front; half = 1 (positive definite, 1x1 pivots
+// only): x = D^{-1/2} x
+__global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, int nrhs,
+                         int half) {
   const int s = blockIdx.x;
   if (s >= nslots) return;
   const SolveNode &P = sn[s];
@@ -357,7 +360,9 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
     bool second2 = !isfinite(d[2 * i]);
     for (int r = 0; r < nrhs; ++r) {
       double *xr = x + (size_t)r * ldx;
-      if (first2) {
+      if (half) {
+        xr[Pperm[i]] *= sqrt(d[2 * i]);
+      } else if (first2) {
         double x1 = xr[Pperm[i]], x2 = xr[Pperm[i + 1]];
         xr[Pperm[i]] = d[2 * i] * x1 + d[2 * i + 1] * x2;
         xr[Pperm[i + 1]] = d[2 * i + 1] * x1 + d[2 * i + 3] * x2;
@@ -675,9 +680,10 @@ void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *s
                        child_list, wg_off, nlvl, nrhs);
 }
 void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, int ldx,
-                   int nrhs) {
+                   int nrhs, int half) {
   if (nslots > 0)
-    hipLaunchKernelGGL(k_dsolve, dim3(nslots), dim3(64), 0, st, sn, nslots, x, ldx, nrhs);
+    hipLaunchKernelGGL(k_dsolve, dim3(nslots), dim3(64), 0, st, sn, nslots, x, ldx, nrhs,
+                       half);
 }
 // rows_off / nrows: the level's 256-row workgroup table and its total
 void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots, double *x,

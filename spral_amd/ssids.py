@@ -31,6 +31,7 @@ SSIDS_ERROR_NOT_POS_DEF = -6
 SSIDS_ERROR_ORDER = -8
 SSIDS_ERROR_X_SIZE = -10
 SSIDS_ERROR_JOB_OOR = -11
+SSIDS_ERROR_NOT_LLT = -13
 SSIDS_ERROR_NOT_LDLT = -14
 SSIDS_ERROR_ALLOCATION = -50
 SSIDS_ERROR_CUDA_UNKNOWN = -51
@@ -265,6 +266,9 @@ def ssids_solve(x, akeep, fkeep, options=None, job=None):
     if job < 0 or job > 4:
         inf.flag = SSIDS_ERROR_JOB_OOR
         return inf
+    if fkeep.pos_def and job in (SSIDS_SOLVE_JOB_DIAG, SSIDS_SOLVE_JOB_DIAG_BWD):
+        inf.flag = SSIDS_ERROR_JOB_OOR  # src/ssids/ssids.f90:1237-1240
+        return inf
     xa = np.asarray(x)
     one = xa.ndim == 1
     X = xa.reshape(akeep.n, -1)
@@ -329,6 +333,8 @@ def ssids_solve_refine(x, akeep, fkeep, options=None, maxit=5, tol=1e-14):
 
 
 def _solve_all(L, num, nrhs, x2c, n):
+    # src/ssids/fkeep.F90:269-297: fwd then diag_bwd (for L L^T the subtree's
+    # diag_bwd is its bwd)
     flag = L.spral_ssids_hip_solve_fwd(num, nrhs, x2c.reshape(-1), n)
     if flag < 0:
         return flag
@@ -358,6 +364,25 @@ def ssids_enquire_indef(akeep, fkeep, options=None):
     piv[akeep.invp] = po  # subtree result is by elimination position
     piv = np.where(piv >= 0, piv + 1, piv - 1)
     return piv, d.reshape(n, 2).T.copy(), inf
+
+
+def ssids_enquire_posdef(akeep, fkeep, options=None):
+    """Returns (d, inform): the diagonal of the Cholesky factor in pivot order,
+    as ssids_enquire_posdef (src/ssids/ssids.f90:1256-1291)."""
+    L = _lib.lib()
+    inf = SSIDSInform()
+    if fkeep is None or not fkeep.numeric:
+        inf.flag = SSIDS_ERROR_CALL_SEQUENCE
+        return None, inf
+    if not fkeep.pos_def:
+        inf.flag = SSIDS_ERROR_NOT_LLT
+        return None, inf
+    d = np.zeros(akeep.n, np.float64)
+    flag = L.spral_ssids_hip_enquire_posdef(fkeep.numeric, d)
+    if flag < 0:
+        inf.flag = flag
+        return None, inf
+    return d, inf
 
 
 def ssids_alter(d, akeep, fkeep, options=None):
