@@ -18,8 +18,10 @@
 //               trailing panel columns with v_mfma_f64_16x16x4_f64 on 64x64
 //               tiles (the contribution block is updated lazily by k_gather).
 //  k_swap_*     move failed columns behind the unprocessed candidates.
-//  k_tpp        threshold partial pivoting on the failed columns
-//               (src/ssids/cpu/kernels/ldlt_tpp.cxx:166-270).
+//  k_tail       end of a level, one workgroup per front: APP result,
+//               threshold partial pivoting on the failed columns
+//               (src/ssids/cpu/kernels/ldlt_tpp.cxx:166-270), D and L21 D
+//               operands of the parent's gather, solve position index.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -663,55 +665,6 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
 
 // D (not its inverse) per pivot for the lazy contribution update:
 // Dd[3k] = D_kk, Dd[3k+1] = D_k,partner, Dd[3k+2] = partner offset.
-__global__ void k_dprep(NodeDev *nodes, const int *lvl_slots) {
-  NodeDev &N = nodes[lvl_slots[blockIdx.x]];
-  const int ne = N.nelim;
-  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
-    double e1, e2;
-    int two;
-    pivot_d(N.D, k, ne, e1, e2, two);
-    if (two == 0) {
-      N.Dd[3 * k] = e1;
-      N.Dd[3 * k + 1] = 0.0;
-      N.Dd[3 * k + 2] = 0.0;
-    } else if (two == 1) {
-      N.Dd[3 * k] = e1;
-      N.Dd[3 * k + 1] = e2;
-      N.Dd[3 * k + 2] = 1.0;
-    } else {
-      N.Dd[3 * k] = e2;
-      N.Dd[3 * k + 1] = e1;
-      N.Dd[3 * k + 2] = -1.0;
-    }
-  }
-}
-
-// L21 D for the contribution rows [n, m) of a non-root front: the operand
-// the parent's gather multiplies with L21^T (one load per element there).
-// k_apply writes it for the APP pivots; this covers the pivots eliminated by
-// threshold partial pivoting afterwards.  Workgroups (front, part): rows
-// strided over the parts.
-constexpr int kLdParts = 8;
-__global__ __launch_bounds__(kThreads) void k_ldprep(NodeDev *nodes, const int *lvl_slots) {
-  const NodeDev &N = nodes[lvl_slots[blockIdx.x]];
-  if (N.skip || !N.LD) return;
-  const int ne = N.nelim, n = N.n, m = N.m, ldl = N.ldl;
-  const int t0 = N.nelim_app;  // APP pivots: written by k_apply
-  if (t0 >= ne) return;
-  const double *L = N.L, *Dd = N.Dd;
-  double *LD = N.LD;
-  for (int i = n + blockIdx.y * kThreads + threadIdx.x; i < m; i += kLdParts * kThreads) {
-#pragma unroll 8
-    for (int t = t0; t < ne; ++t) {
-      const double e = Dd[3 * t], f = Dd[3 * t + 1];
-      const int off = (int)Dd[3 * t + 2];
-      double v = L[(size_t)t * ldl + i] * e;
-      if (off != 0) v += L[(size_t)(t + off) * ldl + i] * f;
-      LD[(size_t)t * ldl + i] = v;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Gather assembly of one level (replaces zero + scatter + extend-add + the
 // Schur update of the children's contribution blocks).  A workgroup owns a
@@ -729,10 +682,12 @@ constexpr int kGT = 64;
 // Per level, for every child of the level's fronts: the parent position of
 // each trailing index (pmap) and, for each 64-row tile of the parent, the
 // first trailing index whose parent position reaches the tile (ptab).
+constexpr int kMapParts = 8;  // workgroups per front (trailing indices strided over them)
 __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *lvl_slots,
                                                      const int *child_list) {
   const NodeDev &P = nodes[lvl_slots[blockIdx.x]];
   if (P.skip) return;
+  const int part = blockIdx.y;
   const int T = (P.m + kGT - 1) / kGT;
   int doff = 0;
   for (int ci = P.cbeg; ci < P.cend; ++ci) {
@@ -741,7 +696,7 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
     const int cdoff = doff;
     doff += c.n - c.nelim;
     const int nt = c.m - c.nelim;
-    for (int t = threadIdx.x; t < nt; t += kThreads) {
+    for (int t = part * kThreads + threadIdx.x; t < nt; t += kMapParts * kThreads) {
       const int k = c.nelim + t;
       const int pk = child_pos(c, cdoff, P.nd, k);
       c.pmap[t] = pk;
@@ -751,7 +706,7 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
       if (t == nt - 1)
         for (int I = tile + 1; I <= T; ++I) tab[I] = c.m;
     }
-    if (nt == 0)
+    if (nt == 0 && part == 0)
       for (int I = threadIdx.x; I <= T; I += kThreads) tab[I] = c.m;
   }
 }
@@ -1038,22 +993,6 @@ __global__ __launch_bounds__(kThreads) void k_swap(NodeDev *nodes, const int *lv
   }
 }
 
-__global__ void k_finalize(NodeDev *nodes, const int *lvl_slots, int nlvl) {
-  int li = blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= nlvl) return;
-  NodeDev &N = nodes[lvl_slots[li]];
-  if (N.nsteps > 0) {
-    int p = N.p + N.npass;
-    int q = N.q - (N.w - N.npass);
-    N.p = p;
-    N.q = q;
-    N.nelim = p;
-  } else {
-    N.nelim = 0;
-  }
-  N.nelim_app = N.nelim;
-}
-
 // ---------------------------------------------------------------------------
 // Threshold partial pivoting on the remaining fully-summed columns
 // [nelim, n) of the panel, one workgroup per front.  Updates the panel
@@ -1170,21 +1109,19 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kThreads) void k_tpp(NodeDev *nodes, const int *lvl_slots,
-                                                  FactorParams fp) {
-  __shared__ TppShared sh;
-  __shared__ int s_t;
-  NodeDev &N = nodes[lvl_slots[blockIdx.x]];
+// TPP on the columns [nelim, n) that APP left, by the whole workgroup;
+// returns the new nelim (uniform) and stores it in N.nelim.
+__device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShared &sh,
+                         int &s_t) {
   const int n = N.n, m = N.m;
-  int nelim = N.nelim;
-  if (nelim >= n) return;
-  if (!(fp.pivot_tpp || N.is_root || fp.tpp_failed)) return;
+  if (nelim >= n) return nelim;
+  if (!(fp.pivot_tpp || N.is_root || fp.tpp_failed)) return nelim;
   const double u = fp.u, small = fp.small;
   double *ld = N.bak, *ld2 = N.bak + N.ldbak;
   const int tid = threadIdx.x;
   while (nelim < n) {
     if (tpp_col_small(N, nelim, nelim, m, small, sh)) {
-      if (!fp.action) { if (tid == 0) N.flag = -5; return; }
+      if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
       for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
       if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
       __syncthreads();
@@ -1195,7 +1132,7 @@ __global__ __launch_bounds__(kThreads) void k_tpp(NodeDev *nodes, const int *lvl
     int done = 0;
     for (p = nelim + 1; p < n; ++p) {
       if (tpp_col_small(N, p, nelim, m, small, sh)) {
-        if (!fp.action) { if (tid == 0) N.flag = -5; return; }
+        if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
         tpp_swap(N, p, nelim, m);
         for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
         if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
@@ -1286,6 +1223,87 @@ __global__ __launch_bounds__(kThreads) void k_tpp(NodeDev *nodes, const int *lvl
     }
   }
   if (tid == 0) N.nelim = nelim;
+  return nelim;
+}
+
+// ---------------------------------------------------------------------------
+// End of a level, one workgroup per front, in order:
+//  * the APP result: nelim = p + npass of the last step (nelim_app);
+//  * TPP on the columns APP failed (tpp_front);
+//  * Dd: D per pivot in the (value, partner value, partner offset) form the
+//    gather's L21 D operand uses;
+//  * L21 D for the rows [n, m) and the TPP pivots (k_apply wrote it for the
+//    APP pivots);
+//  * ipos: pre-pivot -> post-pivot position of the fully-summed variables
+//    (the variable-indexed scratch is conflict free: the fully-summed
+//    variable sets of one level are disjoint).
+// Values passed between the phases stay in registers; descriptor fields
+// written here are re-read only after a barrier.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lvl_slots,
+                                                   FactorParams fp, int *scratch) {
+  __shared__ TppShared sh;
+  __shared__ int s_t;
+  NodeDev &N = nodes[lvl_slots[blockIdx.x]];
+  const int tid = threadIdx.x;
+  int nelim;
+  {
+    const int nsteps = N.nsteps, p = N.p, npass = N.npass, q = N.q, w = N.w;
+    nelim = (nsteps > 0) ? p + npass : 0;
+    __syncthreads();  // every thread has read the step state
+    if (tid == 0) {
+      if (nsteps > 0) {
+        N.p = nelim;
+        N.q = q - (w - npass);
+      }
+      N.nelim = nelim;
+      N.nelim_app = nelim;
+    }
+    __syncthreads();
+  }
+  const int t0 = nelim;  // APP pivots end here
+  nelim = tpp_front(N, fp, nelim, sh, s_t);
+  __syncthreads();
+  // Dd
+  for (int k = tid; k < nelim; k += kThreads) {
+    double e1, e2;
+    int two;
+    pivot_d(N.D, k, nelim, e1, e2, two);
+    if (two == 0) {
+      N.Dd[3 * k] = e1;
+      N.Dd[3 * k + 1] = 0.0;
+      N.Dd[3 * k + 2] = 0.0;
+    } else if (two == 1) {
+      N.Dd[3 * k] = e1;
+      N.Dd[3 * k + 1] = e2;
+      N.Dd[3 * k + 2] = 1.0;
+    } else {
+      N.Dd[3 * k] = e2;
+      N.Dd[3 * k + 1] = e1;
+      N.Dd[3 * k + 2] = -1.0;
+    }
+  }
+  __syncthreads();
+  // L21 D for the TPP pivots
+  if (!N.skip && N.LD && t0 < nelim) {
+    const int n = N.n, m = N.m, ldl = N.ldl;
+    const double *L = N.L, *Dd = N.Dd;
+    double *LD = N.LD;
+    for (int i = n + tid; i < m; i += kThreads) {
+      for (int t = t0; t < nelim; ++t) {
+        const double e = Dd[3 * t], f = Dd[3 * t + 1];
+        const int off = (int)Dd[3 * t + 2];
+        double v = L[(size_t)t * ldl + i] * e;
+        if (off != 0) v += L[(size_t)(t + off) * ldl + i] * f;
+        LD[(size_t)t * ldl + i] = v;
+      }
+    }
+  }
+  // ipos
+  const int n = N.n;
+  for (int i = tid; i < n; i += kThreads) scratch[N.perm[i]] = i;
+  __syncthreads();
+  for (int j = tid; j < n; j += kThreads) N.ipos[j] = scratch[N.perm0[j]];
 }
 
 // inertia and delay counts over all fronts (integer atomics: deterministic)
@@ -1430,14 +1448,13 @@ void launch_gather(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
 void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
                    const int *child_list) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_maptab, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, child_list);
+    hipLaunchKernelGGL(k_maptab, dim3(nlvl, kMapParts), dim3(kThreads), 0, st, nodes, slots,
+                       child_list);
 }
-void launch_dprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
-  if (nlvl > 0) hipLaunchKernelGGL(k_dprep, dim3(nlvl), dim3(256), 0, st, nodes, slots);
-}
-void launch_ldprep(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
+void launch_tail(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, FactorParams fp,
+                 int *scratch) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_ldprep, dim3(nlvl, kLdParts), dim3(kThreads), 0, st, nodes, slots);
+    hipLaunchKernelGGL(k_tail, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, fp, scratch);
 }
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
@@ -1462,16 +1479,6 @@ void launch_swap(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
   if (nwg > 0)
     hipLaunchKernelGGL(k_swap, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
                        nlvl, step, phase);
-}
-void launch_finalize(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_finalize, dim3((nlvl + 255) / 256), dim3(256), 0, st, nodes, slots,
-                       nlvl);
-}
-void launch_tpp(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
-                FactorParams fp) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_tpp, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, fp);
 }
 void launch_stats(hipStream_t st, const NodeDev *nodes, int nslots, int *counters) {
   if (nslots > 0)

@@ -25,8 +25,6 @@ namespace ssids_hip {
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
 void launch_gather(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
                    const double *, const int64_t *, const int *, const int *, const double *);
-void launch_dprep(hipStream_t, int, NodeDev *, const int *);
-void launch_ldprep(hipStream_t, int, NodeDev *, const int *);
 void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
 void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, int,
@@ -34,10 +32,8 @@ void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, in
 void launch_update(hipStream_t, int, NodeDev *, const int *, const int *, int, int, int,
                    FactorParams);
 void launch_swap(hipStream_t, int, NodeDev *, const int *, const int *, int, int, int);
-void launch_finalize(hipStream_t, int, NodeDev *, const int *);
-void launch_tpp(hipStream_t, int, NodeDev *, const int *, FactorParams);
+void launch_tail(hipStream_t, int, NodeDev *, const int *, FactorParams, int *);
 void launch_stats(hipStream_t, const NodeDev *, int, int *);
-void launch_ipos(hipStream_t, int, NodeDev *, const int *, int *);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
 void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
@@ -639,13 +635,9 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
         launch_swap(st, lv.cap[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 1);
       }
     }
-    launch_finalize(st, nl, N.d_nodes, d_slots);
-    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
-    launch_dprep(st, nl, N.d_nodes, d_slots);
-    launch_ldprep(st, nl, N.d_nodes, d_slots);
+    launch_tail(st, nl, N.d_nodes, d_slots, fp, N.d_scratch_n);
     if (lv.cap[2 * ms + 2] > 0)
       launch_update(st, lv.cap[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
-    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
   }
   HIPCHK(hipGetLastError());
   launch_pack_nodes(st, N.d_nodes, nn, P.d_pack);
@@ -750,13 +742,9 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
         launch_swap(st, tot[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 1);
       }
     }
-    launch_finalize(st, nl, N.d_nodes, d_slots);
-    launch_tpp(st, nl, N.d_nodes, d_slots, fp);
-    launch_dprep(st, nl, N.d_nodes, d_slots);
-    launch_ldprep(st, nl, N.d_nodes, d_slots);
+    launch_tail(st, nl, N.d_nodes, d_slots, fp, N.d_scratch_n);
     if (tot[2 * ms + 2] > 0)
       launch_update(st, tot[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
-    launch_ipos(st, nl, N.d_nodes, d_slots, N.d_scratch_n);
     HIPCHK(hipGetLastError());
     // read back the level's results (nelim, flag)
     std::vector<NodeDev> back(nl);

@@ -119,15 +119,6 @@ __device__ inline void stage_lower(gcdouble *PL, int ldl, int j0, int wb,
   }
 }
 
-__global__ void k_ipos(NodeDev *nodes, const int *slots, int *scratch) {
-  // one workgroup per front of the level; fully-summed variable sets of one
-  // level are disjoint, so the variable-indexed scratch is conflict free.
-  const NodeDev &N = nodes[slots[blockIdx.x]];
-  for (int i = threadIdx.x; i < N.n; i += blockDim.x) scratch[N.perm[i]] = i;
-  __syncthreads();
-  for (int j = threadIdx.x; j < N.n; j += blockDim.x) N.ipos[j] = scratch[N.perm0[j]];
-}
-
 // ---- forward --------------------------------------------------------------
 // One workgroup per front.  X[0:n] = own b + the children's updates on
 // fully-summed positions (children taken four at a time, a wave each; their
@@ -659,10 +650,6 @@ void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *
                      nrhs);
 }
 
-void launch_ipos(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int *scratch) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_ipos, dim3(nlvl), dim3(256), 0, st, nodes, slots, scratch);
-}
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                     const int *child_list, double *x, int ldx, int nrhs) {
   if (nlvl > 0)
