@@ -24,6 +24,7 @@
 //               operands of the parent's gather, solve position index.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -1307,34 +1308,54 @@ __global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lv
 }
 
 // inertia and delay counts over all fronts (integer atomics: deterministic)
-__global__ void k_stats(const NodeDev *nodes, int nslots, int *counters) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
-  const NodeDev &N = nodes[s];
-  int neg = 0, two = 0, zero = 0;
-  const double *d = N.D;
-  for (int i = 0; i < N.nelim;) {
-    double a11 = d[2 * i], a21 = d[2 * i + 1];
-    if (i + 1 == N.nelim || isfinite(d[2 * i + 2])) {
-      if (a11 == 0.0) zero++;
-      if (a11 < 0.0) neg++;
-      i += 1;
-    } else {
-      double a22 = d[2 * i + 3];
-      two++;
-      double det = a11 * a22 - a21 * a21, tr = a11 + a22;
-      if (det < 0) neg++;
-      else if (tr < 0) neg += 2;
-      i += 2;
+constexpr int kStatsWaves = 16, kStatsGroups = 64;
+__global__ __launch_bounds__(64 * kStatsWaves) void k_stats(const NodeDev *nodes, int nslots,
+                                                            int *counters) {
+  // a wave per front (fronts strided over the grid's waves), a lane per
+  // pivot: pivot i is the second member of a 2x2 when its D^{-1} slot holds
+  // infinity ([d11, d21 | inf, d22]), the first member when the next one is,
+  // a 1x1 otherwise.  Sums go wave -> workgroup (LDS) -> 7 atomics per
+  // workgroup (integer: deterministic).
+  __shared__ int red[kStatsWaves][7];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int c[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int s = blockIdx.x * kStatsWaves + wv; s < nslots; s += gridDim.x * kStatsWaves) {
+    const NodeDev &N = nodes[s];
+    const int ne = N.nelim;
+    gcdouble *d = (gcdouble *)N.D;
+    for (int i = lane; i < ne; i += 64) {
+      const double a11 = d[2 * i];
+      if (!isfinite(a11)) continue;
+      if (i + 1 < ne && !isfinite(d[2 * i + 2])) {
+        const double a21 = d[2 * i + 1], a22 = d[2 * i + 3];
+        c[1]++;
+        const double det = a11 * a22 - a21 * a21, tr = a11 + a22;
+        if (det < 0) c[0]++;
+        else if (tr < 0) c[0] += 2;
+      } else {
+        if (a11 == 0.0) c[2]++;
+        if (a11 < 0.0) c[0]++;
+      }
+    }
+    if (lane == 0) {
+      c[3] += N.n - N.nelim;
+      c[4] += N.n - N.nelim_app;  // not_first_pass
+      c[5] += (N.flag == -5);
+      c[6] += (N.flag == -6);
     }
   }
-  atomicAdd(&counters[0], neg);
-  atomicAdd(&counters[1], two);
-  atomicAdd(&counters[2], zero);
-  atomicAdd(&counters[3], N.n - N.nelim);
-  atomicAdd(&counters[4], N.n - N.nelim_app);  // not_first_pass
-  atomicAdd(&counters[5], N.flag == -5 ? 1 : 0);
-  atomicAdd(&counters[6], N.flag == -6 ? 1 : 0);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    int v = c[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[wv][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    int v = 0;
+    for (int q = 0; q < kStatsWaves; ++q) v += red[q][threadIdx.x];
+    if (v) atomicAdd(&counters[threadIdx.x], v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1482,8 +1503,9 @@ void launch_swap(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
 }
 void launch_stats(hipStream_t st, const NodeDev *nodes, int nslots, int *counters) {
   if (nslots > 0)
-    hipLaunchKernelGGL(k_stats, dim3((nslots + 255) / 256), dim3(256), 0, st, nodes, nslots,
-                       counters);
+    hipLaunchKernelGGL(k_stats,
+                       dim3(std::min(kStatsGroups, (nslots + kStatsWaves - 1) / kStatsWaves)),
+                       dim3(64 * kStatsWaves), 0, st, nodes, nslots, counters);
 }
 
 }  // namespace ssids_hip
