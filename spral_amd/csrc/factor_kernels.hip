@@ -712,7 +712,7 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_gather(
+__global__ __launch_bounds__(kThreads, 3) void k_gather(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
     const int *acix, const double *scaling) {
@@ -854,7 +854,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(
       for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      constexpr int KC = 4;  // k-steps (of 4 pivots) per buffer
+      constexpr int KC = 2;  // k-steps (of 4 pivots) per buffer
       double xa0[KC], xa1[KC], xb0[KC], xb1[KC], ya0[KC], ya1[KC], yb0[KC], yb1[KC];
       auto load = [&](int T0, double *A0, double *A1, double *B0, double *B1) {
 #pragma unroll
