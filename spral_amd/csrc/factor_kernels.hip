@@ -712,7 +712,7 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
   }
 }
 
-__global__ __launch_bounds__(kThreads, 3) void k_gather(
+__global__ __launch_bounds__(kThreads, 4) void k_gather(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
     const int *acix, const double *scaling) {
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kThreads, 3) void k_gather(
     }
   }
   // this tile's A entries (entry -> column precomputed), loaded up front
-  constexpr int kAE = 4;  // entries per thread held in registers
+  constexpr int kAE = 1;  // entries per thread held in registers
   const int cs0 = max(c0, nd) - nd, cs1 = min(c0 + kGT, n) - nd;
   int64_t ae0 = 0, ae1 = 0;
   if (cs1 > cs0) {
