@@ -363,8 +363,11 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
 __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *lvl_slots,
                                                     const int *wg_off, int nlvl, int step,
                                                     FactorParams fp) {
-  __shared__ double ms[kBlk][kBlk + 1];  // M = L_bb^{-1}, then the left-rows chunk
-  __shared__ double xs[kBlk][kBlk + 1];  // A21[:, lp] M^T for this workgroup's 64 rows
+  // M = L_bb^{-1} (or the left-rows chunk), then, once the MFMAs have read
+  // it, X = A21[:, lp] M^T for this workgroup's 64 rows (one tile of LDS:
+  // four workgroups fit a CU)
+  __shared__ double ms[kBlk][kBlk + 1];
+  double (*xs)[kBlk + 1] = ms;
   __shared__ double dv[2 * kBlk + 2];
   __shared__ int lps[kBlk];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
@@ -431,6 +434,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
       acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], ms[g * 16 + ri][4 * u + kq], acc[g],
                                                     0, 0, 0);
   }
+  __syncthreads();  // every wave has read M before X overwrites it
 #pragma unroll
   for (int g = 0; g < 4; ++g)
 #pragma unroll
@@ -547,8 +551,10 @@ __device__ inline void tile_of(int tile, int TI, int TJ, int &I, int &J) {
 __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *lvl_slots,
                                                      const int *wg_off, int nlvl, int step,
                                                      int mode, FactorParams fp) {
-  __shared__ double As[kTile][kBlk + 1];
-  __shared__ double Bs[kTile][kBlk + 1];
+  // 32-pivot K chunks: the two operand tiles take 34 KB, four workgroups a CU
+  constexpr int kUpdK = 32;
+  __shared__ double As[kTile][kUpdK + 1];
+  __shared__ double Bs[kTile][kUpdK + 1];
   const int li = find_node(wg_off, nlvl, blockIdx.x);
   if (li < 0) return;
   NodeDev &N = nodes[lvl_slots[li]];
@@ -583,7 +589,7 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
   const int ntiles = TJ * TI - TJ * (TJ - 1) / 2;
   const double *D = N.D;
   const int ldl = N.ldl;
-  const int Kc = kBlk;  // K chunk
+  const int Kc = kUpdK;  // K chunk
   for (int tile = local; tile < ntiles; tile += count) {
     int I, J;
     tile_of(tile, TI, TJ, I, J);
