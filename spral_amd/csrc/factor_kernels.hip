@@ -76,7 +76,7 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 // |l| <= 1/u over the full column.  The elimination order lp[] permutes the
 // block on write back.
 // ---------------------------------------------------------------------------
-constexpr int kDiagWaves = 8;
+constexpr int kDiagWaves = 4;
 
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl_slots,
@@ -334,22 +334,23 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   }
   __syncthreads();
   {
-    // wave v: columns [8v, 8v+8) (NW = 8); lane = 8 * (column in wave) + phase,
-    // rows phase, phase+8, ... of that column in registers
-    static_assert(NW == 8, "inverse layout assumes 8 waves");
-    const int col = wv * 8 + (lane >> 3), ph = lane & 7;
-    double mm[8];
+    // wave v: columns [Cv, Cv+C) (C = 64/NW); NW lanes per column: lane =
+    // NW * (column in wave) + phase, rows phase, phase+NW, ... in registers
+    constexpr int C = kBlk / NW;
+    static_assert((NW & (NW - 1)) == 0 && C * NW == kBlk, "NW: a power of two");
+    const int col = wv * C + lane / NW, ph = lane % NW;
+    double mm[C];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) mm[q] = (ph + 8 * q == col) ? 1.0 : 0.0;
+    for (int q = 0; q < C; ++q) mm[q] = (ph + NW * q == col) ? 1.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < kBlk - 1; ++k) {
-      const double mk = __shfl(mm[k >> 3], (lane & ~7) | (k & 7));
+      const double mk = __shfl(mm[k / NW], (lane & ~(NW - 1)) | (k % NW));
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (ph + 8 * q > k) mm[q] -= mi[k][ph + 8 * q] * mk;
+      for (int q = 0; q < C; ++q)
+        if (ph + NW * q > k) mm[q] -= mi[k][ph + NW * q] * mk;
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) N.W[(ph + 8 * q) * kBlk + col] = mm[q];  // row-major
+    for (int q = 0; q < C; ++q) N.W[(ph + NW * q) * kBlk + col] = mm[q];  // row-major
   }
   // permute the variable list (rows of earlier eliminated columns: k_apply)
   const int pv = (tid < w) ? N.perm[p + lp[tid]] : 0;
