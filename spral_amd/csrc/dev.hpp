@@ -83,14 +83,16 @@ __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ inline int idiv_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // Per-level launch tables.  A level whose fronts need at most S APP steps has
-// 2S+3 tables, each an exclusive prefix over the level's fronts of the
+// 2S+5 tables, each an exclusive prefix over the level's fronts of the
 // workgroups one launch gives each front (last entry = total):
 //   0          gather: lower 64x64 tiles of the front
 //   1..S       apply, step t-1: t-1 chunks of earlier rows + 64-row blocks below
 //   S+1        failed-column swap
 //   S+2..2S+1  trailing panel update, step t-S-2
 //   2S+2       contribution update of a root's TPP pivots
-__host__ __device__ inline int level_tables(int S) { return 2 * S + 3; }
+//   2S+3       gather, first tile column only
+//   2S+4       gather, the remaining tiles
+__host__ __device__ inline int level_tables(int S) { return 2 * S + 5; }
 __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int nsteps,
                                                  int is_root) {
   if (t == 0) {
@@ -110,9 +112,15 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
     const int nt = TJ * TI - TJ * (TJ - 1) / 2;
     return imax(1, imin(nt, 4096));
   }
-  if (!is_root || m <= n) return 0;
-  const int T = idiv_up(m - n, kBlk);
-  return imax(1, imin(T * (T + 1) / 2, 4096));
+  if (t == 2 * S + 2) {
+    if (!is_root || m <= n) return 0;
+    const int T = idiv_up(m - n, kBlk);
+    return imax(1, imin(T * (T + 1) / 2, 4096));
+  }
+  // gather split in two launches: the first 64-column tile column (what
+  // the first pivot step needs), then the remaining tiles
+  const int T = idiv_up(m, kBlk);
+  return (t == 2 * S + 3) ? T : T * (T + 1) / 2 - T;
 }
 
 // Index li of the table segment holding workgroup b: off[li] <= b < off[li+1]

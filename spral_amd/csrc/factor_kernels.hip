@@ -719,6 +719,9 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
   }
 }
 
+// PART 0: every tile of the level's fronts; 1: the first tile column only
+// (the pivot block and the rows below it); 2: the remaining tiles
+template <int PART>
 __global__ __launch_bounds__(kThreads, 4) void k_gather(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
@@ -738,8 +741,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_gather(
   gcint *prows = (gcint *)P.rows;
   const int64_t *pacol = P.acol;
   const int T = (m + kGT - 1) / kGT;
-  const int local = blockIdx.x - wg_off[li];
-  if (local >= T * (T + 1) / 2) return;
+  const int local = blockIdx.x - wg_off[li] + (PART == 2 ? T : 0);
+  if (local >= (PART == 1 ? T : T * (T + 1) / 2)) return;
   int I, J;
   tile_of(local, T, T, I, J);
   const int r0 = I * kGT, c0 = J * kGT;
@@ -1465,13 +1468,14 @@ void launch_pack_nodes(hipStream_t st, const NodeDev *nodes, int nn, int *out) {
   if (nn > 0)
     hipLaunchKernelGGL(k_pack_nodes, dim3((nn + 255) / 256), dim3(256), 0, st, nodes, nn, out);
 }
-void launch_gather(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+void launch_gather(hipStream_t st, int part, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, const int *child_list, const double *aval,
                    const int64_t *asrc, const int *arow, const int *acix,
                    const double *scaling) {
-  if (nwg > 0)
-    hipLaunchKernelGGL(k_gather, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,
-                       child_list, aval, asrc, arow, acix, scaling);
+  if (nwg <= 0) return;
+  auto kern = (part == 1) ? k_gather<1> : (part == 2) ? k_gather<2> : k_gather<0>;
+  hipLaunchKernelGGL(kern, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,
+                     child_list, aval, asrc, arow, acix, scaling);
 }
 void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
                    const int *child_list) {

@@ -23,7 +23,7 @@
 namespace ssids_hip {
 
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
-void launch_gather(hipStream_t, int, NodeDev *, const int *, const int *, int, const int *,
+void launch_gather(hipStream_t, int, int, NodeDev *, const int *, const int *, int, const int *,
                    const double *, const int64_t *, const int *, const int *, const double *);
 void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
@@ -212,6 +212,10 @@ struct Plan {
 struct Numeric {
   Symbolic *symb = nullptr;
   hipStream_t stream = nullptr;
+  // second stream of the static plan: a level's gather beyond its first tile
+  // column runs there while the first pivot step runs on `stream`
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_level = nullptr, ev_rest = nullptr;
   Pool pool;
   Plan plan;
   std::vector<int> nd_seen;               // per slot: most delays seen in a factor
@@ -268,6 +272,9 @@ struct Numeric {
     if (d_r) (void)hipFree(d_r);
     if (d_norms) (void)hipFree(d_norms);
     if (d_active) (void)hipFree(d_active);
+    if (ev_level) (void)hipEventDestroy(ev_level);
+    if (ev_rest) (void)hipEventDestroy(ev_rest);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -612,6 +619,20 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   HIPCHK(hipMemcpyAsync(N.d_nodes, P.d_tmpl, nn * sizeof(NodeDev), hipMemcpyDeviceToDevice,
                         st));
   int *overflow = N.d_counters + 8;
+  // The gather of a level is split: its first tile column (the first pivot
+  // block and the rows below it) on `st`, the other tiles on `st2`.  The
+  // first step's pivot block and apply touch only the first 64 columns, so
+  // they overlap the rest of the gather; `st` joins `st2` before the first
+  // trailing update.  A timing run keeps one whole-level gather per level
+  // (measured alone).
+  const bool split = !opt.timing;
+  hipStream_t st2 = nullptr;
+  if (split) {
+    if (!N.stream2) HIPCHK(hipStreamCreate(&N.stream2));
+    if (!N.ev_level) HIPCHK(hipEventCreateWithFlags(&N.ev_level, hipEventDisableTiming));
+    if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
+    st2 = N.stream2;
+  }
   for (int L = 0; L < nlev; ++L) {
     const int s0 = S.lvl_ptr[L], nl = S.lvl_ptr[L + 1] - s0;
     const Plan::Level &lv = P.lv[L];
@@ -621,13 +642,32 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     const int *d_slots = N.d_slots_all + s0;
     launch_level_setup(st, N.d_nodes, s0, nl, S.d_child, fp.pivot_tpp, ms, tab, overflow);
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
-    if (opt.timing) tev.emplace_back(timed_event(st));
-    launch_gather(st, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
-                  S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
-    if (opt.timing) tev.emplace_back(timed_event(st));
+    bool joined = true;
+    if (split) {
+      HIPCHK(hipEventRecord(N.ev_level, st));
+      HIPCHK(hipStreamWaitEvent(st2, N.ev_level, 0));
+      launch_gather(st, 1, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
+                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+      launch_gather(st2, 2, lv.cap[2 * ms + 4], N.d_nodes, d_slots, toff(2 * ms + 4), nl,
+                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+      HIPCHK(hipEventRecord(N.ev_rest, st2));
+      joined = false;
+    } else {
+      if (opt.timing) tev.emplace_back(timed_event(st));
+      launch_gather(st, 0, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
+                    S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+      if (opt.timing) tev.emplace_back(timed_event(st));
+    }
+    auto join = [&]() {
+      if (!joined) HIPCHK(hipStreamWaitEvent(st, N.ev_rest, 0));
+      joined = true;
+    };
     for (int stp = 0; stp < ms; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
       launch_apply(st, lv.cap[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
+      // one step (every n <= 64): the update stays inside the first tile
+      // column too; with more steps it reaches the later columns
+      if (ms > 1) join();
       launch_update(st, lv.cap[ms + 2 + stp], N.d_nodes, d_slots, toff(ms + 2 + stp), nl, stp,
                     0, fp);
       if (lv.multi) {
@@ -635,6 +675,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
         launch_swap(st, lv.cap[ms + 1], N.d_nodes, d_slots, toff(ms + 1), nl, stp, 1);
       }
     }
+    join();
     launch_tail(st, nl, N.d_nodes, d_slots, fp, N.d_scratch_n);
     if (lv.cap[2 * ms + 2] > 0)
       launch_update(st, lv.cap[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
@@ -729,8 +770,8 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
     const int ms = maxsteps;
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
     if (opt.timing) tev.emplace_back(timed_event(st));
-    launch_gather(st, tot[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval, S.d_asrc,
-                  S.d_arow, S.d_acix, N.d_scal);
+    launch_gather(st, 0, tot[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
+                  S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
     if (opt.timing) tev.emplace_back(timed_event(st));
     for (int stp = 0; stp < ms; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
