@@ -415,15 +415,26 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
   const int r = rb + wave * 16 + ri;  // A row of this lane (fragment layout)
   const bool rok = r < m;
   double av[kBlk / 4];
+  {  // unconditional loads (row and pivot clamped into range), unused masked
+    const int rc = imin(r, m - 1);
+    gcdouble *Lg = (gcdouble *)L;
 #pragma unroll
-  for (int u = 0; u < kBlk / 4; ++u) {
-    const int k = 4 * u + kq;
-    av[u] = (rok && k < w) ? L[(size_t)(p + lps[k]) * ldl + r] : 0.0;
+    for (int u = 0; u < kBlk / 4; ++u) {
+      const int k = 4 * u + kq;
+      av[u] = masked(Lg[(size_t)(p + lps[imin(k, w - 1)]) * ldl + rc], rok && k < w);
+    }
   }
+  // descriptor fields once: the global stores below could otherwise alias
+  // them and force a reload per iteration
+  gdouble *bak = (gdouble *)N.bak;
+  gdouble *LD = (gdouble *)N.LD;
+  gdouble *Lw = (gdouble *)L;
+  unsigned long long *colmax = N.colmax;
+  const int ldbak = N.ldbak, nfs = N.n;
 #pragma unroll
   for (int u = 0; u < kBlk / 4; ++u) {
     const int k = 4 * u + kq;
-    if (rok && k < w) N.bak[(size_t)k * N.ldbak + (r - r0)] = av[u];
+    if (rok && k < w) bak[(size_t)k * ldbak + (r - r0)] = av[u];
   }
   double4_t acc[4];
 #pragma unroll
@@ -466,10 +477,10 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
         else y = xj * d11;
       }
       if (ok) {
-        L[(size_t)(p + j) * ldl + rr] = y;
+        Lw[(size_t)(p + j) * ldl + rr] = y;
         // L21 D is the product before D^{-1} (zero for a zero pivot): the
         // parent's gather operand, for the contribution rows
-        if (N.LD && rr >= N.n) N.LD[(size_t)(p + j) * ldl + rr] = zero_piv ? 0.0 : xj;
+        if (LD && rr >= nfs) LD[(size_t)(p + j) * ldl + rr] = zero_piv ? 0.0 : xj;
       }
     }
     // column maximum over this wave's 64 rows (NaN propagates as a failure)
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
     if (lane == 0 && j < w) {
       const unsigned long long bits = isnan(v) ? 0x7ff0000000000001ull
                                                : (unsigned long long)__double_as_longlong(v);
-      atomicMax(&N.colmax[j], bits);
+      atomicMax(&colmax[j], bits);
     }
   }
 }
