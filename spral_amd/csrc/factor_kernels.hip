@@ -560,7 +560,7 @@ __device__ inline void tile_of(int tile, int TI, int TJ, int &I, int &J) {
   I = J + t;
 }
 
-__global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *lvl_slots,
+__global__ __launch_bounds__(kThreads, 4) void k_update(NodeDev *nodes, const int *lvl_slots,
                                                      const int *wg_off, int nlvl, int step,
                                                      int mode, FactorParams fp) {
   // 32-pivot K chunks: the two operand tiles take 34 KB, four workgroups a CU
@@ -602,6 +602,14 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
   const double *D = N.D;
   const int ldl = N.ldl;
   const int Kc = kUpdK;  // K chunk
+  // descriptor fields once (the stores below could alias them)
+  gdouble *NL = (gdouble *)N.L;
+  gdouble *NC = (gdouble *)N.C;
+  gcdouble *dbak = (gcdouble *)N.dbak;
+  gcdouble *bak = (gcdouble *)N.bak;
+  const int *lpv = N.lp;
+  const int nfs = N.n, ldc = N.ldc, ldbak = N.ldbak;
+  const int gmax = imin(m, cend) - 1;  // last row a B operand reads
   for (int tile = local; tile < ntiles; tile += count) {
     int I, J;
     tile_of(tile, TI, TJ, I, J);
@@ -618,29 +626,25 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
       const int kn = min(Kc, K - k0);
       const int Kp = (kn + 3) & ~3;
       __syncthreads();
-      for (int idx = tid; idx < kTile * Kp; idx += kThreads) {
-        int i = idx % kTile, kk = idx / kTile;
-        int k = k0 + kk;
-        int gi = r0 + i, gj = c0 + i;
-        double av = 0.0, bv = 0.0;
-        if (kk < kn) {
-          if (gi < m) av = N.L[(size_t)(a0 + k) * ldl + gi];
-          if (gj < m && gj < cend) {
-            double e1, e2;
-            int two;
-            pivot_d(D, a0 + k, a0 + K, e1, e2, two);
-            double lk = N.L[(size_t)(a0 + k) * ldl + gj];
-            if (two == 0) {
-              bv = lk * e1;
-            } else if (two == 1) {
-              bv = lk * e1 + N.L[(size_t)(a0 + k + 1) * ldl + gj] * e2;
-            } else {
-              bv = N.L[(size_t)(a0 + k - 1) * ldl + gj] * e1 + lk * e2;
-            }
-          }
-        }
+      // every thread's loads unconditional (pivot and rows clamped, a 2x2
+      // partner always inside the pivots), the entries past the chunk or the
+      // front masked to zero.  B = L D keeps the expression of each pivot
+      // kind as it is (its rounding is part of the result).
+#pragma unroll
+      for (int it = 0; it < kTile * kUpdK / kThreads; ++it) {
+        const int idx = tid + it * kThreads, i = idx % kTile, kk = idx / kTile;
+        const int kc = imin(kk, kn - 1), k = a0 + k0 + kc;
+        const int gi = r0 + i, gj = c0 + i, gjc = imin(gj, gmax);
+        const double av = masked(NL[(size_t)k * ldl + imin(gi, m - 1)], kk < kn && gi < m);
+        double e1, e2;
+        int two;
+        pivot_d(D, k, a0 + K, e1, e2, two);
+        const int ofs = (two == 1) ? 1 : (two == 2) ? -1 : 0;
+        const double lk = NL[(size_t)k * ldl + gjc];
+        const double lo = NL[(size_t)(k + ofs) * ldl + gjc];
+        double bv = (two == 0) ? lk * e1 : (two == 1) ? lk * e1 + lo * e2 : lo * e1 + lk * e2;
         As[i][kk] = av;
-        Bs[i][kk] = bv;
+        Bs[i][kk] = masked(bv, kk < kn && gj <= gmax);
       }
       __syncthreads();
       for (int kk = 0; kk < Kp; kk += 4) {
@@ -667,12 +671,13 @@ __global__ __launch_bounds__(kThreads) void k_update(NodeDev *nodes, const int *
           int j = wc + bj * 16 + (lane & 15);
           int gi = r0 + i, gj = c0 + j;
           if (gi < m && gj < cend && gi >= gj) {
-            double *dst = front_at(N, gi, gj);
+            gdouble *dst = (gj < nfs) ? NL + (size_t)gj * ldl + gi
+                                      : NC + (size_t)(gj - nfs) * ldc + (gi - nfs);
             double old;
             if (gj >= fa && gj < fb) {
               int jj = gj - p;
-              if (gi < p + w) old = N.dbak[N.lp[gi - p] + N.lp[jj] * kBlk];
-              else old = N.bak[(size_t)jj * N.ldbak + (gi - p - w)];
+              if (gi < p + w) old = dbak[lpv[gi - p] + lpv[jj] * kBlk];
+              else old = bak[(size_t)jj * ldbak + (gi - p - w)];
             } else {
               old = *dst;
             }
