@@ -78,9 +78,34 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 // ---------------------------------------------------------------------------
 constexpr int kDiagWaves = 4;
 
-template <int NW>
+constexpr int kGT = 64;  // gather tile edge
+
+template <int PART>
+__device__ __forceinline__ int gather_tile(
+    NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
+    const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
+    const int *acix, const double *scaling, double (&acc)[kGT][kGT + 1], int (&mr)[kGT],
+    int (&mc)[kGT]);
+
+struct GatherArgs {  // the level's gather inputs (a pivot launch that assembles)
+  const int *wg_off;
+  int nlvl;
+  const int *child_list;
+  const double *aval;
+  const int64_t *asrc;
+  const int *arow;
+  const int *acix;
+  const double *scaling;
+};
+
+// FIRST: one workgroup per tile of the first tile column of every front (the
+// gather's part 1); the workgroup that built a front's diagonal tile (0, 0)
+// then runs step 0's pivot block on it, as soon as its block is assembled
+// and in a workgroup that already holds its CU, instead of queueing behind
+// the rest of the level's gather.
+template <int NW, bool FIRST>
 __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl_slots,
-                                                  int step, FactorParams fp) {
+                                                  int step, FactorParams fp, GatherArgs ga) {
   constexpr int R = kBlk / NW;
   __shared__ double t[kBlk][kBlk + 1];  // load staging, then L by step: t[k][row]
   __shared__ double cs2[2][2][kBlk];    // broadcast pivot columns (by step parity)
@@ -89,7 +114,18 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
   __shared__ double dinv[2 * kBlk + 2];
   __shared__ int lp[kBlk], inv_lp[kBlk], kind[kBlk];
   __shared__ double mi[kBlk][kBlk + 1];  // inverse of L_bb (elimination order)
-  const int s = lvl_slots[blockIdx.x];
+  int s;
+  if constexpr (FIRST) {
+    static_assert(64 * NW == kThreads, "one workgroup shape for both phases");
+    static_assert(kGT == kBlk, "the gather tile is the pivot block's staging array");
+    // the tile's LDS is the pivot block's own (t, lp, inv_lp), used in turn
+    s = gather_tile<1>(nodes, lvl_slots, ga.wg_off, ga.nlvl, ga.child_list, ga.aval, ga.asrc,
+                       ga.arow, ga.acix, ga.scaling, t, lp, inv_lp);
+    if (s < 0) return;
+    __syncthreads();  // the block's stores are visible to the whole workgroup
+  } else {
+    s = lvl_slots[blockIdx.x];
+  }
   NodeDev &N = nodes[s];
   if (step >= N.nsteps) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, r0 = wv * R;
@@ -701,8 +737,6 @@ __global__ __launch_bounds__(kThreads, 4) void k_update(NodeDev *nodes, const in
 //   global memory in the fragment layout), both through the maps;
 //   then the tile's A entries are added and the tile is stored once.
 // ---------------------------------------------------------------------------
-constexpr int kGT = 64;
-
 // Per level, for every child of the level's fronts: the parent position of
 // each trailing index (pmap) and, for each 64-row tile of the parent, the
 // first trailing index whose parent position reaches the tile (ptab).
@@ -737,15 +771,17 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
 
 // PART 0: every tile of the level's fronts; 1: the first tile column only
 // (the pivot block and the rows below it); 2: the remaining tiles
+// Returns the front's slot when this workgroup built the front's first
+// diagonal tile (0, 0), else -1.
+
 template <int PART>
-__global__ __launch_bounds__(kThreads, 4) void k_gather(
+__device__ __forceinline__ int gather_tile(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
-    const int *acix, const double *scaling) {
-  __shared__ double acc[kGT][kGT + 1];
-  __shared__ int mr[kGT], mc[kGT];
+    const int *acix, const double *scaling, double (&acc)[kGT][kGT + 1], int (&mr)[kGT],
+    int (&mc)[kGT]) {
   const int li = find_node(wg_off, nlvl, blockIdx.x);
-  if (li < 0) return;
+  if (li < 0) return -1;
   const int s = lvl_slots[li];
   NodeDev &P = nodes[s];
   // descriptor fields once, into registers (loads below never wait on them)
@@ -758,7 +794,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_gather(
   const int64_t *pacol = P.acol;
   const int T = (m + kGT - 1) / kGT;
   const int local = blockIdx.x - wg_off[li] + (PART == 2 ? T : 0);
-  if (local >= (PART == 1 ? T : T * (T + 1) / 2)) return;
+  if (local >= (PART == 1 ? T : T * (T + 1) / 2)) return -1;
   int I, J;
   tile_of(local, T, T, I, J);
   const int r0 = I * kGT, c0 = J * kGT;
@@ -958,7 +994,20 @@ __global__ __launch_bounds__(kThreads, 4) void k_gather(
       else PC[(size_t)(gc - n) * pldc + (gr - n)] = acc[i][j];
     }
   }
+  return (I == 0 && J == 0) ? s : -1;
 }
+
+template <int PART>
+__global__ __launch_bounds__(kThreads, 4) void k_gather(
+    NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
+    const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
+    const int *acix, const double *scaling) {
+  __shared__ double acc[kGT][kGT + 1];
+  __shared__ int mr[kGT], mc[kGT];
+  gather_tile<PART>(nodes, lvl_slots, wg_off, nlvl, child_list, aval, asrc, arow, acix,
+                    scaling, acc, mr, mc);
+}
+
 
 // ---------------------------------------------------------------------------
 // Move failed columns of this step behind the unprocessed candidates:
@@ -1484,6 +1533,15 @@ void launch_pack_nodes(hipStream_t st, const NodeDev *nodes, int nn, int *out) {
   if (nn > 0)
     hipLaunchKernelGGL(k_pack_nodes, dim3((nn + 255) / 256), dim3(256), 0, st, nodes, nn, out);
 }
+void launch_gather_first(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
+                         const int *wg_off, int nlvl, const int *child_list,
+                         const double *aval, const int64_t *asrc, const int *arow,
+                         const int *acix, const double *scaling, FactorParams fp) {
+  const GatherArgs ga{wg_off, nlvl, child_list, aval, asrc, arow, acix, scaling};
+  if (nwg > 0)
+    hipLaunchKernelGGL((k_diag<kDiagWaves, true>), dim3(nwg), dim3(kThreads), 0, st, nodes,
+                       slots, 0, fp, ga);
+}
 void launch_gather(hipStream_t st, int part, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, const int *child_list, const double *aval,
                    const int64_t *asrc, const int *arow, const int *acix,
@@ -1507,8 +1565,8 @@ void launch_tail(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, Fac
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_diag<kDiagWaves>, dim3(nlvl), dim3(64 * kDiagWaves), 0, st, nodes,
-                       slots, step, fp);
+    hipLaunchKernelGGL((k_diag<kDiagWaves, false>), dim3(nlvl), dim3(64 * kDiagWaves), 0, st,
+                       nodes, slots, step, fp, GatherArgs{});
 }
 void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                   const int *wg_off, int nlvl, int step, FactorParams fp) {

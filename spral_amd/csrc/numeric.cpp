@@ -25,6 +25,9 @@ namespace ssids_hip {
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
 void launch_gather(hipStream_t, int, int, NodeDev *, const int *, const int *, int, const int *,
                    const double *, const int64_t *, const int *, const int *, const double *);
+void launch_gather_first(hipStream_t, int, NodeDev *, const int *, const int *, int,
+                         const int *, const double *, const int64_t *, const int *,
+                         const int *, const double *, FactorParams);
 void launch_maptab(hipStream_t, int, NodeDev *, const int *, const int *);
 void launch_diag(hipStream_t, int, NodeDev *, const int *, int, FactorParams);
 void launch_apply(hipStream_t, int, NodeDev *, const int *, const int *, int, int,
@@ -646,8 +649,9 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     if (split) {
       HIPCHK(hipEventRecord(N.ev_level, st));
       HIPCHK(hipStreamWaitEvent(st2, N.ev_level, 0));
-      launch_gather(st, 1, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
-                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+      // the first tile column, and step 0's pivot blocks in its diagonal tiles
+      launch_gather_first(st, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
+                          S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp);
       launch_gather(st2, 2, lv.cap[2 * ms + 4], N.d_nodes, d_slots, toff(2 * ms + 4), nl,
                     S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
       HIPCHK(hipEventRecord(N.ev_rest, st2));
@@ -663,7 +667,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
       joined = true;
     };
     for (int stp = 0; stp < ms; ++stp) {
-      launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      if (stp > 0 || !split) launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
       launch_apply(st, lv.cap[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
       // one step (every n <= 64): the update stays inside the first tile
       // column too; with more steps it reaches the later columns
