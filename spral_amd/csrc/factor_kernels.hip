@@ -533,16 +533,18 @@ __global__ __launch_bounds__(kThreads) void k_apply(NodeDev *nodes, const int *l
   }
 }
 
-// number of block pivots that pass the threshold test (split 2x2 avoided)
+// number of block pivots that pass the threshold test (split 2x2 avoided):
+// the first column whose max |l| exceeds 1/u.  Lane j reads column j's
+// maximum, so the w loads are in flight together; every lane of the wave
+// must be active, and the result is wave-uniform.
 __device__ inline int block_npass(const NodeDev &N, double u) {
-  const int w = N.w, p = N.p;
+  const int w = N.w, p = N.p, lane = threadIdx.x & 63;
   const double lim = 1.0 / u;
   const unsigned long long lb = (unsigned long long)__double_as_longlong(lim);
-  int np = w;
-  for (int j = 0; j < w; ++j) {
-    unsigned long long b = __atomic_load_n(&N.colmax[j], __ATOMIC_RELAXED);
-    if (b > lb) { np = j; break; }
-  }
+  const unsigned long long b =
+      (lane < w) ? __atomic_load_n(&N.colmax[lane], __ATOMIC_RELAXED) : 0ull;
+  const unsigned long long fail = __ballot(lane < w && b > lb);
+  int np = fail ? __ffsll((long long)fail) - 1 : w;
   if (np > 0 && np < w) {
     double d11 = N.D[2 * (p + np - 1)], d21 = N.D[2 * (p + np - 1) + 1];
     if (isfinite(d11) && d21 != 0.0) np--;
@@ -1441,13 +1443,12 @@ __global__ __launch_bounds__(64 * kStatsWaves) void k_stats(const NodeDev *nodes
 // the capacity its buffers were planned with, or with a skipped child, is
 // skipped (m = n = 0, no work) and counted in *overflow when it is the first
 // on its path; the host then refactors with the per-level schedule.
-// Thread tid owns fronts tid, tid+1024, ... in both phases, so every front's
-// sizes are read back by the thread that wrote them.
+// Thread tid sizes fronts tid, tid+1024, ...; after a workgroup barrier each
+// wave scans its own share of the tables.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_level_setup(NodeDev *nodes, int s0, int nl,
                                                       const int *child_list, int pivot_tpp,
                                                       int S, int *tab, int *overflow) {
-  __shared__ int wsum[16];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (int li = tid; li < nl; li += 1024) {
     NodeDev &D = nodes[s0 + li];
@@ -1472,12 +1473,15 @@ __global__ __launch_bounds__(1024) void k_level_setup(NodeDev *nodes, int s0, in
       D.nsteps = pivot_tpp ? 0 : (D.n + kBlk - 1) / kBlk;
     }
   }
+  __syncthreads();  // the sizes above are visible to every wave
+  // wave wv scans tables wv, wv + 16, ... over 64-front chunks, the running
+  // sum in a register (no workgroup barrier per table)
   const int ntab = level_tables(S);
-  for (int t = 0; t < ntab; ++t) {
+  for (int t = wv; t < ntab; t += 16) {
     int *out = tab + (size_t)t * (nl + 1);
     int carry = 0;
-    for (int base = 0; base < nl; base += 1024) {
-      const int li = base + tid;
+    for (int base = 0; base < nl; base += 64) {
+      const int li = base + lane;
       int v = 0;
       if (li < nl) {
         const NodeDev &D = nodes[s0 + li];
@@ -1488,19 +1492,10 @@ __global__ __launch_bounds__(1024) void k_level_setup(NodeDev *nodes, int s0, in
         const int y = __shfl_up(x, o);
         if (lane >= o) x += y;
       }
-      if (lane == 63) wsum[wv] = x;
-      __syncthreads();
-      int wpre = 0, tot = 0;
-      for (int w = 0; w < 16; ++w) {
-        const int sw = wsum[w];
-        if (w < wv) wpre += sw;
-        tot += sw;
-      }
-      if (li < nl) out[li] = carry + wpre + x - v;
-      carry += tot;
-      __syncthreads();
+      if (li < nl) out[li] = carry + x - v;
+      carry += __shfl(x, 63);
     }
-    if (tid == 0) out[nl] = carry;
+    if (lane == 0) out[nl] = carry;
   }
 }
 
