@@ -882,6 +882,51 @@ __device__ __forceinline__ int gather_tile(
       const int b = tb + u * (kThreads / 64), k = kc0 + b;
       cv[u] = (rowok && b < ncol && kr >= k) ? Cr[(size_t)(k - cn) * cldc] : 0.0;
     }
+    // L21 (L21 D)^T on MFMA: each wave a 32x32 quadrant, operands in the
+    // 16x16x4 fragment layout (lane i+16k holds row i, pivot k) straight
+    // from L21 and the child's L21 D; two register buffers alternate so
+    // the loads of the next 16 pivots are in flight during the MFMAs
+    const int K = kb;
+    const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
+    const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
+    gcdouble *La = cL + rr0;
+    gcdouble *Lb = cLD + kc0;
+    double4_t g[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    constexpr int KC = 2;  // k-steps (of 4 pivots) per buffer
+    double xa0[KC], xa1[KC], xb0[KC], xb1[KC], ya0[KC], ya1[KC], yb0[KC], yb1[KC];
+    auto load = [&](int T0, double *A0, double *A1, double *B0, double *B1) {
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int t = T0 + 4 * u + kq;
+        const bool ok = t < K;
+        const size_t o = (size_t)t * cldl;
+        A0[u] = (ok && va0) ? La[o + ra0] : 0.0;
+        A1[u] = (ok && va1) ? La[o + ra1] : 0.0;
+        B0[u] = (ok && vb0) ? Lb[o + cb0] : 0.0;
+        B1[u] = (ok && vb1) ? Lb[o + cb1] : 0.0;
+      }
+    };
+    auto mfma = [&](int T0, const double *A0, const double *A1, const double *B0,
+                    const double *B1) {
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        if (T0 + 4 * u >= K) break;
+        g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B0[u], g[0][0], 0, 0, 0);
+        g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B1[u], g[0][1], 0, 0, 0);
+        g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B0[u], g[1][0], 0, 0, 0);
+        g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B1[u], g[1][1], 0, 0, 0);
+      }
+    };
+    // the first two operand buffers go out with the C_c loads, before the
+    // barrier (they depend on the tables only)
+    if (contrib) {
+      load(0, xa0, xa1, xb0, xb1);
+      load(4 * KC, ya0, ya1, yb0, yb1);
+    }
     __syncthreads();  // maps ready
     // failed (delayed) columns of the child: copy from its panel
     const int kf1 = min(k1, cn);
@@ -904,48 +949,8 @@ __device__ __forceinline__ int gather_tile(
           if (b < ncol && kr >= kc0 + b) acc[mra][mc[kc0 - k0 + b]] += cv[u];
         }
       }
-      // L21 (L21 D)^T on MFMA: each wave a 32x32 quadrant, operands in the
-      // 16x16x4 fragment layout (lane i+16k holds row i, pivot k) straight
-      // from L21 and the child's L21 D; two register buffers alternate so
-      // the loads of the next 16 pivots are in flight during the MFMAs
-      const int K = kb;
-      const int ra0 = wr + ri, ra1 = ra0 + 16, cb0 = wc + ri, cb1 = cb0 + 16;
-      const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
-      gcdouble *La = cL + rr0;
-      gcdouble *Lb = cLD + kc0;
-      double4_t g[2][2];
-#pragma unroll
-      for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      constexpr int KC = 2;  // k-steps (of 4 pivots) per buffer
-      double xa0[KC], xa1[KC], xb0[KC], xb1[KC], ya0[KC], ya1[KC], yb0[KC], yb1[KC];
-      auto load = [&](int T0, double *A0, double *A1, double *B0, double *B1) {
-#pragma unroll
-        for (int u = 0; u < KC; ++u) {
-          const int t = T0 + 4 * u + kq;
-          const bool ok = t < K;
-          const size_t o = (size_t)t * cldl;
-          A0[u] = (ok && va0) ? La[o + ra0] : 0.0;
-          A1[u] = (ok && va1) ? La[o + ra1] : 0.0;
-          B0[u] = (ok && vb0) ? Lb[o + cb0] : 0.0;
-          B1[u] = (ok && vb1) ? Lb[o + cb1] : 0.0;
-        }
-      };
-      auto mfma = [&](int T0, const double *A0, const double *A1, const double *B0,
-                      const double *B1) {
-#pragma unroll
-        for (int u = 0; u < KC; ++u) {
-          if (T0 + 4 * u >= K) break;
-          g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B0[u], g[0][0], 0, 0, 0);
-          g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B1[u], g[0][1], 0, 0, 0);
-          g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B0[u], g[1][0], 0, 0, 0);
-          g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B1[u], g[1][1], 0, 0, 0);
-        }
-      };
-      load(0, xa0, xa1, xb0, xb1);
       for (int t0 = 0; t0 < K; t0 += 8 * KC) {
-        load(t0 + 4 * KC, ya0, ya1, yb0, yb1);
+        if (t0 > 0) load(t0 + 4 * KC, ya0, ya1, yb0, yb1);
         mfma(t0, xa0, xa1, xb0, xb1);
         if (t0 + 4 * KC >= K) break;
         load(t0 + 8 * KC, xa0, xa1, xb0, xb1);

@@ -491,8 +491,11 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
                          int nsteps, Keep keep, std::pair<void *, size_t> &cb,
                          std::pair<void *, size_t> &wk) {
   const int i = S.node_of[sl];
-  D.ldl = std::max(m, 1);
-  D.ldc = D.sm - D.sn;
+  // columns start on 128-byte lines (16 doubles), so a 64-row tile column is
+  // four whole lines for the kernels that read and write fronts by tiles
+  auto lines = [](int r) { return (r + 15) & ~15; };
+  D.ldl = lines(std::max(m, 1));
+  D.ldc = (D.sm > D.sn) ? lines(D.sm - D.sn) : 0;
   D.ldbak = std::max(m, 1);
   D.L = (double *)keep((size_t)D.ldl * n * sizeof(double));
   D.pmap = (int *)keep(std::max(m, 1) * sizeof(int));
@@ -502,7 +505,8 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   D.perm0 = (int *)keep(std::max(n, 1) * sizeof(int));
   D.ipos = (int *)keep(std::max(n, 1) * sizeof(int));
   // contribution block and L21 D share one allocation (same lifetime)
-  const size_t cbytes = ((size_t)D.ldc * D.ldc * sizeof(double) + 255) & ~size_t(255);
+  const size_t cbytes =
+      ((size_t)D.ldc * (D.sm - D.sn) * sizeof(double) + 255) & ~size_t(255);
   const size_t ldbytes = D.is_root ? 0 : (size_t)D.ldl * n * sizeof(double);
   char *cbase = (cbytes + ldbytes) ? (char *)N.pool.alloc(cbytes + ldbytes) : nullptr;
   D.C = (D.ldc > 0) ? (double *)cbase : nullptr;
@@ -1530,8 +1534,8 @@ void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int 
   int cm = D.m - D.n;
   N->contrib_val.assign((size_t)cm * cm, 0.0);
   if (cm > 0) {
-    (void)hipMemcpy(N->contrib_val.data(), D.C, (size_t)cm * cm * sizeof(double),
-                    hipMemcpyDeviceToHost);
+    (void)hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
+                      D.ldc * sizeof(double), cm * sizeof(double), cm, hipMemcpyDeviceToHost);
     // symmetrise the lower triangle into the full square the reference passes
     for (int j = 0; j < cm; ++j)
       for (int i = 0; i < j; ++i)

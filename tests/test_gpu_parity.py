@@ -48,7 +48,7 @@ def node_factors(fk, node):
     perm = np.zeros(n, np.int32)
     L.spral_ssids_hip_node_factors(fk.numeric, node, info.ctypes.data, Lm.ctypes.data,
                                    D.ctypes.data, perm.ctypes.data)
-    return dict(m=m, n=n, nd=nd, nelim=ne, sn=sn, L=Lm.T, D=D, perm=perm)
+    return dict(m=m, n=n, nd=nd, nelim=ne, sn=sn, L=Lm.T[:m], D=D, perm=perm)
 
 
 @pytest.mark.parametrize("seed", range(8))
