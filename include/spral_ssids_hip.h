@@ -220,6 +220,29 @@ int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, doubl
  * src/hw_topology/guess_topology.cxx:65). */
 int spral_ssids_hip_device_count(void);
 
+/* ---------------------------------------------------------------------------
+ * Scaling.  Replaces equilib_scale_sym (src/scaling.f90:381-416; algorithm
+ * inf_norm_equilib_sym, :480-521), which ssids_factor runs for
+ * options%scaling >= 4 (src/ssids/ssids.f90:1003-1028): symmetric
+ * infinity-norm equilibration of the lower-triangle CSC (ptr 1-based, n+1
+ * entries; row 1-based), computed on the device.  scaling (n, original
+ * variable order) receives the result; max_iterations and tol are
+ * equilib_options (defaults 10 and 1e-8, tol a single-precision value as in
+ * the reference); *iterations receives equilib_inform%iterations.  Returns 0,
+ * -1 for an invalid argument or pattern (checked on the host before any
+ * launch), -2 when device memory runs out, -3 on a HIP error.  The result is
+ * bit-identical to the reference's.
+ * ------------------------------------------------------------------------- */
+int spral_ssids_hip_equilib_scale_sym(int n, const int64_t *ptr, const int *row,
+                                      const double *val, double *scaling, int max_iterations,
+                                      float tol, int *iterations);
+/* As spral_ssids_hip_equilib_scale_sym with ptr, row, val and scaling in
+ * device memory (the resident-pattern path).  The pattern is not checked on
+ * the host here; row indices outside [1, n] are skipped on the device. */
+int spral_ssids_hip_equilib_scale_sym_device(int n, const int64_t *d_ptr, const int *d_row,
+                                             const double *d_val, double *d_scaling,
+                                             int max_iterations, float tol, int *iterations);
+
 #ifdef __cplusplus
 }
 #endif

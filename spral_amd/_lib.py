@@ -108,6 +108,12 @@ SIGNATURES = {
     "spral_ssids_hip_device_count": (C.c_int, []),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
+    "spral_ssids_hip_equilib_scale_sym": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p, C.c_int,
+                                                    C.c_float, C.POINTER(C.c_int)]),
+    "spral_ssids_hip_equilib_scale_sym_device": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p,
+                                                           C.c_void_p, C.c_void_p, C.c_int,
+                                                           C.c_float, C.POINTER(C.c_int)]),
 }
 
 _lib = None

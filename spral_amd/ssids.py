@@ -226,6 +226,25 @@ def _inform_from_stats(st, akeep):
     return inf
 
 
+def equilib_scale_sym(n, ptr, row, val, max_iterations=10, tol=1e-8):
+    """Symmetric infinity-norm equilibration of the lower CSC (1-based ptr,
+    row) on the device, as equilib_scale_sym (src/scaling.f90:403-416).
+    Returns (scaling, iterations); tol is used at single precision, as the
+    reference's equilib_options%tol."""
+    L = _lib.lib()
+    ptr = _as_i64(ptr)
+    row = _as_i32(row)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    s = np.zeros(max(n, 1))
+    its = C.c_int(0)
+    rc = L.spral_ssids_hip_equilib_scale_sym(n, ptr.ctypes.data, row.ctypes.data,
+                                             val.ctypes.data, s.ctypes.data,
+                                             int(max_iterations), float(tol), C.byref(its))
+    if rc != 0:
+        raise RuntimeError(f"spral_ssids_hip_equilib_scale_sym failed ({rc})")
+    return s[:n], its.value
+
+
 def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
     """Numeric factorization on the GPU; returns (fkeep, inform)."""
     options = options or SSIDSOptions()
@@ -238,6 +257,17 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
     opts = options.to_c()
     sc = None
     scp = None
+    if options.scaling >= 4:
+        # norm equilibration (MC77-like), as src/ssids/ssids.f90:1003-1028:
+        # computed on the device from the values, returned through `scale`
+        s = equilib_scale_sym(akeep.n, akeep.ptr, akeep.row, val)[0]
+        if scale is not None:
+            scale[:] = s
+        scale = s
+    elif options.scaling in (1, 2, 3):
+        raise NotImplementedError(
+            f"options.scaling={options.scaling} (matching-based) is not built; use 0 with "
+            "a user scaling or >= 4 (equilibration)")
     if scale is not None:
         # scaling is applied in elimination order inside the subtree
         sc = np.ascontiguousarray(np.asarray(scale, np.float64)[akeep.invp])
