@@ -198,6 +198,9 @@ struct FactorParams {
   int tpp_failed;   // failed_pivot_method == TPP
   int pivot_tpp;    // pivot_method == TPP (whole panel with TPP)
   int posdef;       // A = L L^T: pivots in order, every one 1x1 and > 0
+  int c32;          // contribution blocks held in FP32, the children's Schur
+                    // updates on FP32 MFMA (mixed precision; the panel, the
+                    // pivot blocks and their tests stay FP64)
 };
 
 }  // namespace ssids_hip

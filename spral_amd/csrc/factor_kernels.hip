@@ -27,12 +27,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "dev.hpp"
 
 namespace ssids_hip {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // front of workgroup b in a level table, or -1 past the table's total (grids
 // sized for the plan's capacities may be larger than the level needs)
@@ -80,7 +82,7 @@ constexpr int kDiagWaves = 4;
 
 constexpr int kGT = 64;  // gather tile edge
 
-template <int PART>
+template <int PART, typename CT>
 __device__ __forceinline__ int gather_tile(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
@@ -103,7 +105,8 @@ struct GatherArgs {  // the level's gather inputs (a pivot launch that assembles
 // then runs step 0's pivot block on it, as soon as its block is assembled
 // and in a workgroup that already holds its CU, instead of queueing behind
 // the rest of the level's gather.
-template <int NW, bool FIRST>
+// C32 (FIRST only): the children's contribution blocks are FP32
+template <int NW, bool FIRST, bool C32>
 __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl_slots,
                                                   int step, FactorParams fp, GatherArgs ga) {
   constexpr int R = kBlk / NW;
@@ -119,8 +122,9 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
     static_assert(64 * NW == kThreads, "one workgroup shape for both phases");
     static_assert(kGT == kBlk, "the gather tile is the pivot block's staging array");
     // the tile's LDS is the pivot block's own (t, lp, inv_lp), used in turn
-    s = gather_tile<1>(nodes, lvl_slots, ga.wg_off, ga.nlvl, ga.child_list, ga.aval, ga.asrc,
-                       ga.arow, ga.acix, ga.scaling, t, lp, inv_lp);
+    typedef typename std::conditional<C32, float, double>::type CT;
+    s = gather_tile<1, CT>(nodes, lvl_slots, ga.wg_off, ga.nlvl, ga.child_list, ga.aval,
+                           ga.asrc, ga.arow, ga.acix, ga.scaling, t, lp, inv_lp);
     if (s < 0) return;
     __syncthreads();  // the block's stores are visible to the whole workgroup
   } else {
@@ -643,6 +647,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_update(NodeDev *nodes, const in
   // descriptor fields once (the stores below could alias them)
   gdouble *NL = (gdouble *)N.L;
   gdouble *NC = (gdouble *)N.C;
+  float __attribute__((address_space(1))) *NCf = (float __attribute__((address_space(1))) *)N.C;
+  const bool c32 = fp.c32;  // the contribution block (mode 1 only) is FP32
   gcdouble *dbak = (gcdouble *)N.dbak;
   gcdouble *bak = (gcdouble *)N.bak;
   const int *lpv = N.lp;
@@ -708,7 +714,11 @@ __global__ __launch_bounds__(kThreads, 4) void k_update(NodeDev *nodes, const in
           int i = wr + bi * 16 + (lane >> 4) + 4 * rg;
           int j = wc + bj * 16 + (lane & 15);
           int gi = r0 + i, gj = c0 + j;
-          if (gi < m && gj < cend && gi >= gj) {
+          if (gi < m && gj < cend && gi >= gj && gj >= nfs && c32) {
+            float __attribute__((address_space(1))) *dc =
+                NCf + (size_t)(gj - nfs) * ldc + (gi - nfs);
+            *dc = (float)((double)*dc - acc[bi][bj][rg]);
+          } else if (gi < m && gj < cend && gi >= gj) {
             gdouble *dst = (gj < nfs) ? NL + (size_t)gj * ldl + gi
                                       : NC + (size_t)(gj - nfs) * ldc + (gi - nfs);
             double old;
@@ -776,12 +786,15 @@ __global__ __launch_bounds__(kThreads) void k_maptab(NodeDev *nodes, const int *
 // Returns the front's slot when this workgroup built the front's first
 // diagonal tile (0, 0), else -1.
 
-template <int PART>
+// CT: element type of the contribution blocks (double, or float for the
+// mixed-precision factorization, whose Schur terms then run on FP32 MFMA)
+template <int PART, typename CT>
 __device__ __forceinline__ int gather_tile(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
     const int *acix, const double *scaling, double (&acc)[kGT][kGT + 1], int (&mr)[kGT],
     int (&mc)[kGT]) {
+  constexpr bool F32 = sizeof(CT) == 4;
   const int li = find_node(wg_off, nlvl, blockIdx.x);
   if (li < 0) return -1;
   const int s = lvl_slots[li];
@@ -790,7 +803,7 @@ __device__ __forceinline__ int gather_tile(
   const int m = P.m, n = P.n, nd = P.nd, pldl = P.ldl, pldc = P.ldc;
   const int cbeg = P.cbeg, cend = P.cend;
   gdouble *PL = (gdouble *)P.L;
-  gdouble *PC = (gdouble *)P.C;
+  CT __attribute__((address_space(1))) *PC = (CT __attribute__((address_space(1))) *)P.C;
   gcint *pctab = (gcint *)P.ctab;
   gcint *prows = (gcint *)P.rows;
   const int64_t *pacol = P.acol;
@@ -855,7 +868,8 @@ __device__ __forceinline__ int gather_tile(
     const int cn = c.n, cm = c.m, kb = c.nelim, cldl = c.ldl, cldc = c.ldc;
     gcdouble *cL = (gcdouble *)c.L;
     gcdouble *cLD = (gcdouble *)c.LD;
-    gcdouble *cC = (gcdouble *)c.C;
+    const CT __attribute__((address_space(1))) *cC =
+        (const CT __attribute__((address_space(1))) *)c.C;
     gcint *cpmap = (gcint *)c.pmap;
     if (cm <= kb) continue;
     gcint *tab = pctab + (size_t)(ci - cbeg) * (T + 1);
@@ -876,11 +890,11 @@ __device__ __forceinline__ int gather_tile(
     double cv[NU];
     const bool rowok = contrib && ta < nrow;
     const int kr = rr0 + ta;
-    gcdouble *Cr = cC + (kr - cn);
+    const CT __attribute__((address_space(1))) *Cr = cC + (kr - cn);
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       const int b = tb + u * (kThreads / 64), k = kc0 + b;
-      cv[u] = (rowok && b < ncol && kr >= k) ? Cr[(size_t)(k - cn) * cldc] : 0.0;
+      cv[u] = (rowok && b < ncol && kr >= k) ? (double)Cr[(size_t)(k - cn) * cldc] : 0.0;
     }
     // L21 (L21 D)^T on MFMA: each wave a 32x32 quadrant, operands in the
     // 16x16x4 fragment layout (lane i+16k holds row i, pivot k) straight
@@ -891,11 +905,15 @@ __device__ __forceinline__ int gather_tile(
     const bool va0 = ra0 < nrow, va1 = ra1 < nrow, vb0 = cb0 < ncol, vb1 = cb1 < ncol;
     gcdouble *La = cL + rr0;
     gcdouble *Lb = cLD + kc0;
-    double4_t g[2][2];
+    // FP64: v_mfma_f64_16x16x4_f64, D[i][j] in lane j + 16 (i % 4), register
+    // i / 4.  Mixed precision: operands rounded to FP32, v_mfma_f32_16x16x4_f32
+    // with FP32 accumulation, D[i][j] in lane j + 16 (i / 4), register i % 4
+    typedef typename std::conditional<F32, float4_t, double4_t>::type acc4_t;
+    acc4_t g[2][2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      for (int bj = 0; bj < 2; ++bj) g[bi][bj] = (acc4_t){0, 0, 0, 0};
     constexpr int KC = 2;  // k-steps (of 4 pivots) per buffer
     double xa0[KC], xa1[KC], xb0[KC], xb1[KC], ya0[KC], ya1[KC], yb0[KC], yb1[KC];
     auto load = [&](int T0, double *A0, double *A1, double *B0, double *B1) {
@@ -915,10 +933,19 @@ __device__ __forceinline__ int gather_tile(
 #pragma unroll
       for (int u = 0; u < KC; ++u) {
         if (T0 + 4 * u >= K) break;
-        g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B0[u], g[0][0], 0, 0, 0);
-        g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B1[u], g[0][1], 0, 0, 0);
-        g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B0[u], g[1][0], 0, 0, 0);
-        g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B1[u], g[1][1], 0, 0, 0);
+        if constexpr (F32) {
+          const float a0 = (float)A0[u], a1 = (float)A1[u], b0 = (float)B0[u],
+                      b1 = (float)B1[u];
+          g[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, g[0][0], 0, 0, 0);
+          g[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, g[0][1], 0, 0, 0);
+          g[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, g[1][0], 0, 0, 0);
+          g[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, g[1][1], 0, 0, 0);
+        } else {
+          g[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B0[u], g[0][0], 0, 0, 0);
+          g[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[u], B1[u], g[0][1], 0, 0, 0);
+          g[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B0[u], g[1][0], 0, 0, 0);
+          g[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[u], B1[u], g[1][1], 0, 0, 0);
+        }
       }
     };
     // the first two operand buffers go out with the C_c loads, before the
@@ -963,11 +990,11 @@ __device__ __forceinline__ int gather_tile(
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
           for (int rg = 0; rg < 4; ++rg) {
-            const int i = wr + bi * 16 + kq + 4 * rg;
+            const int i = wr + bi * 16 + (F32 ? 4 * kq + rg : kq + 4 * rg);
             const int j = wc + bj * 16 + ri;
             if (i < nrow && j < ncol) {
               const int krr = rr0 + i, k = kc0 + j;
-              if (krr >= k) acc[mr[krr - kr0]][mc[k - k0]] -= g[bi][bj][rg];
+              if (krr >= k) acc[mr[krr - kr0]][mc[k - k0]] -= (double)g[bi][bj][rg];
             }
           }
     }
@@ -998,21 +1025,21 @@ __device__ __forceinline__ int gather_tile(
     const int gr = r0 + i, gc = c0 + j;
     if (gr < m && gc < m && gr >= gc) {
       if (gc < n) PL[(size_t)gc * pldl + gr] = acc[i][j];
-      else PC[(size_t)(gc - n) * pldc + (gr - n)] = acc[i][j];
+      else PC[(size_t)(gc - n) * pldc + (gr - n)] = (CT)acc[i][j];
     }
   }
   return (I == 0 && J == 0) ? s : -1;
 }
 
-template <int PART>
+template <int PART, typename CT>
 __global__ __launch_bounds__(kThreads, 4) void k_gather(
     NodeDev *nodes, const int *lvl_slots, const int *wg_off, int nlvl,
     const int *child_list, const double *aval, const int64_t *asrc, const int *arow,
     const int *acix, const double *scaling) {
   __shared__ double acc[kGT][kGT + 1];
   __shared__ int mr[kGT], mc[kGT];
-  gather_tile<PART>(nodes, lvl_slots, wg_off, nlvl, child_list, aval, asrc, arow, acix,
-                    scaling, acc, mr, mc);
+  gather_tile<PART, CT>(nodes, lvl_slots, wg_off, nlvl, child_list, aval, asrc, arow, acix,
+                        scaling, acc, mr, mc);
 }
 
 
@@ -1538,16 +1565,25 @@ void launch_gather_first(hipStream_t st, int nwg, NodeDev *nodes, const int *slo
                          const double *aval, const int64_t *asrc, const int *arow,
                          const int *acix, const double *scaling, FactorParams fp) {
   const GatherArgs ga{wg_off, nlvl, child_list, aval, asrc, arow, acix, scaling};
-  if (nwg > 0)
-    hipLaunchKernelGGL((k_diag<kDiagWaves, true>), dim3(nwg), dim3(kThreads), 0, st, nodes,
-                       slots, 0, fp, ga);
+  if (nwg <= 0) return;
+  if (fp.c32)
+    hipLaunchKernelGGL((k_diag<kDiagWaves, true, true>), dim3(nwg), dim3(kThreads), 0, st,
+                       nodes, slots, 0, fp, ga);
+  else
+    hipLaunchKernelGGL((k_diag<kDiagWaves, true, false>), dim3(nwg), dim3(kThreads), 0, st,
+                       nodes, slots, 0, fp, ga);
 }
 void launch_gather(hipStream_t st, int part, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, const int *child_list, const double *aval,
                    const int64_t *asrc, const int *arow, const int *acix,
-                   const double *scaling) {
+                   const double *scaling, int c32) {
   if (nwg <= 0) return;
-  auto kern = (part == 1) ? k_gather<1> : (part == 2) ? k_gather<2> : k_gather<0>;
+  auto kern = c32 ? ((part == 1)   ? k_gather<1, float>
+                     : (part == 2) ? k_gather<2, float>
+                                   : k_gather<0, float>)
+                  : ((part == 1)   ? k_gather<1, double>
+                     : (part == 2) ? k_gather<2, double>
+                                   : k_gather<0, double>);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,
                      child_list, aval, asrc, arow, acix, scaling);
 }
@@ -1565,7 +1601,8 @@ void launch_tail(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, Fac
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
   if (nlvl > 0)
-    hipLaunchKernelGGL((k_diag<kDiagWaves, false>), dim3(nlvl), dim3(64 * kDiagWaves), 0, st,
+    hipLaunchKernelGGL((k_diag<kDiagWaves, false, false>), dim3(nlvl), dim3(64 * kDiagWaves),
+                       0, st,
                        nodes, slots, step, fp, GatherArgs{});
 }
 void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,

@@ -24,7 +24,8 @@ namespace ssids_hip {
 
 // kernel launchers (factor_kernels.hip / solve_kernels.hip)
 void launch_gather(hipStream_t, int, int, NodeDev *, const int *, const int *, int, const int *,
-                   const double *, const int64_t *, const int *, const int *, const double *);
+                   const double *, const int64_t *, const int *, const int *, const double *,
+                   int);
 void launch_gather_first(hipStream_t, int, NodeDev *, const int *, const int *, int,
                          const int *, const double *, const int64_t *, const int *,
                          const int *, const double *, FactorParams);
@@ -192,6 +193,7 @@ struct Symbolic {
 struct Plan {
   bool valid = false;
   int pivot_tpp = -1;
+  int c32 = -1;                            // contribution-block precision planned for
   std::vector<NodeDev> tmpl;               // slot-indexed static fields
   NodeDev *d_tmpl = nullptr;
   struct Level {
@@ -214,6 +216,7 @@ struct Plan {
 
 struct Numeric {
   Symbolic *symb = nullptr;
+  int c32 = 0;  // contribution blocks in FP32 (schur_precision 1, mixed precision)
   hipStream_t stream = nullptr;
   // second stream of the static plan: a level's gather beyond its first tile
   // column runs there while the first pivot step runs on `stream`
@@ -505,8 +508,8 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   D.perm0 = (int *)keep(std::max(n, 1) * sizeof(int));
   D.ipos = (int *)keep(std::max(n, 1) * sizeof(int));
   // contribution block and L21 D share one allocation (same lifetime)
-  const size_t cbytes =
-      ((size_t)D.ldc * (D.sm - D.sn) * sizeof(double) + 255) & ~size_t(255);
+  const size_t cesz = N.c32 ? sizeof(float) : sizeof(double);
+  const size_t cbytes = ((size_t)D.ldc * (D.sm - D.sn) * cesz + 255) & ~size_t(255);
   const size_t ldbytes = D.is_root ? 0 : (size_t)D.ldl * n * sizeof(double);
   char *cbase = (cbytes + ldbytes) ? (char *)N.pool.alloc(cbytes + ldbytes) : nullptr;
   D.C = (D.ldc > 0) ? (double *)cbase : nullptr;
@@ -546,6 +549,7 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
   if ((int)N.nd_seen.size() != nn) N.nd_seen.assign(nn, 0);
   const double slack = std::max(0.0, multiplier - 1.0);
   P.pivot_tpp = pivot_tpp;
+  P.c32 = N.c32;
   P.tmpl.assign(nn, NodeDev{});
   P.lv.assign(nlev, Plan::Level{});
   std::vector<std::pair<void *, size_t>> cof(nn), wof(nn);
@@ -657,13 +661,13 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
       launch_gather_first(st, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
                           S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp);
       launch_gather(st2, 2, lv.cap[2 * ms + 4], N.d_nodes, d_slots, toff(2 * ms + 4), nl,
-                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
       HIPCHK(hipEventRecord(N.ev_rest, st2));
       joined = false;
     } else {
       if (opt.timing) tev.emplace_back(timed_event(st));
       launch_gather(st, 0, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
-                    S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+                    S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
       if (opt.timing) tev.emplace_back(timed_event(st));
     }
     auto join = [&]() {
@@ -779,7 +783,7 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
     if (opt.timing) tev.emplace_back(timed_event(st));
     launch_gather(st, 0, tot[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
-                  S.d_asrc, S.d_arow, S.d_acix, N.d_scal);
+                  S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
     if (opt.timing) tev.emplace_back(timed_event(st));
     for (int stp = 0; stp < ms; ++stp) {
       launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
@@ -839,6 +843,12 @@ static bool plan_disabled() {
 static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
                         const spral_ssids_hip_options &opt, int64_t naval) {
   Symbolic &S = *N.symb;
+  // Schur-update precision: 0 FP64, 1 FP32 (mixed); FP16 is not built
+  if (opt.schur_precision < 0 || opt.schur_precision > 1) {
+    N.stats.flag = -98;  // SSIDS_ERROR_UNIMPLEMENTED
+    return;
+  }
+  N.c32 = (opt.schur_precision == 1);
   // keep the values (and scaling) for refinement residuals
   if ((size_t)naval > N.d_aval_cap) {
     if (N.d_aval) HIPCHK(hipFree(N.d_aval));
@@ -866,6 +876,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   fp.tpp_failed = (opt.failed_pivot_method != 2);
   fp.pivot_tpp = (opt.pivot_method == 3) && !N.posdef;
   fp.posdef = N.posdef;
+  fp.c32 = N.c32;
   if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
   if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
   if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
@@ -884,7 +895,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   HIPCHK(hipEventRecord(ev0, st));
   bool done = false;
   if (!plan_disabled()) {
-    if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp)
+    if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp || N.plan.c32 != N.c32)
       plan_build(N, fp.pivot_tpp, opt.multiplier);
     done = factor_static(N, fp, opt, tev);
     if (!done) {
@@ -1534,8 +1545,16 @@ void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int 
   int cm = D.m - D.n;
   N->contrib_val.assign((size_t)cm * cm, 0.0);
   if (cm > 0) {
-    (void)hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
-                      D.ldc * sizeof(double), cm * sizeof(double), cm, hipMemcpyDeviceToHost);
+    if (N->c32) {  // mixed precision: the block is held in FP32
+      std::vector<float> f((size_t)cm * cm);
+      (void)hipMemcpy2D(f.data(), cm * sizeof(float), D.C, D.ldc * sizeof(float),
+                        cm * sizeof(float), cm, hipMemcpyDeviceToHost);
+      for (size_t k = 0; k < f.size(); ++k) N->contrib_val[k] = f[k];
+    } else {
+      (void)hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
+                        D.ldc * sizeof(double), cm * sizeof(double), cm,
+                        hipMemcpyDeviceToHost);
+    }
     // symmetrise the lower triangle into the full square the reference passes
     for (int j = 0; j < cm; ++j)
       for (int i = 0; i < j; ++i)

@@ -1,6 +1,7 @@
 // Test-only probes of compiler builtins the kernels rely on (in-bounds: one
-// wave of 64 lanes, fixed-size buffers): the v_mfma_f64_16x16x4_f64
-// operand/result layout (16x4 A, 4x16 B, 16x16 D) and the DPP wave maximum
+// wave of 64 lanes, fixed-size buffers): the v_mfma_f64_16x16x4_f64 and
+// v_mfma_f32_16x16x4_f32 operand/result layouts (16x4 A, 4x16 B, 16x16 D)
+// and the DPP wave maximum
 // of the diagonal-block pivot search.  Built into tests/hip/libprobe.so;
 // never linked by the product library.
 #include <hip/hip_runtime.h>
@@ -27,6 +28,32 @@ extern "C" int probe_mfma_f64(const double *hA, const double *hB, double *hout) 
   (void)hipMemcpy(B, hB, 64 * sizeof(double), hipMemcpyHostToDevice);
   hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, 0, A, B, O);
   hipError_t e = hipMemcpy(hout, O, 256 * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  (void)hipFree(O);
+  return e == hipSuccess ? 0 : -2;
+}
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ void k_probe_f32(const float *A, const float *B, float *out) {
+  const int lane = threadIdx.x;  // 64 lanes
+  float a = A[(lane & 15) * 4 + (lane >> 4)];
+  float b = B[(lane >> 4) * 16 + (lane & 15)];
+  float4_t acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  for (int r = 0; r < 4; ++r) out[lane * 4 + r] = acc[r];  // raw lane/register dump
+}
+
+extern "C" int probe_mfma_f32(const float *hA, const float *hB, float *hout) {
+  float *A, *B, *O;
+  if (hipMalloc(&A, 64 * sizeof(float)) != hipSuccess) return -1;
+  if (hipMalloc(&B, 64 * sizeof(float)) != hipSuccess) return -1;
+  if (hipMalloc(&O, 256 * sizeof(float)) != hipSuccess) return -1;
+  (void)hipMemcpy(A, hA, 64 * sizeof(float), hipMemcpyHostToDevice);
+  (void)hipMemcpy(B, hB, 64 * sizeof(float), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_probe_f32, dim3(1), dim3(64), 0, 0, A, B, O);
+  hipError_t e = hipMemcpy(hout, O, 256 * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(A);
   (void)hipFree(B);
   (void)hipFree(O);

@@ -18,6 +18,7 @@ def probe(lib):
         build.build_probe()
     h = C.CDLL(str(PROBE))
     h.probe_mfma_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    h.probe_mfma_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     h.probe_wave_max_u64.argtypes = [C.c_void_p, C.c_void_p]
     h.probe_table_find.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     return h
@@ -37,6 +38,29 @@ def test_mfma_f64_16x16x4_layout(probe):
         for j in range(16):
             got[i, j] = out[(j + 16 * (i % 4)) * 4 + i // 4]
     assert np.allclose(got, D, rtol=1e-14, atol=1e-14)
+
+
+def test_mfma_f32_16x16x4_layout(probe):
+    """Same operand layout as the FP64 instruction, but D[i][j] in lane
+    j+16(i/4), register i%4 -- the layout of the mixed-precision gather."""
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((16, 4)).astype(np.float32)
+    B = rng.standard_normal((4, 16)).astype(np.float32)
+    out = np.zeros(256, np.float32)
+    assert probe.probe_mfma_f32(A.ctypes.data, B.ctypes.data, out.ctypes.data) == 0
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    got = np.empty((16, 16))
+    for i in range(16):
+        for j in range(16):
+            got[i, j] = out[(j + 16 * (i // 4)) * 4 + i % 4]
+    if not np.allclose(got, D, rtol=1e-5, atol=1e-5):
+        # name the layout the hardware has, for the message
+        where = {}
+        for lane in range(64):
+            for r in range(4):
+                hit = np.argwhere(np.isclose(D, out[lane * 4 + r], rtol=1e-5, atol=1e-6))
+                where[(lane, r)] = [tuple(h) for h in hit[:2]]
+        pytest.fail(f"f32 MFMA layout differs: {dict(list(where.items())[:12])}")
 
 
 @pytest.mark.parametrize("seed", range(6))
