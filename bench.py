@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--cpu-grid", type=int, default=38,
                     help="grid of the bounded CPU-baseline sample (oracle)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -107,8 +109,9 @@ def main():
                                           m.val.size, None, C.byref(copt), C.byref(st))
     assert num, f"factor failed flag={st.flag}"
 
-    def refactor(timing=0):
+    def refactor(timing=0, precision=0):
         copt.timing = timing
+        copt.schur_precision = precision
         s = _lib.Stats()
         f = L.spral_ssids_hip_refactor_device(num, d_val.ptr, None,
                                               C.byref(copt), C.byref(s))
@@ -118,10 +121,10 @@ def main():
     bwd_dev = np.zeros(1)
     its = C.c_int(0)
 
-    def solve():
+    def solve(maxit=3):
         # FP64 solve with refinement to the 1e-14 backward-error bar
         f = L.spral_ssids_hip_solve_refine_device(num, 1, b2.ptr,
-                                                  d_x.ptr, m.n, 3, 1e-14,
+                                                  d_x.ptr, m.n, maxit, 1e-14,
                                                   bwd_dev, C.byref(its), None)
         assert f == 0, f
 
@@ -183,6 +186,36 @@ def main():
                  "flops_per_launch": sp.kernel_flops / max(1, sp.kernel_launches),
                  "intensity_flop_per_byte": round(intensity, 3),
                  "achieved_tflops": round(tflops, 3), "achieved_gbs": round(gbs, 1)})
+    # the same steps on the mixed-precision factorization (FP32 contribution
+    # blocks and Schur updates, FP64 pivots), solved with FP64 refinement to
+    # the same 1e-14 bar
+    mixed = None
+    if not args.no_mixed:
+        mfac, mlat = [], []
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            sm = refactor(precision=1)
+            sync()
+            if i >= args.warmup:
+                mfac.append((time.perf_counter() - t0) * 1e3)
+            solve(maxit=5)
+            sync()
+        for _ in range(args.solves):
+            t0 = time.perf_counter()
+            solve(maxit=5)
+            sync()
+            mlat.append((time.perf_counter() - t0) * 1e3)
+        xm = d_x.download()
+        xmo = np.empty_like(xm)
+        xmo[ak.invp] = xm
+        mf = float(np.mean(mfac))
+        mixed = {"schur_precision": "fp32", "factor_ms": round(mf, 3),
+                 "value": round(flops / (mf * 1e-3) / 1e9, 2),
+                 "solve_ms_p50": round(float(np.percentile(mlat, 50)), 3),
+                 "solve_ms_p99": round(float(np.percentile(mlat, 99)), 3),
+                 "bwd_err": float(matrices.bwd_error(m.full, xmo, b)[0]),
+                 "refine_steps": its.value, "num_neg": sm.num_neg,
+                 "num_delay": sm.num_delay}
     fac_mean = float(np.mean(fac_ms))
     value = flops / (fac_mean * 1e-3) / 1e9
     out = {
@@ -212,6 +245,10 @@ def main():
         "num_two": s.num_two,
         "roofline": roof,
     }
+    if mixed is not None:
+        mixed["factor_plus_solve_ms"] = round(mixed["factor_ms"] + mixed["solve_ms_p50"], 3)
+        mixed["fp64_factor_plus_solve_ms"] = round(fac_mean + out["solve_ms_p50"], 3)
+        out["mixed"] = mixed
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_grid)
     L.spral_ssids_hip_destroy_numeric(num)
