@@ -104,6 +104,10 @@ void spral_ssids_hip_analyse_free(void *akeep);
 void spral_ssids_hip_order_grid_nd(int nx, int ny, int nz, int *order);
 int spral_ssids_hip_order_min_degree(int n, const int64_t *ptr, const int *row,
                                      int *order);
+/* Approximate minimum degree (Amestoy, Davis, Duff 1996) on the quotient
+ * graph, for large general patterns: lower CSC (1-based ptr, row) in,
+ * order(i) = pivot position of i (1-based) out.  Returns 0. */
+int spral_ssids_hip_order_amd(int n, const int64_t *ptr, const int *row, int *order);
 
 /* ---------------------------------------------------------------------------
  * Symbolic subtree: replaces construct_gpu_symbolic_subtree

@@ -72,6 +72,7 @@ SIGNATURES = {
     "spral_ssids_hip_analyse_free": (None, [C.c_void_p]),
     "spral_ssids_hip_order_grid_nd": (None, [C.c_int, C.c_int, C.c_int, i32p]),
     "spral_ssids_hip_order_min_degree": (C.c_int, [C.c_int, i64p, i32p, i32p]),
+    "spral_ssids_hip_order_amd": (C.c_int, [C.c_int, i64p, i32p, i32p]),
     "spral_ssids_hip_create_symbolic": (C.c_void_p, [
         C.c_int, C.c_int, C.c_int, C.c_int, i32p, i32p, i64p, i32p, i64p, i64p,
         C.POINTER(Options), C.POINTER(C.c_int)]),

@@ -105,4 +105,8 @@ int spral_ssids_hip_order_min_degree(int n, const int64_t *ptr, const int *row,
   return ssids_hip::order_min_degree(n, ptr, row, order);
 }
 
+int spral_ssids_hip_order_amd(int n, const int64_t *ptr, const int *row, int *order) {
+  return ssids_hip::order_amd(n, ptr, row, order);
+}
+
 }  // extern "C"

@@ -31,5 +31,6 @@ void expand_pattern(int n, const int64_t* ptr, const int* row,
 // Orderings (1-based order(i) = pivot position of variable i).
 void order_grid_nd(int nx, int ny, int nz, int* order);
 int order_min_degree(int n, const int64_t* ptr, const int* row, int* order);
+int order_amd(int n, const int64_t* ptr, const int* row, int* order);
 
 }  // namespace ssids_hip

@@ -10,9 +10,10 @@ Names, argument meaning and flag values follow the reference:
 The numeric work runs on the GPU through libspral_ssids_hip.so; there is no
 CPU fallback.  Matrices are lower-triangle CSC with 1-based ptr/row as in the
 reference.  Orderings: options.ordering = 0 takes the user's order (as the
-reference); 1 (the reference's METIS) uses this package's minimum degree since
-METIS is not available; 3 is geometric nested dissection for an nx*ny*nz grid
-(pass grid=(nx,ny,nz)).
+reference); 1 (the reference's METIS) uses this package's exact minimum degree
+up to n = AMD_THRESHOLD and its approximate minimum degree above, since METIS is
+not available; 3 is geometric nested dissection for an nx*ny*nz grid (pass
+grid=(nx,ny,nz)); 4 is the approximate minimum degree at any size.
 """
 import ctypes as C
 from dataclasses import dataclass, field
@@ -133,6 +134,11 @@ def _as_i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+# METIS stand-ins (options.ordering = 1): exact minimum degree up to this
+# order, approximate minimum degree above it (or always, with ordering = 4)
+AMD_THRESHOLD = 20000
+
+
 def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None):
     """Analyse the sparsity pattern; returns (akeep, inform)."""
     options = options or SSIDSOptions()
@@ -156,6 +162,11 @@ def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None):
         nx, ny, nz = grid
         ordv = np.zeros(n, np.int32)
         L.spral_ssids_hip_order_grid_nd(nx, ny, nz, ordv)
+    elif options.ordering == 4 or n > AMD_THRESHOLD:
+        # approximate minimum degree: the exact one's explicit graph grows
+        # with the fill
+        ordv = np.zeros(n, np.int32)
+        L.spral_ssids_hip_order_amd(n, ptr, row, ordv)
     else:
         ordv = np.zeros(n, np.int32)
         L.spral_ssids_hip_order_min_degree(n, ptr, row, ordv)
