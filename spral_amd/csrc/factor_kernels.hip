@@ -1556,6 +1556,25 @@ void launch_level_setup(hipStream_t st, NodeDev *nodes, int s0, int nl, const in
     hipLaunchKernelGGL(k_level_setup, dim3(1), dim3(1024), 0, st, nodes, s0, nl, child_list,
                        pivot_tpp, S, tab, overflow);
 }
+// D^{-1} (two per pivot) and the eliminated variables of every front, into
+// contiguous arrays in node order: node i's pivots at [off[i], off[i+1]),
+// with off from the host's read-back of each front's nelim
+__global__ void k_pack_dperm(const NodeDev *nodes, const int *slot_of, const int64_t *off,
+                             int nn, double *out_d, int *out_p) {
+  for (int i = blockIdx.x; i < nn; i += gridDim.x) {
+    const NodeDev &D = nodes[slot_of[i]];
+    const int64_t o = off[i];
+    const int ne = (int)(off[i + 1] - o);
+    for (int k = threadIdx.x; k < 2 * ne; k += blockDim.x) out_d[2 * o + k] = D.D[k];
+    for (int k = threadIdx.x; k < ne; k += blockDim.x) out_p[o + k] = D.perm[k];
+  }
+}
+void launch_pack_dperm(hipStream_t st, const NodeDev *nodes, const int *slot_of,
+                       const int64_t *off, int nn, double *out_d, int *out_p) {
+  if (nn > 0)
+    hipLaunchKernelGGL(k_pack_dperm, dim3(std::min(nn, 4096)), dim3(256), 0, st, nodes,
+                       slot_of, off, nn, out_d, out_p);
+}
 void launch_pack_nodes(hipStream_t st, const NodeDev *nodes, int nn, int *out) {
   if (nn > 0)
     hipLaunchKernelGGL(k_pack_nodes, dim3((nn + 255) / 256), dim3(256), 0, st, nodes, nn, out);

@@ -40,6 +40,8 @@ void launch_tail(hipStream_t, int, NodeDev *, const int *, FactorParams, int *);
 void launch_stats(hipStream_t, const NodeDev *, int, int *);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
+void launch_pack_dperm(hipStream_t, const NodeDev *, const int *, const int64_t *, int, double *,
+                       int *);
 void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
                     int, int);
 void launch_fwd_rows(hipStream_t, int, const SolveNode *, const int *, const int *, int,
@@ -1452,17 +1454,54 @@ int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d
       for (int i = 0; i < 2 * S.n; ++i) d[i] = 0.0;
     int piv = 0;
     double *dp = d;
+    // every front's D^{-1} and eliminated variables, packed on the device and
+    // copied back once (not two copies per front)
+    const int nn = S.nnodes;
+    std::vector<int64_t> off(nn + 1, 0);
+    std::vector<int> slot_of(std::max(nn, 1));
+    for (int i = 0; i < nn; ++i) {
+      slot_of[i] = S.slot[i];
+      off[i + 1] = off[i] + std::max(0, N->nodes[S.slot[i]].nelim);
+    }
+    const int64_t tot = off[nn];
+    std::vector<double> dall(2 * (size_t)std::max<int64_t>(tot, 1));
+    std::vector<int> pall(std::max<int64_t>(tot, 1));
+    if (tot > 0) {
+      int *d_slot = nullptr, *d_p = nullptr;
+      int64_t *d_off = nullptr;
+      double *d_d = nullptr;
+      try {
+        HIPCHK(hipMalloc(&d_slot, nn * sizeof(int)));
+        HIPCHK(hipMalloc(&d_off, (nn + 1) * sizeof(int64_t)));
+        HIPCHK(hipMalloc(&d_d, 2 * tot * sizeof(double)));
+        HIPCHK(hipMalloc(&d_p, tot * sizeof(int)));
+        HIPCHK(hipMemcpy(d_slot, slot_of.data(), nn * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_off, off.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        launch_pack_dperm(N->stream, N->d_nodes, d_slot, d_off, nn, d_d, d_p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dall.data(), d_d, 2 * tot * sizeof(double),
+                              hipMemcpyDeviceToHost, N->stream));
+        HIPCHK(hipMemcpyAsync(pall.data(), d_p, tot * sizeof(int), hipMemcpyDeviceToHost,
+                              N->stream));
+        HIPCHK(hipStreamSynchronize(N->stream));
+      } catch (HipError &) {
+        (void)hipFree(d_slot);
+        (void)hipFree(d_off);
+        (void)hipFree(d_d);
+        (void)hipFree(d_p);
+        throw;
+      }
+      (void)hipFree(d_slot);
+      (void)hipFree(d_off);
+      (void)hipFree(d_d);
+      (void)hipFree(d_p);
+    }
     // pivot order follows the original node order (postorder), as the
     // reference walks nodes 0..nnodes-1
-    for (int i = 0; i < S.nnodes; ++i) {
-      const NodeDev &D = N->nodes[S.slot[i]];
-      int ne = D.nelim;
-      std::vector<double> dh(2 * (size_t)std::max(ne, 1));
-      std::vector<int> ph(std::max(ne, 1));
-      if (ne > 0) {
-        HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(ph.data(), D.perm, ne * sizeof(int), hipMemcpyDeviceToHost));
-      }
+    for (int i = 0; i < nn; ++i) {
+      const int ne = (int)(off[i + 1] - off[i]);
+      const double *dh = dall.data() + 2 * off[i];
+      const int *ph = pall.data() + off[i];
       for (int k = 0; k < ne;) {
         if (k + 1 == ne || std::isfinite(dh[2 * k + 2])) {
           if (piv_order) piv_order[ph[k]] = piv;

@@ -5,6 +5,8 @@
 * kkt_laplace(...)                 : [H A^T; A 0] with H a shifted 7-point
                                       Laplacian and A random with exactly k
                                       distinct N(0,1) entries per row;
+* kkt_random(n, seed)              : one system of the batched workload (random
+                                      SPD H, 20% constraint rows);
 * random_indef / random_posdef     : small random test matrices in the spirit of
                                       gen_random_indef / gen_random_posdef
                                       (tests/ssids/ssids.f90:2830-2900).
@@ -73,6 +75,29 @@ def kkt_laplace(grid=16, shift=-0.5, m_con=1024, k=3, seed=SEED_BASE + 1) -> Low
     nh = grid ** 3
     h = _lap7(grid, grid, grid) + (6.0 + shift) * sp.identity(nh, format="csc")
     a = random_rows(rng, m_con, nh, k)
+    kk = sp.bmat([[h, a.T], [a, None]], format="csc")
+    return LowerCSC(kk)
+
+
+def kkt_random(n, seed=SEED_BASE + 4) -> LowerCSC:
+    """One system of the batched workload: K = [H A^T; A 0] of order n with
+    m = floor(0.2 n) constraint rows.  H (n - m square) has 8 off-diagonal
+    entries per row before symmetrisation (4 random N(0,1) partners per row,
+    mirrored), and the diagonal 1.1 * sum|off| of its row, so H is SPD; A
+    (m x (n - m)) has exactly 3 distinct N(0,1) entries per row.  The
+    inertia is then m negative eigenvalues when A has full row rank."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m_con = int(0.2 * n)
+    nh = n - m_con
+    r = np.repeat(np.arange(nh), 4)
+    c = rng.integers(0, nh, r.size)
+    keep = r != c
+    off = sp.coo_matrix((rng.standard_normal(int(keep.sum())), (r[keep], c[keep])),
+                        shape=(nh, nh)).tocsc()
+    off = off + off.T
+    diag = 1.1 * np.asarray(abs(off).sum(axis=1)).ravel() + 1e-3
+    h = off + sp.diags(diag)
+    a = random_rows(rng, m_con, nh, 3)
     kk = sp.bmat([[h, a.T], [a, None]], format="csc")
     return LowerCSC(kk)
 

@@ -122,6 +122,7 @@ class AKeep:
 class FKeep:
     numeric: int = None
     pos_def: bool = False
+    symbolic: int = None       # the analyse handle the factors belong to
     scaling: np.ndarray = None
     inform: SSIDSInform = field(default_factory=SSIDSInform)
 
@@ -283,12 +284,33 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
         # scaling is applied in elimination order inside the subtree
         sc = np.ascontiguousarray(np.asarray(scale, np.float64)[akeep.invp])
         scp = sc.ctypes.data_as(C.c_void_p)
+    if (fkeep is not None and fkeep.numeric and fkeep.symbolic == akeep.symbolic
+            and fkeep.pos_def == bool(posdef)):
+        # a re-factorization of the same analyse reuses the handle: its device
+        # pool, its static plan and the delays seen so far (the reference's
+        # fkeep is intent(inout) in ssids_factor)
+        from . import devmem
+        d_val = devmem.DeviceArray.from_host(val)
+        d_sc = devmem.DeviceArray.from_host(sc) if sc is not None else None
+        L.spral_ssids_hip_refactor_device(fkeep.numeric, d_val.ptr,
+                                          d_sc.ptr if d_sc is not None else None,
+                                          C.byref(opts), C.byref(st))
+        inf = _inform_from_stats(st, akeep)
+        fkeep.scaling, fkeep.inform = sc, inf
+        if inf.flag < 0:
+            L.spral_ssids_hip_destroy_numeric(fkeep.numeric)
+            fkeep.numeric = None
+        elif inf.flag == SSIDS_WARNING_FACT_SINGULAR and not options.action:
+            inf.flag = SSIDS_ERROR_SINGULAR
+        return fkeep, inf
     if fkeep is not None and fkeep.numeric:
         L.spral_ssids_hip_destroy_numeric(fkeep.numeric)
+        fkeep.numeric = None
     num = L.spral_ssids_hip_factor(bool(posdef), akeep.symbolic, val, val.size, scp,
                                    C.byref(opts), C.byref(st))
     inf = _inform_from_stats(st, akeep)
-    fk = FKeep(numeric=num, pos_def=bool(posdef), scaling=sc, inform=inf)
+    fk = FKeep(numeric=num, pos_def=bool(posdef), scaling=sc, inform=inf,
+               symbolic=akeep.symbolic)
     if num and inf.flag == SSIDS_WARNING_FACT_SINGULAR and not options.action:
         inf.flag = SSIDS_ERROR_SINGULAR
     return fk, inf
