@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--cpu-grid", type=int, default=38,
                     help="grid of the bounded CPU-baseline sample (oracle)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead of the default workload: factor+solve this many independent "
+                         "KKT systems (n uniform in [200, 5000]) in one batched call, against a "
+                         "loop of single calls")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -77,12 +81,74 @@ def cpu_baseline(grid):
                       f"{ak.num_flops:.3e} flops) in {dt:.2f} s"}
 
 
+def batch_bench(args):
+    """Independent systems in one call (spral_amd.batch) against a loop of
+    single calls on the same systems and orders; one JSON line."""
+    from spral_amd import batch, devmem, matrices, ssids
+    count = args.batch
+    rng = np.random.Generator(np.random.PCG64(20261202 + 4))
+    sizes = rng.integers(200, 5001, count)
+    mats = [matrices.kkt_random(int(n), seed=20261202 + 4 + i) for i, n in enumerate(sizes)]
+    b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=4))
+    assert inf.flag >= 0, inf
+    rhs = [m.full @ rng.uniform(-1, 1, m.n) for m in mats]
+    val = np.concatenate([m.val for m in mats])
+    for _ in range(args.warmup):
+        batch.batch_factor(b, val)
+        batch.batch_solve(b, rhs)
+    devmem.synchronize()
+    ts = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        finf, neg, rank = batch.batch_factor(b, val)
+        xs = batch.batch_solve(b, rhs)
+        devmem.synchronize()
+        ts.append(time.perf_counter() - t0)
+    tb = float(np.mean(ts))
+    bwd = max(float(matrices.bwd_error(m.full, x, r).max()) for m, x, r in zip(mats, xs, rhs))
+    ok = all(int(k) == int(0.2 * m.n) for k, m in zip(neg, mats))
+    # the loop: a fresh handle per system, as a caller of ssids_factor would
+    singles = []
+    for i, m in enumerate(mats):
+        o = ssids.SSIDSOptions(ordering=0)
+        ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, o, order=b.orders[i])
+        singles.append((ak, o))
+    devmem.synchronize()
+    t0 = time.perf_counter()
+    for (ak, o), m, r in zip(singles, mats, rhs):
+        fk, fi = ssids.ssids_factor(False, m.val, ak, o)
+        x = r.copy()
+        ssids.ssids_solve(x, ak, fk, o)
+        ssids.ssids_free(None, fk)
+    devmem.synchronize()
+    tl = time.perf_counter() - t0
+    for ak, _ in singles:
+        ssids.ssids_free(ak)
+    n_total, flops, fronts = int(b.offsets[-1]), int(b.akeep.num_flops), int(b.akeep.nnodes)
+    batch.batch_free(b)
+    print(json.dumps({
+        "metric": "batched factor+solve systems/s", "value": round(count / tb, 2),
+        "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{count} KKT systems, n uniform in [200, 5000], 20% "
+                               "constraint rows, AMD order each, 1 RHS each",
+                   "n_total": n_total, "num_flops": flops, "fronts": fronts,
+                   "parallelism": "single-gpu"},
+        "factor_ms_device": round(finf.factor_ms, 3),
+        "loop_value": round(count / tl, 2), "loop_s": round(tl, 3),
+        "speedup_vs_loop": round(tl / tb, 2), "max_bwd_err_raw": bwd,
+        "inertia_ok": ok}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the "
               "multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.batch > 0:
+        return batch_bench(args)
     from spral_amd import _lib, matrices, ssids
     if not _lib.LIB_PATH.exists():
         from spral_amd import build

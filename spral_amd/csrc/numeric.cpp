@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -75,7 +76,9 @@ struct HipError : std::runtime_error {
   } while (0)
 
 // ---------------------------------------------------------------------------
-// Device memory pool: large chunks, first-fit free list with coalescing.
+// Device memory pool: large chunks, best-fit free blocks (indexed by size, so
+// an allocation is O(log blocks) however many fronts a batch places) with
+// coalescing on release.
 // ---------------------------------------------------------------------------
 class Pool {
  public:
@@ -86,22 +89,21 @@ class Pool {
     bytes = (bytes + 255) & ~size_t(255);
     if (bytes == 0) bytes = 256;
     for (auto &c : chunks_) {
-      for (auto it = c.free.begin(); it != c.free.end(); ++it) {
-        if (it->second >= bytes) {
-          size_t off = it->first, sz = it->second;
-          c.free.erase(it);
-          if (sz > bytes) c.free[off + bytes] = sz - bytes;
-          used_ += bytes;
-          peak_ = std::max(peak_, used_);
-          return (char *)c.base + off;
-        }
-      }
+      auto it = c.bysize.lower_bound({bytes, 0});  // smallest block that fits
+      if (it == c.bysize.end()) continue;
+      const size_t sz = it->first, off = it->second;
+      c.bysize.erase(it);
+      c.free.erase(off);
+      if (sz > bytes) c.put(off + bytes, sz - bytes);
+      used_ += bytes;
+      peak_ = std::max(peak_, used_);
+      return (char *)c.base + off;
     }
     size_t csz = std::max(bytes, chunk_min_);
     Chunk c;
     HIPCHK(hipMalloc(&c.base, csz));
     c.size = csz;
-    if (csz > bytes) c.free[bytes] = csz - bytes;
+    if (csz > bytes) c.put(bytes, csz - bytes);
     chunks_.push_back(std::move(c));
     used_ += bytes;
     peak_ = std::max(peak_, used_);
@@ -114,20 +116,24 @@ class Pool {
     for (auto &c : chunks_) {
       char *b = (char *)c.base;
       if ((char *)p >= b && (char *)p < b + c.size) {
-        size_t off = (char *)p - b;
-        auto it = c.free.emplace(off, bytes).first;
-        auto nx = std::next(it);
-        if (nx != c.free.end() && it->first + it->second == nx->first) {
-          it->second += nx->second;
-          c.free.erase(nx);
+        size_t off = (char *)p - b, sz = bytes;
+        // merge with the free neighbours on either side
+        auto nx = c.free.lower_bound(off);
+        if (nx != c.free.end() && off + sz == nx->first) {
+          sz += nx->second;
+          c.bysize.erase({nx->second, nx->first});
+          nx = c.free.erase(nx);
         }
-        if (it != c.free.begin()) {
-          auto pv = std::prev(it);
-          if (pv->first + pv->second == it->first) {
-            pv->second += it->second;
-            c.free.erase(it);
+        if (nx != c.free.begin()) {
+          auto pv = std::prev(nx);
+          if (pv->first + pv->second == off) {
+            off = pv->first;
+            sz += pv->second;
+            c.bysize.erase({pv->second, pv->first});
+            c.free.erase(pv);
           }
         }
+        c.put(off, sz);
         used_ -= bytes;
         return;
       }
@@ -140,7 +146,12 @@ class Pool {
   struct Chunk {
     void *base = nullptr;
     size_t size = 0;
-    std::map<size_t, size_t> free;
+    std::map<size_t, size_t> free;                 // offset -> size
+    std::set<std::pair<size_t, size_t>> bysize;    // (size, offset)
+    void put(size_t off, size_t sz) {
+      free[off] = sz;
+      bysize.insert({sz, off});
+    }
   };
   std::vector<Chunk> chunks_;
   size_t chunk_min_ = size_t(1) << 30;
