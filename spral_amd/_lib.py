@@ -109,6 +109,13 @@ SIGNATURES = {
     "spral_ssids_hip_device_count": (C.c_int, []),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
+    "spral_rb_peek": (C.c_int, [C.c_char_p] + [C.c_void_p] * 9),
+    "spral_rb_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.c_char_p, C.c_char_p,
+                                 C.c_char_p]),
+    "spral_rb_read": (C.c_void_p, [C.c_char_p, C.c_int, C.c_int, C.c_int]
+                      + [C.c_void_p] * 6),
+    "spral_rb_fetch": (None, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "spral_ssids_hip_equilib_scale_sym": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p,
                                                     C.c_void_p, C.c_void_p, C.c_int,
                                                     C.c_float, C.POINTER(C.c_int)]),
