@@ -472,3 +472,82 @@ def ssids_free(akeep=None, fkeep=None):
         L.spral_ssids_hip_destroy_symbolic(akeep.symbolic)
         akeep.symbolic = None
     return 0
+
+
+# ---------------------------------------------------------------------------
+# Analyse residency: an analysed pattern saved and reloaded, and a cache of
+# analyses keyed by pattern for a resident service.  Arrays only (numpy .npz,
+# nothing pickled); loading rebuilds the device symbolic handle.
+# ---------------------------------------------------------------------------
+_AKEEP_ARRAYS = ("ptr", "row", "order", "invp", "sptr", "sparent", "rptr", "rlist", "nptr",
+                 "nlist")
+
+
+def ssids_save_akeep(akeep, path):
+    """Writes the analyse result (the arrays of akeep and its inform sizes)."""
+    np.savez(path, n=akeep.n, nnodes=akeep.nnodes, num_factor=akeep.num_factor,
+             num_flops=akeep.num_flops, nz=akeep.nz, flag=akeep.inform.flag,
+             maxfront=akeep.inform.maxfront, maxsupernode=akeep.inform.maxsupernode,
+             matrix_rank=akeep.inform.matrix_rank,
+             **{k: getattr(akeep, k) for k in _AKEEP_ARRAYS})
+
+
+def ssids_load_akeep(path, options=None):
+    """Reads a saved analyse; with options.use_gpu the device symbolic handle
+    is rebuilt from the arrays (no ordering or analyse is rerun)."""
+    options = options or SSIDSOptions()
+    z = np.load(path, allow_pickle=False)
+    ak = AKeep(n=int(z["n"]))
+    for k in _AKEEP_ARRAYS:
+        setattr(ak, k, z[k])
+    ak.nnodes, ak.nz = int(z["nnodes"]), int(z["nz"])
+    ak.num_factor, ak.num_flops = int(z["num_factor"]), int(z["num_flops"])
+    inf = SSIDSInform(flag=int(z["flag"]), maxfront=int(z["maxfront"]),
+                      maxsupernode=int(z["maxsupernode"]), matrix_rank=int(z["matrix_rank"]),
+                      num_factor=ak.num_factor, num_flops=ak.num_flops, num_sup=ak.nnodes)
+    if options.use_gpu and ak.nnodes > 0:
+        L = _lib.lib()
+        f = C.c_int(0)
+        opts = options.to_c()
+        ak.symbolic = L.spral_ssids_hip_create_symbolic(
+            options.device, ak.n, 1, ak.nnodes + 1, _as_i32(ak.sptr), _as_i32(ak.sparent),
+            _as_i64(ak.rptr), _as_i32(ak.rlist), _as_i64(ak.nptr), _as_i64(ak.nlist),
+            C.byref(opts), C.byref(f))
+        if not ak.symbolic:
+            inf.flag = f.value
+    ak.inform = inf
+    return ak, inf
+
+
+class AnalyseCache:
+    """Analyses kept per sparsity pattern: a pattern seen before (same n,
+    ptr, row and ordering options) returns its resident akeep without a new
+    ordering, analyse or device upload."""
+
+    def __init__(self):
+        self._store = {}
+
+    @staticmethod
+    def key(n, ptr, row, options, grid=None):
+        import hashlib
+        h = hashlib.sha256()
+        h.update(np.int64(n).tobytes())
+        h.update(_as_i64(ptr).tobytes())
+        h.update(_as_i32(row).tobytes())
+        h.update(repr((options.ordering, options.nemin, options.device, grid)).encode())
+        return h.hexdigest()
+
+    def analyse(self, n, ptr, row, options=None, grid=None):
+        options = options or SSIDSOptions()
+        k = self.key(n, ptr, row, options, grid)
+        if k not in self._store:
+            self._store[k] = ssids_analyse(False, n, ptr, row, options, grid=grid)
+        return self._store[k]
+
+    def __len__(self):
+        return len(self._store)
+
+    def free(self):
+        for ak, _ in self._store.values():
+            ssids_free(ak)
+        self._store.clear()
