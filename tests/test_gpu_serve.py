@@ -1,0 +1,94 @@
+"""The resident solver service (spral_amd/serve.py).
+
+Three patterns get interleaved refactor and solve requests (1, 4 or 16
+right-hand sides) from one producer thread each. The HBM budget is small
+enough to force evictions, so some patterns are refactored from scratch on
+their next request. Bars, written below: every result is bitwise the one of
+replaying its pattern's requests alone (the refactor, then that solve by
+itself, with the same FP64 refinement), and every solve reaches the scaled
+backward error 1e-14. Solves were batched (fewer rounds than requests), and
+p50/p99 latencies are reported.
+"""
+import threading
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from spral_amd import matrices, serve, ssids
+
+pytestmark = pytest.mark.gpu
+
+BWD_TOL = 1e-14
+
+
+def scripts(seed):
+    rng = np.random.default_rng(seed)
+    pats = {"lap": matrices.laplace3d(10, diag=5.7),
+            "kkt": matrices.kkt_laplace(grid=8, m_con=100),
+            "rnd": matrices.random_indef(300, density=0.03, seed=5)}
+    plan = {}
+    for name, m in pats.items():
+        steps = [("refactor", m.val.copy())]
+        for i in range(14):
+            if rng.random() < 0.2:
+                # new values on the same pattern: a refactor
+                v = m.val * (1.0 + 0.01 * rng.standard_normal(m.val.size))
+                steps.append(("refactor", v))
+            else:
+                k = int(rng.choice([1, 4, 16]))
+                b = rng.uniform(-1, 1, (m.n, k))
+                steps.append(("solve", b if k > 1 else b[:, 0]))
+        plan[name] = steps
+    return pats, plan
+
+
+def test_service_results_equal_alone(lib):
+    pats, plan = scripts(11)
+    opts = ssids.SSIDSOptions()
+    svc = serve.SolverService(hbm_budget_bytes=3 << 20, options=opts)  # forces evictions
+    for name, m in pats.items():
+        svc.register(name, m.n, m.ptr, m.row)
+    svc.start()
+    futures = {name: [] for name in plan}
+
+    def produce(name):
+        for kind, data in plan[name]:
+            f = svc.submit_refactor(name, data) if kind == "refactor" else \
+                svc.submit_solve(name, data)
+            futures[name].append(f)
+
+    th = [threading.Thread(target=produce, args=(n,)) for n in plan]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    results = {n: [f.result(timeout=120) for f in fs] for n, fs in futures.items()}
+    nreq = sum(len(v) for v in plan.values())
+    print(f"  {nreq} requests in {svc.rounds} rounds; solve latency {svc.latency_ms('solve')}",
+          flush=True)
+    svc.stop()
+    # replay alone
+    for name, m in pats.items():
+        o = ssids.SSIDSOptions()
+        ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, o)
+        fk = None
+        vals = None
+        for (kind, data), got in zip(plan[name], results[name]):
+            if kind == "refactor":
+                vals = data
+                if fk is not None:
+                    ssids.ssids_free(None, fk)
+                fk, inf = ssids.ssids_factor(False, vals, ak, o)
+                assert got.flag == inf.flag and got.num_neg == inf.num_neg
+                continue
+            cols = data.reshape(m.n, -1)
+            low = sp.csc_matrix((vals, m.row - 1, m.ptr - 1), shape=(m.n, m.n))
+            full = low + sp.tril(low, k=-1).T
+            for j in range(cols.shape[1]):
+                x = cols[:, j].copy()
+                ssids.ssids_solve_refine(x, ak, fk, o, tol=BWD_TOL)
+                assert np.array_equal(got.reshape(m.n, -1)[:, j], x), (name, j)
+                assert matrices.bwd_error(full, x, cols[:, j]).max() <= BWD_TOL
+        ssids.ssids_free(ak, fk)
+    assert svc.rounds < nreq
