@@ -42,6 +42,11 @@ def parse():
                     help="instead of the default workload: factor+solve this many independent "
                          "KKT systems (n uniform in [200, 5000]) in one batched call, against a "
                          "loop of single calls")
+    ap.add_argument("--serve", type=float, default=0.0,
+                    help="instead of the default workload: a Poisson stream of refactor (30%%) "
+                         "and solve (70%%, k in 1/4/16/64) requests on resident KKT patterns "
+                         "for this many seconds")
+    ap.add_argument("--rate", type=float, default=40.0, help="--serve arrival rate (1/s)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -141,6 +146,68 @@ def batch_bench(args):
         "inertia_ok": ok}), flush=True)
 
 
+def serve_bench(args):
+    """The resident solver service (spral_amd.serve) under a Poisson stream;
+    one JSON line with throughput and latency by request type."""
+    import threading
+    from spral_amd import matrices, serve, ssids
+    rng = np.random.Generator(np.random.PCG64(20261202 + 5))
+    grids = (16, 20, 24, 28)
+    pats = {f"kkt{g}": matrices.kkt_laplace(grid=g, m_con=g ** 3 // 8, seed=20261202 + 5 + g)
+            for g in grids}
+    nh = {f"kkt{g}": g ** 3 for g in grids}
+    svc = serve.SolverService(hbm_budget_bytes=200 << 30,
+                              options=ssids.SSIDSOptions(ordering=4))
+    for name, m in pats.items():
+        svc.register(name, m.n, m.ptr, m.row)
+    svc.start()
+    # H's diagonal entries (rows < nh on the diagonal) get the barrier terms
+    diag_pos = {}
+    for name, m in pats.items():
+        col = np.repeat(np.arange(m.n), np.diff(m.ptr))
+        diag_pos[name] = np.nonzero((m.row - 1 == col) & (col < nh[name]))[0]
+        svc.submit_refactor(name, m.val).result()  # resident before the stream
+    svc.lat = {"refactor": [], "solve": []}
+    names = list(pats)
+    futs, kinds = [], []
+    t_end = time.perf_counter() + args.serve
+    t_start = time.perf_counter()
+    while time.perf_counter() < t_end:
+        time.sleep(rng.exponential(1.0 / args.rate))
+        name = names[int(rng.integers(len(names)))]
+        m = pats[name]
+        if rng.random() < 0.3:
+            v = m.val.copy()
+            v[diag_pos[name]] += 10.0 ** rng.uniform(-8, 8, diag_pos[name].size)
+            futs.append(svc.submit_refactor(name, v))
+            kinds.append("refactor")
+        else:
+            k = int(rng.choice([1, 4, 16, 64]))
+            futs.append(svc.submit_solve(name, rng.uniform(-1, 1, (m.n, k))))
+            kinds.append("solve")
+    for f in futs:
+        f.result()
+    wall = time.perf_counter() - t_start
+    lr, ls = svc.latency_ms("refactor"), svc.latency_ms("solve")
+    rounds = svc.rounds
+    svc.stop()
+    print(json.dumps({
+        "metric": "served requests/s with p50/p99 latency", "value": round(len(futs) / wall, 2),
+        "unit": "requests/s", "n_gpus": 1, "steps": len(futs), "warmup": len(pats),
+        "ms_per_step": round(wall * 1e3 / max(1, len(futs)), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Poisson stream at {args.rate}/s for {args.serve} s on "
+                               f"{len(pats)} resident KKT patterns (7-pt H of grid^3 + "
+                               "grid^3/8 constraint rows, grids " + str(list(grids)) +
+                               "), AMD orders, 30% refactors with log-uniform [1e-8, 1e8] barrier "
+                               "terms on H's diagonal, 70% solves with k in {1,4,16,64}, "
+                               "FP64 refinement to 1e-14",
+                   "n": [int(m.n) for m in pats.values()], "parallelism": "single-gpu"},
+        "refactor_ms": lr, "solve_ms": ls, "rounds": rounds,
+        "requests": {"refactor": kinds.count("refactor"), "solve": kinds.count("solve")}}),
+        flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -149,6 +216,8 @@ def main():
         sys.exit(2)
     if args.batch > 0:
         return batch_bench(args)
+    if args.serve > 0:
+        return serve_bench(args)
     from spral_amd import _lib, matrices, ssids
     if not _lib.LIB_PATH.exists():
         from spral_amd import build
