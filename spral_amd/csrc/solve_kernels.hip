@@ -223,6 +223,96 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
   }
 }
 
+// k_fwd_tri for nrhs >= 2: each wave takes its own right-hand sides
+// (r = wave, wave + 4, ...), so no workgroup barrier falls inside a
+// right-hand side.  Every sum keeps k_fwd_tri's order (the children's
+// values arrive on k_fwd_tri's schedule; a row's triangle products are
+// summed in column order), so each column is bitwise what k_fwd_tri gives.
+// Wave-scope fences order one lane's LDS / global writes before another
+// lane's reads.
+__device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+__global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, const int *slots,
+                                                         const int *child_list, double *x,
+                                                         int ldx, int nrhs) {
+  __shared__ double xs[kGroups][kTriCap];
+  __shared__ double ybs[kGroups][kBlk];
+  const SolveNode &P = sn[slots[blockIdx.x]];
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gcint *Pipos = (gcint *)P.ipos;
+  gcint *Prows = (gcint *)P.rows;
+  gcint *Pctab = (gcint *)P.ctab;
+  gdouble *PX = (gdouble *)P.X;
+  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
+  const int cbeg = P.cbeg, cend = P.cend;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int T = idiv_up(m, 64), Tn = idiv_up(n, 64);
+  double *yb = ybs[wave];
+  for (int r = wave; r < nrhs; r += kGroups) {
+    gdouble *X = PX + (size_t)r * m;
+    double *xr = x + (size_t)r * ldx;
+    double *acc = (n <= kTriCap) ? xs[wave] : (double *)X;
+    for (int p = lane; p < n; p += 64) {
+      const int vrow = (p >= nd) ? Prows[p - nd] : -1;
+      acc[Pipos[p]] = (vrow >= 0) ? xr[vrow] : 0.0;
+    }
+    wave_fence();
+    // k_fwd_tri's schedule: children four at a time, 64-entry chunk by
+    // chunk, the four children's chunks in turn (a position fed by several
+    // children receives their values in that order)
+    for (int c0 = cbeg; c0 < cend; c0 += kGroups) {
+      int nch = 0;
+      for (int g = 0; g < kGroups && c0 + g < cend; ++g) {
+        gcint *tab = Pctab + (size_t)(c0 + g - cbeg) * (T + 1);
+        nch = imax(nch, idiv_up(tab[Tn] - tab[0], 64));
+      }
+      for (int it = 0; it < nch; ++it) {
+        for (int g = 0; g < kGroups && c0 + g < cend; ++g) {
+          const int ci = c0 + g;
+          const SolveNode &c = sn[child_list[ci]];
+          gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
+          const int kb = tab[0], ke = tab[Tn];
+          const int k = kb + it * 64 + lane;
+          if (k < ke) {
+            const int pre = (k < c.n) ? c.doff + (k - c.nelim) : nd + c.rmap[k - c.n];
+            if (pre < n) acc[Pipos[pre]] += c.X[(size_t)r * c.m + k];
+          }
+          wave_fence();
+        }
+      }
+    }
+    if (ne <= kBigTri) {
+      for (int j0 = 0; j0 < ne; j0 += kBlk) {
+        const int j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+        const double v =
+            fwd_diag_wave(PL, ldl, j0, wb, (lane < wb) ? acc[j0 + lane] : 0.0, lane);
+        if (lane < wb) acc[j0 + lane] = v;
+        yb[lane] = v;
+        wave_fence();
+        for (int i = j1 + lane; i < ne; i += 64) {
+          double sum = 0.0;
+#pragma unroll
+          for (int u0 = 0; u0 < kBlk; u0 += 16) {
+            double lv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+              lv[u] = PL[(size_t)(j0 + imin(u0 + u, wb - 1)) * ldl + i];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) sum += lv[u] * yb[u0 + u];
+          }
+          acc[i] -= sum;
+        }
+        wave_fence();
+      }
+      for (int p = lane; p < ne; p += 64) xr[Pperm[p]] = acc[p];
+    }
+    if (acc != (double *)X)
+      for (int p = lane; p < n; p += 64) X[p] = acc[p];
+    wave_fence();
+  }
+}
+
 // 64-row workgroups over [ne, m) of every front of the level (RC right-hand
 // sides per pass):
 //   X[i] = (gathered fully-summed value, i < n) or (sum of the children's
@@ -493,6 +583,88 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
   }
 }
 
+// k_bwd_tri for nrhs >= 2: the right-hand sides four at a time, wave w on
+// right-hand side r0 + w; the staged diagonal block is shared by the four.
+// Each wave reproduces the four row groups of k_bwd_tri (its partial sums
+// over workgroups g, g+4, ..., and its triangle rows [j1 + 64g, ...)) and
+// combines them in k_bwd_tri's order, so each column is bitwise what
+// k_bwd_tri gives.
+__global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, const int *slots,
+                                                         const int *wg_off, double *x,
+                                                         int ldx, int nrhs) {
+  __shared__ double xts[kGroups][kBigTri];
+  __shared__ double lb[kBlk][kBlk + 1];
+  const int li = blockIdx.x;
+  const SolveNode &P = sn[slots[li]];
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gdouble *PX = (gdouble *)P.X;
+  gcdouble *part = (gcdouble *)P.P;
+  const int ldl = P.ldl, m = P.m, ne = P.nelim;
+  const int count = wg_off[li + 1] - wg_off[li];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r0 = 0; r0 < nrhs; r0 += kGroups) {
+    const int r = r0 + wave;
+    const bool mine = r < nrhs;
+    double *xt = xts[wave];
+    if (mine) {
+      gdouble *X = PX + (size_t)r * m;
+      double *xr = x + (size_t)r * ldx;
+      double *acc = (ne <= kBigTri) ? xt : (double *)X;
+      for (int jc = 0; jc < ne; jc += kYChunk) {
+#pragma unroll
+        for (int k = 0; k < kYChunk / 64; ++k) {
+          const int j = jc + lane + 64 * k;
+          double sg[kGroups];
+#pragma unroll
+          for (int g = 0; g < kGroups; ++g) {
+            double sum = 0.0;
+            for (int w = g; w < count; w += kGroups)
+              if (j < ne) sum += part[((size_t)w * nrhs + r) * ne + j];
+            sg[g] = sum;
+          }
+          if (j < ne) acc[j] = xr[Pperm[j]] - (((sg[0] + sg[1]) + sg[2]) + sg[3]);
+        }
+      }
+    }
+    if (ne > kBigTri) continue;  // the triangle is solved by the big-front launches
+    __syncthreads();
+    for (int b = idiv_up(ne, kBlk) - 1; b >= 0; --b) {
+      const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0, rows = ne - j1;
+      stage_lower(PL, ldl, j0, wb, lb);
+      __syncthreads();  // the staged block is read by every wave
+      if (mine) {
+        double sg[kGroups];
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+          double sw = 0.0;
+          if (g * 64 < rows) {
+            const int t = 64 * g + lane;
+            const int i = j1 + imin(t, rows - 1);
+            const double xi = (t < rows) ? xt[i] : 0.0;
+            double a[kBlk];
+#pragma unroll
+            for (int u = 0; u < kBlk; ++u)
+              a[u] = PL[(size_t)(j0 + imin(u, wb - 1)) * ldl + i] * xi;
+            sw = wave_reduce_scatter<kBlk>(a, lane);
+          }
+          sg[g] = sw;
+        }
+        double v = 0.0;
+        if (lane < wb) v = xt[j0 + lane] - (((sg[0] + sg[1]) + sg[2]) + sg[3]);
+        v = bwd_diag_wave(lb, v, lane);
+        if (lane < wb) xt[j0 + lane] = v;
+      }
+      __syncthreads();
+    }
+    if (mine) {
+      double *xr = x + (size_t)r * ldx;
+      for (int j = lane; j < ne; j += 64) xr[Pperm[j]] = xt[j];
+    }
+    __syncthreads();
+  }
+}
+
 // ---- big fronts -------------------------------------------------------------
 // Forward, block [j0, j1): every workgroup solves the diagonal block from
 // X[j0:j1] (as left by the previous block's launch); workgroup q then updates
@@ -652,9 +824,13 @@ void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *
 
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                     const int *child_list, double *x, int ldx, int nrhs) {
-  if (nlvl > 0)
+  if (nlvl <= 0) return;
+  if (nrhs == 1)
     hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
                        ldx, nrhs);
+  else  // a wave per right-hand side: the same sums, no barrier per right-hand side
+    hipLaunchKernelGGL(k_fwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list,
+                       x, ldx, nrhs);
 }
 void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                      const int *child_list, int nrhs, const int *wg_off, int nwg) {
@@ -684,8 +860,12 @@ void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
       hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nrows), dim3(kThreads), 0, st, sn, slots,
                          rows_off, nlvl, x, ldx, nrhs);
   }
-  hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x, ldx,
-                     nrhs);
+  if (nrhs == 1)
+    hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
+                       ldx, nrhs);
+  else  // four right-hand sides at a time, one a wave
+    hipLaunchKernelGGL(k_bwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
+                       ldx, nrhs);
 }
 
 }  // namespace ssids_hip
