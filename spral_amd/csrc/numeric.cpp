@@ -205,6 +205,7 @@ struct Symbolic {
 // parent has been assembled, which stream order makes safe.
 struct Plan {
   bool valid = false;
+  int gen = 0;                             // bumped by every build (graphs key on it)
   int pivot_tpp = -1;
   int c32 = -1;                            // contribution-block precision planned for
   std::vector<NodeDev> tmpl;               // slot-indexed static fields
@@ -235,6 +236,11 @@ struct Numeric {
   // column runs there while the first pivot step runs on `stream`
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_level = nullptr, ev_rest = nullptr;
+  // the static-plan factorization captured as a graph, replayed by later
+  // refactors with the same plan, parameters and buffers
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<double> gkey;
   Pool pool;
   Plan plan;
   std::vector<int> nd_seen;               // per slot: most delays seen in a factor
@@ -291,6 +297,8 @@ struct Numeric {
     if (d_r) (void)hipFree(d_r);
     if (d_norms) (void)hipFree(d_norms);
     if (d_active) (void)hipFree(d_active);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (ev_level) (void)hipEventDestroy(ev_level);
     if (ev_rest) (void)hipEventDestroy(ev_rest);
     if (stream2) (void)hipStreamDestroy(stream2);
@@ -563,6 +571,7 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
   const double slack = std::max(0.0, multiplier - 1.0);
   P.pivot_tpp = pivot_tpp;
   P.c32 = N.c32;
+  ++P.gen;
   P.tmpl.assign(nn, NodeDev{});
   P.lv.assign(nlev, Plan::Level{});
   std::vector<std::pair<void *, size_t>> cof(nn), wof(nn);
@@ -631,6 +640,11 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
   P.valid = true;
 }
 
+static bool graphs_disabled() {
+  const char *e = std::getenv("SPRAL_SSIDS_HIP_NO_GRAPH");
+  return e && *e && *e != '0';
+}
+
 // One factorization on the static plan: all levels launched back to back, one
 // read-back at the end.  Returns false when some front's delays exceeded its
 // capacity (its results and its ancestors' are then void).
@@ -640,8 +654,6 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   Plan &P = N.plan;
   const int nn = S.nnodes, nlev = (int)S.lvl_ptr.size() - 1;
   hipStream_t st = N.stream;
-  HIPCHK(hipMemcpyAsync(N.d_nodes, P.d_tmpl, nn * sizeof(NodeDev), hipMemcpyDeviceToDevice,
-                        st));
   int *overflow = N.d_counters + 8;
   // The gather of a level is split: its first tile column (the first pivot
   // block and the rows below it) on `st`, the other tiles on `st2`.  The
@@ -657,6 +669,11 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
     st2 = N.stream2;
   }
+  // the whole launch sequence (no host round trip, no memset: the counters
+  // are cleared before it), captured once as a graph when not timing
+  auto enqueue = [&]() {
+  HIPCHK(hipMemcpyAsync(N.d_nodes, P.d_tmpl, nn * sizeof(NodeDev), hipMemcpyDeviceToDevice,
+                        st));
   for (int L = 0; L < nlev; ++L) {
     const int s0 = S.lvl_ptr[L], nl = S.lvl_ptr[L + 1] - s0;
     const Plan::Level &lv = P.lv[L];
@@ -705,8 +722,31 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     if (lv.cap[2 * ms + 2] > 0)
       launch_update(st, lv.cap[2 * ms + 2], N.d_nodes, d_slots, toff(2 * ms + 2), nl, 0, 1, fp);
   }
-  HIPCHK(hipGetLastError());
   launch_pack_nodes(st, N.d_nodes, nn, P.d_pack);
+  };
+  if (split && !graphs_disabled()) {
+    // a graph is valid for one plan, one set of pivot parameters and the
+    // same value and scaling buffers
+    const std::vector<double> key = {(double)P.gen, fp.u, fp.small, (double)fp.action,
+                                     (double)fp.tpp_failed, (double)fp.pivot_tpp,
+                                     (double)fp.posdef, (double)fp.c32,
+                                     (double)(uintptr_t)N.d_aval, (double)(uintptr_t)N.d_scal};
+    if (!N.gexec || N.gkey != key) {
+      if (N.gexec) HIPCHK(hipGraphExecDestroy(N.gexec));
+      if (N.graph) HIPCHK(hipGraphDestroy(N.graph));
+      N.gexec = nullptr;
+      N.graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+      enqueue();
+      HIPCHK(hipStreamEndCapture(st, &N.graph));
+      HIPCHK(hipGraphInstantiate(&N.gexec, N.graph, nullptr, nullptr, 0));
+      N.gkey = key;
+    }
+    HIPCHK(hipGraphLaunch(N.gexec, st));
+  } else {
+    enqueue();
+  }
+  HIPCHK(hipGetLastError());
   std::vector<int> pack(8 * (size_t)nn);
   int ovf = 0;
   HIPCHK(hipMemcpyAsync(pack.data(), P.d_pack, pack.size() * sizeof(int),
