@@ -47,6 +47,10 @@ def parse():
                          "and solve (70%%, k in 1/4/16/64) requests on resident KKT patterns "
                          "for this many seconds")
     ap.add_argument("--rate", type=float, default=40.0, help="--serve arrival rate (1/s)")
+    ap.add_argument("--kkt27", type=int, default=0,
+                    help="instead of the default workload: the mixed-precision KKT system "
+                         "(27-point H on this grid, grid^3 * 88000/512000 constraint rows with "
+                         "4 entries) factored and solved in FP64 and mixed precision")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -146,6 +150,65 @@ def batch_bench(args):
         "inertia_ok": ok}), flush=True)
 
 
+def kkt27_bench(args):
+    """FP64 against mixed precision (FP32 Schur updates) on the KKT system
+    [H A^T; A 0] with H a 27-point stencil (N(0,1) couplings, diagonal
+    27 + |N(0,1)|) and A with 4 N(0,1) entries per row, AMD order.  A step
+    is a refactor on the warm handle plus a solve refined in FP64 to 1e-14.
+    One JSON line; `value` is the FP64 factor GFLOP/s."""
+    from spral_amd import devmem, matrices, ssids
+    g = args.kkt27
+    mc = int(round(g ** 3 * 88000 / 512000))
+    m = matrices.kkt_stencil27(g, mc)
+    opts = ssids.SSIDSOptions(ordering=4)
+    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts)
+    assert ainf.flag >= 0, ainf
+    rng = np.random.Generator(np.random.PCG64(20261202 + 3))
+    x_true = rng.uniform(-1, 1, m.n)
+    b = m.full @ x_true
+    res = {}
+    for prec in (0, 1):
+        opts.schur_precision = prec
+        fk = None
+        fac, sol = [], []
+        for step in range(args.warmup + args.steps):
+            devmem.synchronize()
+            t0 = time.perf_counter()
+            fk, inf = ssids.ssids_factor(False, m.val, ak, opts, fkeep=fk)
+            devmem.synchronize()
+            t1 = time.perf_counter()
+            x = b.copy()
+            _, _, its = ssids.ssids_solve_refine(x, ak, fk, opts, tol=1e-14)
+            t2 = time.perf_counter()
+            print(f"  precision {prec} step {step}: factor {(t1 - t0) * 1e3:.1f} ms, "
+                  f"solve {(t2 - t1) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+            if step >= args.warmup:
+                fac.append(t1 - t0)
+                sol.append(t2 - t1)
+        res[prec] = {"factor_ms": round(float(np.mean(fac)) * 1e3, 1),
+                     "gflops": round(ak.num_flops / float(np.mean(fac)) / 1e9, 1),
+                     "solve_ms": round(float(np.mean(sol)) * 1e3, 2), "refine_steps": int(its),
+                     "bwd_err": float(matrices.bwd_error(m.full, x, b).max()),
+                     "fwd_err_max": float(np.abs(x - x_true).max()),
+                     "num_neg": int(inf.num_neg), "num_delay": int(inf.num_delay),
+                     "num_two": int(inf.num_two), "inertia_ok": int(inf.num_neg) == mc}
+        ssids.ssids_free(None, fk)
+    ssids.ssids_free(ak)
+    f64 = res[0]
+    print(json.dumps({
+        "metric": "factor GFLOP/s + solve ms at bwd err <=1e-14, FP64 vs mixed",
+        "value": f64["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(f64["factor_ms"] + f64["solve_ms"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"KKT: 27-point H on {g}^3, {mc} constraint rows x 4, AMD order, "
+                               "1 RHS, host buffers",
+                   "n": m.n, "nz": m.nz, "num_flops": int(ak.num_flops),
+                   "num_factor": int(ak.num_factor), "maxfront": int(ainf.maxfront),
+                   "parallelism": "single-gpu"},
+        "fp64": f64, "mixed": res[1]}), flush=True)
+
+
 def serve_bench(args):
     """The resident solver service (spral_amd.serve) under a Poisson stream;
     one JSON line with throughput and latency by request type."""
@@ -218,6 +281,8 @@ def main():
         return batch_bench(args)
     if args.serve > 0:
         return serve_bench(args)
+    if args.kkt27 > 0:
+        return kkt27_bench(args)
     from spral_amd import _lib, matrices, ssids
     if not _lib.LIB_PATH.exists():
         from spral_amd import build

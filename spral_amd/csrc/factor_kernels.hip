@@ -1181,6 +1181,8 @@ __device__ void tpp_swap(NodeDev &N, int c1, int c2, int m) {
   __syncthreads();
 }
 
+constexpr int kTppCols = 8;  // trailing columns per pass of the TPP update
+
 __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
   const double d11 = N.D[2 * k];
   for (int r = k + 1 + threadIdx.x; r < m; r += kThreads) {
@@ -1188,9 +1190,22 @@ __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
     Lat(N, r, k) = ld[r] * d11;
   }
   __syncthreads();
-  for (int c = k + 1; c < n; ++c) {
-    double sc = ld[c];
-    for (int r = c + threadIdx.x; r < m; r += kThreads) Lat(N, r, c) -= Lat(N, r, k) * sc;
+  // trailing update, kTppCols columns per pass: a thread keeps its row's
+  // pivot entry in a register and issues the column loads together
+  for (int c0 = k + 1; c0 < n; c0 += kTppCols) {
+    double sc[kTppCols];
+#pragma unroll
+    for (int j = 0; j < kTppCols; ++j) sc[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
+    for (int r = c0 + threadIdx.x; r < m; r += kThreads) {
+      const double lk = Lat(N, r, k);
+      double v[kTppCols];
+#pragma unroll
+      for (int j = 0; j < kTppCols; ++j)
+        v[j] = (c0 + j < n && r >= c0 + j) ? Lat(N, r, c0 + j) : 0.0;
+#pragma unroll
+      for (int j = 0; j < kTppCols; ++j)
+        if (c0 + j < n && r >= c0 + j) Lat(N, r, c0 + j) = v[j] - lk * sc[j];
+    }
   }
   if (threadIdx.x == 0) Lat(N, k, k) = 1.0;
   __syncthreads();
@@ -1206,10 +1221,23 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
     Lat(N, r, k + 1) = d21 * x1 + d22 * x2;
   }
   __syncthreads();
-  for (int c = k + 2; c < n; ++c) {
-    double s1 = ld[c], s2 = ld2[c];
-    for (int r = c + threadIdx.x; r < m; r += kThreads)
-      Lat(N, r, c) -= Lat(N, r, k) * s1 + Lat(N, r, k + 1) * s2;
+  for (int c0 = k + 2; c0 < n; c0 += kTppCols) {
+    double s1[kTppCols], s2[kTppCols];
+#pragma unroll
+    for (int j = 0; j < kTppCols; ++j) {
+      s1[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
+      s2[j] = (c0 + j < n) ? ld2[c0 + j] : 0.0;
+    }
+    for (int r = c0 + threadIdx.x; r < m; r += kThreads) {
+      const double l1 = Lat(N, r, k), l2 = Lat(N, r, k + 1);
+      double v[kTppCols];
+#pragma unroll
+      for (int j = 0; j < kTppCols; ++j)
+        v[j] = (c0 + j < n && r >= c0 + j) ? Lat(N, r, c0 + j) : 0.0;
+#pragma unroll
+      for (int j = 0; j < kTppCols; ++j)
+        if (c0 + j < n && r >= c0 + j) Lat(N, r, c0 + j) = v[j] - (l1 * s1[j] + l2 * s2[j]);
+    }
   }
   if (threadIdx.x == 0) {
     Lat(N, k, k) = 1.0;

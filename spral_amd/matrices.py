@@ -7,6 +7,10 @@
                                       distinct N(0,1) entries per row;
 * kkt_random(n, seed)              : one system of the batched workload (random
                                       SPD H, 20% constraint rows);
+* kkt_stencil27(grid, m_con, seed)  : the mixed-precision KKT workload: H a
+                                      27-point stencil with N(0,1) couplings
+                                      and diagonal 27 + |N(0,1)|, A random
+                                      with 4 N(0,1) entries per row;
 * random_indef / random_posdef     : small random test matrices in the spirit of
                                       gen_random_indef / gen_random_posdef
                                       (tests/ssids/ssids.f90:2830-2900).
@@ -98,6 +102,40 @@ def kkt_random(n, seed=SEED_BASE + 4) -> LowerCSC:
     diag = 1.1 * np.asarray(abs(off).sum(axis=1)).ravel() + 1e-3
     h = off + sp.diags(diag)
     a = random_rows(rng, m_con, nh, 3)
+    kk = sp.bmat([[h, a.T], [a, None]], format="csc")
+    return LowerCSC(kk)
+
+
+def kkt_stencil27(grid=80, m_con=88000, k=4, seed=SEED_BASE + 3) -> LowerCSC:
+    """K = [H A^T; A 0] of order grid^3 + m_con.  H couples each grid point
+    with its 26 neighbours of the 3x3x3 cube; each coupling is one N(0,1)
+    value (the same in both triangles), and the diagonal is 27 + |N(0,1)|.
+    The 26 couplings of a row sum to about 20.7 in absolute value, so H is
+    diagonally dominant in practice, and with A (m_con x grid^3, exactly k
+    distinct N(0,1) columns per row) of full row rank the inertia has m_con
+    negative eigenvalues."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = grid
+    nh = g ** 3
+    idx = np.arange(nh).reshape(g, g, g)  # [z, y, x]
+    rows, cols = [], []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if (dz, dy, dx) <= (0, 0, 0):
+                    continue  # each unordered pair once: offsets after 0
+                src = idx[max(0, -dz):g - max(0, dz), max(0, -dy):g - max(0, dy),
+                          max(0, -dx):g - max(0, dx)]
+                dst = idx[max(0, dz):g + min(0, dz) or None, max(0, dy):g + min(0, dy) or None,
+                          max(0, dx):g + min(0, dx) or None]
+                rows.append(dst.ravel())
+                cols.append(src.ravel())
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    v = rng.standard_normal(r.size)
+    low = sp.coo_matrix((v, (np.maximum(r, c), np.minimum(r, c))), shape=(nh, nh)).tocsc()
+    h = low + low.T + sp.diags(27.0 + np.abs(rng.standard_normal(nh)))
+    a = random_rows(rng, m_con, nh, k)
     kk = sp.bmat([[h, a.T], [a, None]], format="csc")
     return LowerCSC(kk)
 
