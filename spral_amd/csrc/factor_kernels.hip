@@ -1108,9 +1108,15 @@ __global__ __launch_bounds__(kThreads) void k_swap(NodeDev *nodes, const int *lv
 // [nelim, n) of the panel, one workgroup per front.  Updates the panel
 // columns only; the contribution block is updated afterwards (k_update mode 1).
 // ---------------------------------------------------------------------------
+// 16 waves: the trailing update streams the panel from L2 and HBM, and
+// more waves keep more of its loads in flight.  On a delay-heavy KKT root
+// 1024 threads beat 512 (no spills) by 7% and 256 by 28%; the 216 bytes a
+// lane spills at this size sit outside the update loops.
+constexpr int kTailThreads = 1024;
+
 struct TppShared {
-  double red[kThreads / 64];
-  int ired[kThreads / 64];
+  double red[kTailThreads / 64];
+  int ired[kTailThreads / 64];
   int flag;
 };
 
@@ -1120,7 +1126,7 @@ __device__ double wg_max(double v, TppShared &sh) {
   if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sh.red[0];
-  for (int i = 1; i < kThreads / 64; ++i) r = fmax(r, sh.red[i]);
+  for (int i = 1; i < kTailThreads / 64; ++i) r = fmax(r, sh.red[i]);
   __syncthreads();
   return r;
 }
@@ -1131,7 +1137,7 @@ __device__ int wg_and(int v, TppShared &sh) {
   if ((threadIdx.x & 63) == 0) sh.ired[threadIdx.x >> 6] = v;
   __syncthreads();
   int r = sh.ired[0];
-  for (int i = 1; i < kThreads / 64; ++i) r &= sh.ired[i];
+  for (int i = 1; i < kTailThreads / 64; ++i) r &= sh.ired[i];
   __syncthreads();
   return r;
 }
@@ -1144,9 +1150,9 @@ __device__ inline double &Lat(NodeDev &N, int r, int c) {
 __device__ int tpp_col_small(NodeDev &N, int idx, int from, int to, double small,
                              TppShared &sh) {
   int ok = 1;
-  for (int c = from + threadIdx.x; c < idx; c += kThreads)
+  for (int c = from + threadIdx.x; c < idx; c += kTailThreads)
     if (!(fabs(Lat(N, idx, c)) < small)) ok = 0;
-  for (int r = idx + threadIdx.x; r < to; r += kThreads)
+  for (int r = idx + threadIdx.x; r < to; r += kTailThreads)
     if (!(fabs(Lat(N, r, idx)) < small)) ok = 0;
   return wg_and(ok, sh);
 }
@@ -1154,9 +1160,9 @@ __device__ int tpp_col_small(NodeDev &N, int idx, int from, int to, double small
 __device__ double tpp_rc_max(NodeDev &N, int col, int nelim, int m, int skip,
                              TppShared &sh) {
   double best = 0.0;
-  for (int c = nelim + threadIdx.x; c < col; c += kThreads)
+  for (int c = nelim + threadIdx.x; c < col; c += kTailThreads)
     if (c != skip) best = fmax(best, fabs(Lat(N, col, c)));
-  for (int r = col + 1 + threadIdx.x; r < m; r += kThreads)
+  for (int r = col + 1 + threadIdx.x; r < m; r += kTailThreads)
     if (r != skip) best = fmax(best, fabs(Lat(N, r, col)));
   return wg_max(best, sh);
 }
@@ -1165,13 +1171,13 @@ __device__ void tpp_swap(NodeDev &N, int c1, int c2, int m) {
   if (c1 == c2) return;
   if (c2 < c1) { int t = c1; c1 = c2; c2 = t; }
   const int tid = threadIdx.x;
-  for (int c = tid; c < c1; c += kThreads) {
+  for (int c = tid; c < c1; c += kTailThreads) {
     double t = Lat(N, c1, c); Lat(N, c1, c) = Lat(N, c2, c); Lat(N, c2, c) = t;
   }
-  for (int i = c1 + 1 + tid; i < c2; i += kThreads) {
+  for (int i = c1 + 1 + tid; i < c2; i += kTailThreads) {
     double t = Lat(N, i, c1); Lat(N, i, c1) = Lat(N, c2, i); Lat(N, c2, i) = t;
   }
-  for (int r = c2 + 1 + tid; r < m; r += kThreads) {
+  for (int r = c2 + 1 + tid; r < m; r += kTailThreads) {
     double t = Lat(N, r, c1); Lat(N, r, c1) = Lat(N, r, c2); Lat(N, r, c2) = t;
   }
   if (tid == 0) {
@@ -1185,7 +1191,7 @@ constexpr int kTppCols = 8;  // trailing columns per pass of the TPP update
 
 __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
   const double d11 = N.D[2 * k];
-  for (int r = k + 1 + threadIdx.x; r < m; r += kThreads) {
+  for (int r = k + 1 + threadIdx.x; r < m; r += kTailThreads) {
     ld[r] = Lat(N, r, k);
     Lat(N, r, k) = ld[r] * d11;
   }
@@ -1196,7 +1202,7 @@ __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
     double sc[kTppCols];
 #pragma unroll
     for (int j = 0; j < kTppCols; ++j) sc[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
-    for (int r = c0 + threadIdx.x; r < m; r += kThreads) {
+    for (int r = c0 + threadIdx.x; r < m; r += kTailThreads) {
       const double lk = Lat(N, r, k);
       double v[kTppCols];
 #pragma unroll
@@ -1213,7 +1219,7 @@ __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
 
 __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *ld2) {
   const double d11 = N.D[2 * k], d21 = N.D[2 * k + 1], d22 = N.D[2 * k + 3];
-  for (int r = k + 2 + threadIdx.x; r < m; r += kThreads) {
+  for (int r = k + 2 + threadIdx.x; r < m; r += kTailThreads) {
     double x1 = Lat(N, r, k), x2 = Lat(N, r, k + 1);
     ld[r] = x1;
     ld2[r] = x2;
@@ -1228,7 +1234,7 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
       s1[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
       s2[j] = (c0 + j < n) ? ld2[c0 + j] : 0.0;
     }
-    for (int r = c0 + threadIdx.x; r < m; r += kThreads) {
+    for (int r = c0 + threadIdx.x; r < m; r += kTailThreads) {
       const double l1 = Lat(N, r, k), l2 = Lat(N, r, k + 1);
       double v[kTppCols];
 #pragma unroll
@@ -1260,7 +1266,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
   while (nelim < n) {
     if (tpp_col_small(N, nelim, nelim, m, small, sh)) {
       if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
-      for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
+      for (int r = nelim + tid; r < m; r += kTailThreads) Lat(N, r, nelim) = 0.0;
       if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
       __syncthreads();
       ++nelim;
@@ -1272,7 +1278,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
       if (tpp_col_small(N, p, nelim, m, small, sh)) {
         if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
         tpp_swap(N, p, nelim, m);
-        for (int r = nelim + tid; r < m; r += kThreads) Lat(N, r, nelim) = 0.0;
+        for (int r = nelim + tid; r < m; r += kTailThreads) Lat(N, r, nelim) = 0.0;
         if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
         __syncthreads();
         ++nelim;
@@ -1378,7 +1384,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
 // Values passed between the phases stay in registers; descriptor fields
 // written here are re-read only after a barrier.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lvl_slots,
+__global__ __launch_bounds__(kTailThreads) void k_tail(NodeDev *nodes, const int *lvl_slots,
                                                    FactorParams fp, int *scratch) {
   __shared__ TppShared sh;
   __shared__ int s_t;
@@ -1403,7 +1409,7 @@ __global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lv
   nelim = tpp_front(N, fp, nelim, sh, s_t);
   __syncthreads();
   // Dd
-  for (int k = tid; k < nelim; k += kThreads) {
+  for (int k = tid; k < nelim; k += kTailThreads) {
     double e1, e2;
     int two;
     pivot_d(N.D, k, nelim, e1, e2, two);
@@ -1427,7 +1433,7 @@ __global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lv
     const int n = N.n, m = N.m, ldl = N.ldl;
     const double *L = N.L, *Dd = N.Dd;
     double *LD = N.LD;
-    for (int i = n + tid; i < m; i += kThreads) {
+    for (int i = n + tid; i < m; i += kTailThreads) {
       for (int t = t0; t < nelim; ++t) {
         const double e = Dd[3 * t], f = Dd[3 * t + 1];
         const int off = (int)Dd[3 * t + 2];
@@ -1439,9 +1445,9 @@ __global__ __launch_bounds__(kThreads) void k_tail(NodeDev *nodes, const int *lv
   }
   // ipos
   const int n = N.n;
-  for (int i = tid; i < n; i += kThreads) scratch[N.perm[i]] = i;
+  for (int i = tid; i < n; i += kTailThreads) scratch[N.perm[i]] = i;
   __syncthreads();
-  for (int j = tid; j < n; j += kThreads) N.ipos[j] = scratch[N.perm0[j]];
+  for (int j = tid; j < n; j += kTailThreads) N.ipos[j] = scratch[N.perm0[j]];
 }
 
 // inertia and delay counts over all fronts (integer atomics: deterministic)
@@ -1643,7 +1649,7 @@ void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
 void launch_tail(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, FactorParams fp,
                  int *scratch) {
   if (nlvl > 0)
-    hipLaunchKernelGGL(k_tail, dim3(nlvl), dim3(kThreads), 0, st, nodes, slots, fp, scratch);
+    hipLaunchKernelGGL(k_tail, dim3(nlvl), dim3(kTailThreads), 0, st, nodes, slots, fp, scratch);
 }
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
