@@ -1108,36 +1108,41 @@ __global__ __launch_bounds__(kThreads) void k_swap(NodeDev *nodes, const int *lv
 // [nelim, n) of the panel, one workgroup per front.  Updates the panel
 // columns only; the contribution block is updated afterwards (k_update mode 1).
 // ---------------------------------------------------------------------------
-// 16 waves: the trailing update streams the panel from L2 and HBM, and
-// more waves keep more of its loads in flight.  On a delay-heavy KKT root
-// 1024 threads beat 512 (no spills) by 7% and 256 by 28%; the 216 bytes a
-// lane spills at this size sit outside the update loops.
-constexpr int kTailThreads = 1024;
+// Workgroup size of k_tail.  A level with fewer fronts than CUs gets 16
+// waves per front: the TPP trailing update streams the panel from L2 and HBM,
+// and more waves keep more of its loads in flight (on a delay-heavy KKT root
+// 1024 threads beat 512 by 7% and 256 by 28%; the 216 bytes a lane spills at
+// 1024 sit outside the update loops).  Wider levels keep 4 waves per front,
+// so that many small fronts share each CU.
+constexpr int kTailWide = 1024, kTailNarrow = 256, kTailWideMaxFronts = 256;
 
+template <int NT>
 struct TppShared {
-  double red[kTailThreads / 64];
-  int ired[kTailThreads / 64];
+  double red[NT / 64];
+  int ired[NT / 64];
   int flag;
 };
 
-__device__ double wg_max(double v, TppShared &sh) {
+template <int NT>
+__device__ double wg_max(double v, TppShared<NT> &sh) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh.red[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = sh.red[0];
-  for (int i = 1; i < kTailThreads / 64; ++i) r = fmax(r, sh.red[i]);
+  for (int i = 1; i < NT / 64; ++i) r = fmax(r, sh.red[i]);
   __syncthreads();
   return r;
 }
 
-__device__ int wg_and(int v, TppShared &sh) {
+template <int NT>
+__device__ int wg_and(int v, TppShared<NT> &sh) {
   for (int off = 32; off > 0; off >>= 1) v &= __shfl_xor(v, off);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh.ired[threadIdx.x >> 6] = v;
   __syncthreads();
   int r = sh.ired[0];
-  for (int i = 1; i < kTailThreads / 64; ++i) r &= sh.ired[i];
+  for (int i = 1; i < NT / 64; ++i) r &= sh.ired[i];
   __syncthreads();
   return r;
 }
@@ -1147,37 +1152,40 @@ __device__ inline double &Lat(NodeDev &N, int r, int c) {
 }
 
 // all |entries| of row/col idx within [from, to) below small?
+template <int NT>
 __device__ int tpp_col_small(NodeDev &N, int idx, int from, int to, double small,
-                             TppShared &sh) {
+                             TppShared<NT> &sh) {
   int ok = 1;
-  for (int c = from + threadIdx.x; c < idx; c += kTailThreads)
+  for (int c = from + threadIdx.x; c < idx; c += NT)
     if (!(fabs(Lat(N, idx, c)) < small)) ok = 0;
-  for (int r = idx + threadIdx.x; r < to; r += kTailThreads)
+  for (int r = idx + threadIdx.x; r < to; r += NT)
     if (!(fabs(Lat(N, r, idx)) < small)) ok = 0;
   return wg_and(ok, sh);
 }
 
+template <int NT>
 __device__ double tpp_rc_max(NodeDev &N, int col, int nelim, int m, int skip,
-                             TppShared &sh) {
+                             TppShared<NT> &sh) {
   double best = 0.0;
-  for (int c = nelim + threadIdx.x; c < col; c += kTailThreads)
+  for (int c = nelim + threadIdx.x; c < col; c += NT)
     if (c != skip) best = fmax(best, fabs(Lat(N, col, c)));
-  for (int r = col + 1 + threadIdx.x; r < m; r += kTailThreads)
+  for (int r = col + 1 + threadIdx.x; r < m; r += NT)
     if (r != skip) best = fmax(best, fabs(Lat(N, r, col)));
   return wg_max(best, sh);
 }
 
+template <int NT>
 __device__ void tpp_swap(NodeDev &N, int c1, int c2, int m) {
   if (c1 == c2) return;
   if (c2 < c1) { int t = c1; c1 = c2; c2 = t; }
   const int tid = threadIdx.x;
-  for (int c = tid; c < c1; c += kTailThreads) {
+  for (int c = tid; c < c1; c += NT) {
     double t = Lat(N, c1, c); Lat(N, c1, c) = Lat(N, c2, c); Lat(N, c2, c) = t;
   }
-  for (int i = c1 + 1 + tid; i < c2; i += kTailThreads) {
+  for (int i = c1 + 1 + tid; i < c2; i += NT) {
     double t = Lat(N, i, c1); Lat(N, i, c1) = Lat(N, c2, i); Lat(N, c2, i) = t;
   }
-  for (int r = c2 + 1 + tid; r < m; r += kTailThreads) {
+  for (int r = c2 + 1 + tid; r < m; r += NT) {
     double t = Lat(N, r, c1); Lat(N, r, c1) = Lat(N, r, c2); Lat(N, r, c2) = t;
   }
   if (tid == 0) {
@@ -1189,9 +1197,10 @@ __device__ void tpp_swap(NodeDev &N, int c1, int c2, int m) {
 
 constexpr int kTppCols = 8;  // trailing columns per pass of the TPP update
 
+template <int NT>
 __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
   const double d11 = N.D[2 * k];
-  for (int r = k + 1 + threadIdx.x; r < m; r += kTailThreads) {
+  for (int r = k + 1 + threadIdx.x; r < m; r += NT) {
     ld[r] = Lat(N, r, k);
     Lat(N, r, k) = ld[r] * d11;
   }
@@ -1202,7 +1211,7 @@ __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
     double sc[kTppCols];
 #pragma unroll
     for (int j = 0; j < kTppCols; ++j) sc[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
-    for (int r = c0 + threadIdx.x; r < m; r += kTailThreads) {
+    for (int r = c0 + threadIdx.x; r < m; r += NT) {
       const double lk = Lat(N, r, k);
       double v[kTppCols];
 #pragma unroll
@@ -1217,9 +1226,10 @@ __device__ void tpp_elim1(NodeDev &N, int k, int m, int n, double *ld) {
   __syncthreads();
 }
 
+template <int NT>
 __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *ld2) {
   const double d11 = N.D[2 * k], d21 = N.D[2 * k + 1], d22 = N.D[2 * k + 3];
-  for (int r = k + 2 + threadIdx.x; r < m; r += kTailThreads) {
+  for (int r = k + 2 + threadIdx.x; r < m; r += NT) {
     double x1 = Lat(N, r, k), x2 = Lat(N, r, k + 1);
     ld[r] = x1;
     ld2[r] = x2;
@@ -1234,7 +1244,7 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
       s1[j] = (c0 + j < n) ? ld[c0 + j] : 0.0;
       s2[j] = (c0 + j < n) ? ld2[c0 + j] : 0.0;
     }
-    for (int r = c0 + threadIdx.x; r < m; r += kTailThreads) {
+    for (int r = c0 + threadIdx.x; r < m; r += NT) {
       const double l1 = Lat(N, r, k), l2 = Lat(N, r, k + 1);
       double v[kTppCols];
 #pragma unroll
@@ -1255,7 +1265,8 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
 
 // TPP on the columns [nelim, n) that APP left, by the whole workgroup;
 // returns the new nelim (uniform) and stores it in N.nelim.
-__device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShared &sh,
+template <int NT>
+__device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShared<NT> &sh,
                          int &s_t) {
   const int n = N.n, m = N.m;
   if (nelim >= n) return nelim;
@@ -1266,7 +1277,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
   while (nelim < n) {
     if (tpp_col_small(N, nelim, nelim, m, small, sh)) {
       if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
-      for (int r = nelim + tid; r < m; r += kTailThreads) Lat(N, r, nelim) = 0.0;
+      for (int r = nelim + tid; r < m; r += NT) Lat(N, r, nelim) = 0.0;
       if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
       __syncthreads();
       ++nelim;
@@ -1277,8 +1288,8 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
     for (p = nelim + 1; p < n; ++p) {
       if (tpp_col_small(N, p, nelim, m, small, sh)) {
         if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
-        tpp_swap(N, p, nelim, m);
-        for (int r = nelim + tid; r < m; r += kTailThreads) Lat(N, r, nelim) = 0.0;
+        tpp_swap<NT>(N, p, nelim, m);
+        for (int r = nelim + tid; r < m; r += NT) Lat(N, r, nelim) = 0.0;
         if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
         __syncthreads();
         ++nelim;
@@ -1322,8 +1333,8 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
         }
       }
       if (ok2) {
-        tpp_swap(N, t, nelim, m);
-        tpp_swap(N, p, nelim + 1, m);
+        tpp_swap<NT>(N, t, nelim, m);
+        tpp_swap<NT>(N, p, nelim + 1, m);
         if (tid == 0) {
           N.D[2 * nelim] = d0;
           N.D[2 * nelim + 1] = d1;
@@ -1331,20 +1342,20 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
           N.D[2 * nelim + 3] = d3;
         }
         __syncthreads();
-        tpp_elim2(N, nelim, m, n, ld, ld2);
+        tpp_elim2<NT>(N, nelim, m, n, ld, ld2);
         nelim += 2;
         done = 1;
         break;
       }
       maxp = fmax(maxp, fabs(Lat(N, p, t)));
       if (fabs(Lat(N, p, p)) >= u * maxp) {
-        tpp_swap(N, p, nelim, m);
+        tpp_swap<NT>(N, p, nelim, m);
         if (tid == 0) {
           N.D[2 * nelim] = 1.0 / Lat(N, nelim, nelim);
           N.D[2 * nelim + 1] = 0.0;
         }
         __syncthreads();
-        tpp_elim1(N, nelim, m, n, ld);
+        tpp_elim1<NT>(N, nelim, m, n, ld);
         nelim += 1;
         done = 1;
         break;
@@ -1359,7 +1370,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
           N.D[2 * nelim + 1] = 0.0;
         }
         __syncthreads();
-        tpp_elim1(N, nelim, m, n, ld);
+        tpp_elim1<NT>(N, nelim, m, n, ld);
         nelim += 1;
       } else {
         break;
@@ -1384,9 +1395,10 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
 // Values passed between the phases stay in registers; descriptor fields
 // written here are re-read only after a barrier.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kTailThreads) void k_tail(NodeDev *nodes, const int *lvl_slots,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_tail(NodeDev *nodes, const int *lvl_slots,
                                                    FactorParams fp, int *scratch) {
-  __shared__ TppShared sh;
+  __shared__ TppShared<NT> sh;
   __shared__ int s_t;
   NodeDev &N = nodes[lvl_slots[blockIdx.x]];
   const int tid = threadIdx.x;
@@ -1409,7 +1421,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(NodeDev *nodes, const int
   nelim = tpp_front(N, fp, nelim, sh, s_t);
   __syncthreads();
   // Dd
-  for (int k = tid; k < nelim; k += kTailThreads) {
+  for (int k = tid; k < nelim; k += NT) {
     double e1, e2;
     int two;
     pivot_d(N.D, k, nelim, e1, e2, two);
@@ -1433,7 +1445,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(NodeDev *nodes, const int
     const int n = N.n, m = N.m, ldl = N.ldl;
     const double *L = N.L, *Dd = N.Dd;
     double *LD = N.LD;
-    for (int i = n + tid; i < m; i += kTailThreads) {
+    for (int i = n + tid; i < m; i += NT) {
       for (int t = t0; t < nelim; ++t) {
         const double e = Dd[3 * t], f = Dd[3 * t + 1];
         const int off = (int)Dd[3 * t + 2];
@@ -1445,9 +1457,9 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(NodeDev *nodes, const int
   }
   // ipos
   const int n = N.n;
-  for (int i = tid; i < n; i += kTailThreads) scratch[N.perm[i]] = i;
+  for (int i = tid; i < n; i += NT) scratch[N.perm[i]] = i;
   __syncthreads();
-  for (int j = tid; j < n; j += kTailThreads) N.ipos[j] = scratch[N.perm0[j]];
+  for (int j = tid; j < n; j += NT) N.ipos[j] = scratch[N.perm0[j]];
 }
 
 // inertia and delay counts over all fronts (integer atomics: deterministic)
@@ -1648,8 +1660,13 @@ void launch_maptab(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots,
 }
 void launch_tail(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, FactorParams fp,
                  int *scratch) {
-  if (nlvl > 0)
-    hipLaunchKernelGGL(k_tail, dim3(nlvl), dim3(kTailThreads), 0, st, nodes, slots, fp, scratch);
+  if (nlvl <= 0) return;
+  if (nlvl < kTailWideMaxFronts)
+    hipLaunchKernelGGL(k_tail<kTailWide>, dim3(nlvl), dim3(kTailWide), 0, st, nodes, slots, fp,
+                       scratch);
+  else
+    hipLaunchKernelGGL(k_tail<kTailNarrow>, dim3(nlvl), dim3(kTailNarrow), 0, st, nodes, slots,
+                       fp, scratch);
 }
 void launch_diag(hipStream_t st, int nlvl, NodeDev *nodes, const int *slots, int step,
                  FactorParams fp) {
