@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--solves", type=int, default=50, help="solve latency samples")
-    ap.add_argument("--cpu-grid", type=int, default=38,
+    ap.add_argument("--cpu-grid", type=int, default=48,
                     help="grid of the bounded CPU-baseline sample (oracle)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
