@@ -73,6 +73,26 @@ def test_mixed_kkt_config1(lib):
     ssids.ssids_free(ak, mixed[0])
 
 
+def test_mixed_kkt_stencil27(lib):
+    """The mixed-precision KKT workload at 12^3 (n = 2,025): H is SPD and A
+    has full row rank (checked on the CPU in test_matrices), so both
+    factorizations give exactly m_con negative pivots. Many constraint
+    columns fail APP and go through the TPP fallback."""
+    mc = 297
+    m = matrices.kkt_stencil27(12, mc)
+    ak, (f64, i64, o64), mixed = factor_pair(m)
+    print(f"  delays fp64={i64.num_delay} mixed={mixed[1].num_delay}", flush=True)
+    assert i64.num_neg == mixed[1].num_neg == mc
+    assert i64.matrix_rank == mixed[1].matrix_rank == m.n
+    xt, b = rhs(m, 1, 7)
+    x = np.array(b, copy=True)
+    ssids.ssids_solve_refine(x, ak, f64, o64, tol=BWD_TOL)
+    assert matrices.bwd_error(m.full, x, b).max() <= BWD_TOL
+    check_mixed(m, ak, mixed, seed=7)
+    ssids.ssids_free(None, f64)
+    ssids.ssids_free(ak, mixed[0])
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_mixed_random_with_delays(lib, seed):
     m = matrices.random_indef(200 + 60 * seed, density=0.03, seed=40 + seed)
