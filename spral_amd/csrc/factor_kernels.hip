@@ -1296,15 +1296,22 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
         done = 1;
         break;
       }
-      // t = argmax |a(p, nelim..p-1)| (first index on ties)
-      if (tid == 0) {
-        int best = nelim;
-        double bv = fabs(Lat(N, p, nelim));
-        for (int c = nelim + 1; c < p; ++c) {
-          double v = fabs(Lat(N, p, c));
+      // t = argmax |a(p, nelim..p-1)| (first index on ties), by wave 0: each
+      // lane keeps the first maximum of its columns, and the merge prefers
+      // the larger value, then the smaller index
+      if (tid < 64) {
+        int best = INT_MAX;
+        double bv = -1.0;
+        for (int c = nelim + tid; c < p; c += 64) {
+          const double v = fabs(Lat(N, p, c));
           if (v > bv) { bv = v; best = c; }
         }
-        s_t = best;
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off);
+          const int oi = __shfl_xor(best, off);
+          if (ov > bv || (ov == bv && oi < best)) { bv = ov; best = oi; }
+        }
+        if (tid == 0) s_t = best;
       }
       __syncthreads();
       const int t = s_t;
