@@ -179,7 +179,14 @@ int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x,
  * backward error max|b-Ax| / (||A||inf ||x||inf + ||b||inf)
  * (tests/ssids/ssids.f90:1762-1810) is <= tol, or maxit corrections.
  * bwd (host, nrhs) and *iters receive the final state.  b, x in elimination
- * order.  _device: b and x are distinct buffers in HBM. */
+ * order.  _device: b and x are distinct buffers in HBM.
+ * Returns 0 when every column reached tol, SPRAL_SSIDS_HIP_WARNING_NOT_CONVERGED
+ * when some column is still above tol after maxit corrections (x holds the
+ * last iterate), SPRAL_SSIDS_HIP_ERROR_NONFINITE when a residual, solution or
+ * right-hand side holds a NaN or infinity (bwd of that column is NaN), and a
+ * negative SSIDS flag on other errors. */
+#define SPRAL_SSIDS_HIP_WARNING_NOT_CONVERGED 60
+#define SPRAL_SSIDS_HIP_ERROR_NONFINITE (-60)
 int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, double *x,
                                  int ldx, int maxit, double tol, double *bwd, int *iters);
 int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,

@@ -1270,7 +1270,10 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
                          int &s_t) {
   const int n = N.n, m = N.m;
   if (nelim >= n) return nelim;
-  if (!(fp.pivot_tpp || N.is_root || fp.tpp_failed)) return nelim;
+  // as factor_node_indef (src/ssids/cpu/factor.hxx:77-80): TPP finishes a
+  // front with no contribution block (m == n), or every front when the pivot
+  // method or the failed-pivot method is TPP
+  if (!(fp.pivot_tpp || m == n || fp.tpp_failed)) return nelim;
   const double u = fp.u, small = fp.small;
   double *ld = N.bak, *ld2 = N.bak + N.ldbak;
   const int tid = threadIdx.x;
@@ -1472,15 +1475,17 @@ __global__ __launch_bounds__(NT) void k_tail(NodeDev *nodes, const int *lvl_slot
 // inertia and delay counts over all fronts (integer atomics: deterministic)
 constexpr int kStatsWaves = 16, kStatsGroups = 64;
 __global__ __launch_bounds__(64 * kStatsWaves) void k_stats(const NodeDev *nodes, int nslots,
-                                                            int *counters) {
+                                                            int *counters, int pivot_tpp,
+                                                            int tpp_failed) {
   // a wave per front (fronts strided over the grid's waves), a lane per
   // pivot: pivot i is the second member of a 2x2 when its D^{-1} slot holds
   // infinity ([d11, d21 | inf, d22]), the first member when the next one is,
   // a 1x1 otherwise.  Sums go wave -> workgroup (LDS) -> 7 atomics per
   // workgroup (integer: deterministic).
-  __shared__ int red[kStatsWaves][7];
+  constexpr int kC = 8;
+  __shared__ int red[kStatsWaves][kC];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int c[7] = {0, 0, 0, 0, 0, 0, 0};
+  int c[kC] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int s = blockIdx.x * kStatsWaves + wv; s < nslots; s += gridDim.x * kStatsWaves) {
     const NodeDev &N = nodes[s];
     const int ne = N.nelim;
@@ -1501,19 +1506,27 @@ __global__ __launch_bounds__(64 * kStatsWaves) void k_stats(const NodeDev *nodes
     }
     if (lane == 0) {
       c[3] += N.n - N.nelim;
-      c[4] += N.n - N.nelim_app;  // not_first_pass
+      // not_first_pass / not_second_pass as src/ssids/cpu/factor.hxx:74-105:
+      // the columns APP left (TPP pivoting: the columns TPP left), and the
+      // columns a TPP pass over APP's failures left
+      if (pivot_tpp) {
+        c[4] += N.n - N.nelim;
+      } else if (N.nelim_app < N.n) {
+        c[4] += N.n - N.nelim_app;
+        if (N.m == N.n || tpp_failed) c[7] += N.n - N.nelim;
+      }
       c[5] += (N.flag == -5);
       c[6] += (N.flag == -6);
     }
   }
 #pragma unroll
-  for (int j = 0; j < 7; ++j) {
+  for (int j = 0; j < kC; ++j) {
     int v = c[j];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     if (lane == 0) red[wv][j] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 7) {
+  if (threadIdx.x < kC) {
     int v = 0;
     for (int q = 0; q < kStatsWaves; ++q) v += red[q][threadIdx.x];
     if (v) atomicAdd(&counters[threadIdx.x], v);
@@ -1700,11 +1713,13 @@ void launch_swap(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
     hipLaunchKernelGGL(k_swap, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
                        nlvl, step, phase);
 }
-void launch_stats(hipStream_t st, const NodeDev *nodes, int nslots, int *counters) {
+void launch_stats(hipStream_t st, const NodeDev *nodes, int nslots, int *counters,
+                  const FactorParams &fp) {
   if (nslots > 0)
     hipLaunchKernelGGL(k_stats,
                        dim3(std::min(kStatsGroups, (nslots + kStatsWaves - 1) / kStatsWaves)),
-                       dim3(64 * kStatsWaves), 0, st, nodes, nslots, counters);
+                       dim3(64 * kStatsWaves), 0, st, nodes, nslots, counters, fp.pivot_tpp,
+                       fp.tpp_failed);
 }
 
 }  // namespace ssids_hip

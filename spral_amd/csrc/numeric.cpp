@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <set>
 #include <memory>
@@ -38,7 +39,7 @@ void launch_update(hipStream_t, int, NodeDev *, const int *, const int *, int, i
                    FactorParams);
 void launch_swap(hipStream_t, int, NodeDev *, const int *, const int *, int, int, int);
 void launch_tail(hipStream_t, int, NodeDev *, const int *, FactorParams, int *);
-void launch_stats(hipStream_t, const NodeDev *, int, int *);
+void launch_stats(hipStream_t, const NodeDev *, int, int *, const FactorParams &);
 void launch_level_setup(hipStream_t, NodeDev *, int, int, const int *, int, int, int *, int *);
 void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
 void launch_pack_dperm(hipStream_t, const NodeDev *, const int *, const int64_t *, int, double *,
@@ -275,7 +276,8 @@ struct Numeric {
   double *d_r = nullptr;                  // refinement workspace (2 x n x nrhs)
   size_t d_r_cap = 0;
   unsigned long long *d_norms = nullptr;  // 1 + 3*nrhs ordered-bit maxima
-  int *d_active = nullptr;
+  int *d_active = nullptr;                // nrhs flags
+  int d_norms_nrhs = 0;                   // columns d_norms/d_active hold
   double anorm = -1.0;
   // contribution export
   std::vector<double> contrib_val, contrib_delay_val;
@@ -961,7 +963,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   const int nlev = (int)S.lvl_ptr.size() - 1;
   int maxfront = 0;
   for (int sl = 0; sl < nn; ++sl) maxfront = std::max(maxfront, N.nodes[sl].m);
-  launch_stats(st, N.d_nodes, nn, N.d_counters);
+  launch_stats(st, N.d_nodes, nn, N.d_counters, fp);
   int cnt[16];
   HIPCHK(hipMemcpyAsync(cnt, N.d_counters, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ev1, st));
@@ -977,7 +979,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     o.num_zero = cnt[2];
     o.num_delay = cnt[3];
     o.not_first_pass = cnt[4];
-    o.not_second_pass = cnt[3];
+    o.not_second_pass = cnt[7];
     o.flag = (cnt[2] > 0) ? 7 : 0;  // SSIDS_WARNING_FACT_SINGULAR
     if (cnt[5] > 0) o.flag = -5;
     if (cnt[6] > 0) o.flag = -6;  // SSIDS_ERROR_NOT_POS_DEF
@@ -1212,12 +1214,22 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
   size_t need = (size_t)ld * nrhs;
   if (need > N.d_r_cap) {
     if (N.d_r) HIPCHK(hipFree(N.d_r));
+    N.d_r = nullptr;
+    N.d_r_cap = 0;
     HIPCHK(hipMalloc(&N.d_r, std::max(need, size_t(1)) * sizeof(double)));
     N.d_r_cap = need;
+  }
+  // the per-column norms and flags are sized by nrhs alone (a call with a
+  // larger ld and fewer columns can fill d_r without them growing)
+  if (nrhs > N.d_norms_nrhs) {
     if (N.d_norms) HIPCHK(hipFree(N.d_norms));
     if (N.d_active) HIPCHK(hipFree(N.d_active));
+    N.d_norms = nullptr;
+    N.d_active = nullptr;
+    N.d_norms_nrhs = 0;
     HIPCHK(hipMalloc(&N.d_norms, (1 + 3 * (size_t)nrhs) * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&N.d_active, nrhs * sizeof(int)));
+    N.d_norms_nrhs = nrhs;
   }
   std::vector<unsigned long long> nb(1 + 3 * (size_t)nrhs);
   if (N.anorm < 0) {
@@ -1236,6 +1248,7 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
   solve_scaled(N, nrhs, d_x, ld, st);
   std::vector<int> active(nrhs, 1);
   int it = 0;
+  bool nonfinite = false, converged = false;
   for (;;) {
     launch_residual(st, n, S.d_fptr, S.d_fcol, S.d_fsrc, N.d_aval, d_x, d_b, N.d_r, ld, nrhs);
     HIPCHK(hipMemsetAsync(N.d_norms, 0, (1 + 3 * (size_t)nrhs) * sizeof(unsigned long long), st));
@@ -1246,17 +1259,25 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     bool all_ok = true;
+    nonfinite = false;
     for (int k = 0; k < nrhs; ++k) {
       double rv, xv, bv;
       std::memcpy(&rv, &nb[1 + 3 * k], 8);
       std::memcpy(&xv, &nb[1 + 3 * k + 1], 8);
       std::memcpy(&bv, &nb[1 + 3 * k + 2], 8);
-      double den = N.anorm * xv + bv;
-      bwd[k] = den > 0 ? rv / den : 0.0;
-      active[k] = bwd[k] > tol ? 1 : 0;
+      const double den = N.anorm * xv + bv;
+      // the maxima propagate NaN; a non-finite residual, solution or
+      // right-hand side is never taken as converged
+      if (!std::isfinite(rv) || !std::isfinite(den)) {
+        bwd[k] = std::numeric_limits<double>::quiet_NaN();
+        nonfinite = true;
+      } else {
+        bwd[k] = den > 0 ? rv / den : 0.0;
+      }
+      active[k] = (bwd[k] <= tol) ? 0 : 1;
       if (active[k]) all_ok = false;
     }
-    if (all_ok || it >= maxit) break;
+    if (all_ok || it >= maxit || nonfinite) break;
     HIPCHK(hipMemcpyAsync(N.d_active, active.data(), nrhs * sizeof(int),
                           hipMemcpyHostToDevice, st));
     solve_scaled(N, nrhs, N.d_r, ld, st);
@@ -1264,7 +1285,10 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
     ++it;
   }
   *iters = it;
-  return 0;
+  converged = true;
+  for (int k = 0; k < nrhs; ++k) converged = converged && active[k] == 0;
+  if (nonfinite) return SPRAL_SSIDS_HIP_ERROR_NONFINITE;
+  return converged ? 0 : SPRAL_SSIDS_HIP_WARNING_NOT_CONVERGED;
 }
 
 }  // namespace ssids_hip

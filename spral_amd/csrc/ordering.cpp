@@ -6,7 +6,8 @@
 // numeric path and to any CPU baseline:
 //  * order_grid_nd: geometric nested dissection of an nx*ny*nz grid whose
 //    variable (i,j,k) is i + nx*(j + ny*k) (0-based), separators are middle
-//    planes of the longest box dimension, ordered after both halves;
+//    planes of the longest box dimension, ordered after both halves, each
+//    with the vertices next to an enclosing separator first;
 //  * order_min_degree: exact minimum degree on the explicit elimination graph
 //    (ties to the lowest index), for moderate n;
 //  * order_amd: approximate minimum degree on the quotient graph, for large
@@ -26,7 +27,13 @@ struct Box {
   int lo[3], hi[3];  // half-open
 };
 
-void nd_box(const Box& b, int nx, int ny, std::vector<int>& seq) {
+// Separator vertices that touch the box's outside (an ancestor separator,
+// never the domain boundary) come first, the rest after.  Each such vertex
+// brings one ancestor row of its own into the fill; eliminating them first
+// lets every later separator column share one structure, so the rest of the
+// separator is a single supernode instead of one per grid line.
+void nd_box(const Box& b, const int dims[3], std::vector<int>& seq) {
+  const int nx = dims[0], ny = dims[1];
   int ext[3] = {b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]};
   if (ext[0] <= 0 || ext[1] <= 0 || ext[2] <= 0) return;
   int vol = ext[0] * ext[1] * ext[2];
@@ -45,11 +52,22 @@ void nd_box(const Box& b, int nx, int ny, std::vector<int>& seq) {
   right.lo[d] = mid + 1;
   sep.lo[d] = mid;
   sep.hi[d] = mid + 1;
-  nd_box(left, nx, ny, seq);
-  nd_box(right, nx, ny, seq);
-  for (int k = sep.lo[2]; k < sep.hi[2]; ++k)
-    for (int j = sep.lo[1]; j < sep.hi[1]; ++j)
-      for (int i = sep.lo[0]; i < sep.hi[0]; ++i) seq.push_back(i + nx * (j + ny * k));
+  nd_box(left, dims, seq);
+  nd_box(right, dims, seq);
+  auto touches_outside = [&](const int p[3]) {
+    for (int e = 0; e < 3; ++e) {
+      if (p[e] == b.lo[e] && b.lo[e] > 0) return true;
+      if (p[e] == b.hi[e] - 1 && b.hi[e] < dims[e]) return true;
+    }
+    return false;
+  };
+  for (int pass = 0; pass < 2; ++pass)
+    for (int k = sep.lo[2]; k < sep.hi[2]; ++k)
+      for (int j = sep.lo[1]; j < sep.hi[1]; ++j)
+        for (int i = sep.lo[0]; i < sep.hi[0]; ++i) {
+          const int p[3] = {i, j, k};
+          if (touches_outside(p) == (pass == 0)) seq.push_back(i + nx * (j + ny * k));
+        }
 }
 
 }  // namespace
@@ -58,7 +76,8 @@ void order_grid_nd(int nx, int ny, int nz, int* order) {
   std::vector<int> seq;
   seq.reserve((size_t)nx * ny * nz);
   Box b{{0, 0, 0}, {nx, ny, nz}};
-  nd_box(b, nx, ny, seq);
+  const int dims[3] = {nx, ny, nz};
+  nd_box(b, dims, seq);
   for (size_t p = 0; p < seq.size(); ++p) order[seq[p]] = (int)p + 1;
 }
 

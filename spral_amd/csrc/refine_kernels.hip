@@ -41,10 +41,14 @@ __global__ void k_colmax(int n, const double *v, int ld, int nrhs, int slot,
                          unsigned long long *out) {
   const int k = blockIdx.y;
   if (k >= nrhs) return;
+  // NaN-propagating maximum (fmax would drop a NaN): as an ordered bit
+  // pattern a positive NaN sorts above +inf, so the atomic keeps it too
+  auto nmax = [](double a, double b) { return (b > a || b != b) ? b : a; };
   double m = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    m = fmax(m, fabs(v[(size_t)k * ld + i]));
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    m = nmax(m, fabs(v[(size_t)k * ld + i]));
+  for (int off = 32; off > 0; off >>= 1) m = nmax(m, __shfl_xor(m, off));
+  m = fabs(m);  // a NaN with the sign bit clear
   if ((threadIdx.x & 63) == 0)
     atomicMax(&out[k * 3 + slot], (unsigned long long)__double_as_longlong(m));
 }

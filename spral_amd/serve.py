@@ -201,5 +201,11 @@ class SolverService:
 
     @staticmethod
     def _footprint(p):
-        st = p.fkeep.inform
-        return int(getattr(st, "pool_peak_bytes", 0) or 0) or 8 * int(p.akeep.num_factor) * 2
+        """HBM held by a resident pattern: the factorization pool's peak
+        (factors, contribution blocks, workspaces) plus the solve workspace
+        for the largest right-hand-side count served (64)."""
+        peak = int(p.fkeep.inform.pool_peak_bytes)
+        # per column: every front's rows (the row lists, twice for the
+        # backward partial sums) and four vectors of n
+        solve_ws = 8 * 64 * (2 * len(p.akeep.rlist) + 4 * int(p.akeep.n))
+        return max(peak, 8 * int(p.akeep.num_factor)) + solve_ws

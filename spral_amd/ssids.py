@@ -95,6 +95,7 @@ class SSIDSInform:
     cuda_error: int = 0
     factor_ms: float = 0.0
     num_levels: int = 0
+    pool_peak_bytes: int = 0  # peak device pool use of the factorization
 
 
 @dataclass
@@ -235,6 +236,7 @@ def _inform_from_stats(st, akeep):
     inf.num_sup = akeep.nnodes
     inf.factor_ms = st.factor_ms
     inf.num_levels = st.num_levels
+    inf.pool_peak_bytes = st.pool_peak_bytes
     return inf
 
 
@@ -363,10 +365,16 @@ def ssids_solve(x, akeep, fkeep, options=None, job=None):
     return inf
 
 
+REFINE_WARNING_NOT_CONVERGED = 60  # a column above tol after maxit corrections
+REFINE_ERROR_NONFINITE = -60       # a NaN or infinity in b, x or the residual
+
+
 def ssids_solve_refine(x, akeep, fkeep, options=None, maxit=5, tol=1e-14):
     """Solve with FP64 iterative refinement on the device (no counterpart in
     the reference).  x (n,) or (n, nrhs) holds b on entry and the solution on
-    exit.  Returns (inform, bwd per RHS, refinement steps)."""
+    exit.  Returns (inform, bwd per RHS, refinement steps); inform.flag is
+    REFINE_WARNING_NOT_CONVERGED when some column stays above tol and
+    REFINE_ERROR_NONFINITE when a column is not finite."""
     L = _lib.lib()
     inf = SSIDSInform()
     if fkeep is None or not fkeep.numeric or fkeep.inform.flag < 0:
@@ -384,8 +392,12 @@ def ssids_solve_refine(x, akeep, fkeep, options=None, maxit=5, tol=1e-14):
     flag = L.spral_ssids_hip_solve_refine(fkeep.numeric, nrhs, b2.reshape(-1), x2.reshape(-1),
                                           n, maxit, tol, bwd, C.byref(it))
     if flag < 0:
+        # SPRAL_SSIDS_HIP_ERROR_NONFINITE (-60): a NaN or infinity in b, x or
+        # the residual; bwd of that column is NaN
         inf.flag = flag
-        return inf, None, 0
+        return inf, (bwd if flag == REFINE_ERROR_NONFINITE else None), it.value
+    if flag == REFINE_WARNING_NOT_CONVERGED:
+        inf.flag = max(inf.flag, flag)
     out = np.empty((n, nrhs))
     out[akeep.invp, :] = x2.T
     if one:

@@ -11,7 +11,8 @@ import pytest
 
 from oracle.oracle_py import OracleFactor
 from spral_amd import _lib, matrices, ssids
-from helpers import analyse, brute_inertia, doc_example, rhs
+from helpers import (analyse, brute_inertia, check_ldlt_identities, dinv_blocks, doc_example,
+                     rhs, solve_operators)
 
 pytestmark = pytest.mark.gpu
 
@@ -180,39 +181,157 @@ def test_solve_jobs_compose(lib):
     ssids.ssids_free(ak, fk)
 
 
-def test_enquire_and_alter(lib, oracle_lib):
-    m = matrices.random_indef(200, density=0.03, seed=8)
+def _gpu_ops(ak, fk, opts, n):
+    return solve_operators(lambda b, job: gpu_solve(ak, fk, opts, b, job=job), n)
+
+
+@pytest.mark.parametrize("case", ["delays0", "delays1", "kkt"])
+def test_enquire_and_solve_jobs_identities(lib, oracle_lib, case):
+    """spral_ssids_hip_enquire_indef and the four solve jobs against what
+    A = Q^T L D L^T Q fixes (helpers.check_ldlt_identities), on matrices with
+    delays and 2x2 pivots; inertia recounted from the enquired blocks."""
+    print(f"enquire identities case={case}", flush=True)
+    if case.startswith("delays"):
+        m = matrices.random_indef(90 + 40 * int(case[-1]), density=0.06, seed=40 + int(case[-1]))
+    else:
+        m = matrices.kkt_laplace(grid=5, m_con=30)
     ak, fk, finf, opts = gpu_factor(m)
+    assert finf.matrix_rank == m.n
+    if case.startswith("delays"):
+        assert finf.num_delay > 0 and finf.num_two > 0
     piv, d, inf = ssids.ssids_enquire_indef(ak, fk, opts)
     assert inf.flag >= 0
-    # piv_order: a signed permutation of 1..n, 2x2 members negative in pairs
-    assert np.array_equal(np.sort(np.abs(piv)), np.arange(1, m.n + 1))
-    assert int((piv < 0).sum()) == 2 * finf.num_two
-    # inertia recomputed from the enquired D^{-1}
-    neg, i = 0, 0
-    while i < m.n:
-        if d[1, i] != 0.0 and i + 1 < m.n:
-            a, b_, c = d[0, i], d[1, i], d[0, i + 1]
-            det, tr = a * c - b_ * b_, a + c
-            neg += 1 if det < 0 else (2 if tr < 0 else 0)
-            i += 2
-        else:
-            neg += 1 if d[0, i] < 0 else 0
-            i += 1
-    assert neg == finf.num_neg
-    xt, b = rhs(m, 1, 4)
-    x0 = gpu_solve(ak, fk, opts, b)
-    assert ssids.ssids_alter(d, ak, fk, opts).flag == 0
-    x1 = gpu_solve(ak, fk, opts, b)
-    assert np.array_equal(x0, x1)
-    d2 = d.copy()
-    d2[0, :] *= 2.0
-    d2[1, :] *= 2.0
-    ssids.ssids_alter(d2, ak, fk, opts)
-    y = gpu_solve(ak, fk, opts, b, job=ssids.SSIDS_SOLVE_JOB_DIAG)
-    y0 = gpu_solve(ak, fk, opts, b, job=ssids.SSIDS_SOLVE_JOB_DIAG)
-    assert np.allclose(y, y0)
+    check_ldlt_identities(m.full.toarray(), piv, d, _gpu_ops(ak, fk, opts, m.n))
+    ev = np.linalg.eigvalsh(dinv_blocks(piv, d))
+    assert int((ev < 0).sum()) == finf.num_neg
     ssids.ssids_free(ak, fk)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_enquire_matches_oracle_tpp(lib, oracle_lib, seed):
+    """With pivot_method = TPP and no delayed pivots the GPU takes the
+    oracle's pivot decisions: piv_order identical, D^{-1} equal to 1e-12."""
+    print(f"enquire vs oracle seed={seed}", flush=True)
+    m = matrices.random_indef(150 + 30 * seed, density=0.04, seed=60 + seed, big_last=False,
+                              zero_diag_frac=0.0)
+    ak, fk, finf, opts = gpu_factor(m, pivot_method=3)
+    of = OracleFactor(ak, m.val)
+    assert of.stats["num_delay"] == 0 and finf.num_delay == 0
+    assert of.stats["num_two"] > 0
+    piv, d, inf = ssids.ssids_enquire_indef(ak, fk, opts)
+    po, do = of.enquire()
+    assert np.array_equal(piv, po)
+    assert np.abs(d - do).max() <= FACTOR_TOL * max(1.0, np.abs(do).max())
+    ssids.ssids_free(ak, fk)
+
+
+def test_alter_scales_diag_solve(lib):
+    """After alter(2 D^{-1}) the diagonal solve returns twice the result of
+    the unaltered factors, and alter(D^{-1}) restores the full solve bitwise."""
+    m = matrices.random_indef(200, density=0.03, seed=8)
+    ak, fk, finf, opts = gpu_factor(m)
+    assert finf.num_two > 0
+    piv, d, inf = ssids.ssids_enquire_indef(ak, fk, opts)
+    xt, b = rhs(m, 2, 4)
+    y0 = gpu_solve(ak, fk, opts, b, job=ssids.SSIDS_SOLVE_JOB_DIAG)
+    x0 = gpu_solve(ak, fk, opts, b)
+    assert ssids.ssids_alter(2.0 * d, ak, fk, opts).flag == 0
+    y1 = gpu_solve(ak, fk, opts, b, job=ssids.SSIDS_SOLVE_JOB_DIAG)
+    assert np.array_equal(y1, 2.0 * y0)
+    piv2, d2, _ = ssids.ssids_enquire_indef(ak, fk, opts)
+    assert np.array_equal(piv2, piv) and np.array_equal(d2, 2.0 * d)
+    assert ssids.ssids_alter(d, ak, fk, opts).flag == 0
+    assert np.array_equal(gpu_solve(ak, fk, opts, b), x0)
+    ssids.ssids_free(ak, fk)
+
+
+def _subtree_of(ak, root):
+    """First node of the subtree rooted at `root` (0-based): the subtree is
+    the contiguous range [first, root] of the analyse's postorder."""
+    par = ak.sparent - 1
+
+    def below(s):
+        while 0 <= s < root:
+            s = par[s]
+        return s == root
+
+    first = root
+    while first > 0 and below(first - 1):
+        first -= 1
+    return first
+
+
+@pytest.mark.parametrize("case", ["spd", "indef"])
+def test_get_contrib_of_subtree(lib, case):
+    """A proper subtree factored alone exports the Schur update of its
+    eliminated variables onto the root's remaining rows:
+    contrib = -A[R,V] A[V,V]^{-1} A[V,R] over the entries the subtree
+    assembles (columns in V), as gpu_numeric_subtree%get_contrib
+    (src/ssids/gpu/subtree.f90:522-538) hands to the parent subtree."""
+    import ctypes as C
+    print(f"get_contrib case={case}", flush=True)
+    if case == "spd":
+        m = matrices.laplace3d(8, diag=6.5)
+        ak, inf = analyse(m, grid=(8, 8, 8), use_gpu=False)
+    else:
+        m = matrices.random_indef(160, density=0.04, seed=70, big_last=False)
+        ak, inf = analyse(m, use_gpu=False)
+    par = ak.sparent - 1
+    nn = ak.nnodes
+    # the largest proper subtree whose root has a contribution block
+    sizes = np.zeros(nn, np.int64)
+    for s in range(nn):
+        sizes[s] += ak.sptr[s + 1] - ak.sptr[s]
+        if 0 <= par[s] < nn:
+            sizes[par[s]] += sizes[s]
+    cand = [s for s in range(nn) if 0 <= par[s] < nn
+            and ak.rptr[s + 1] - ak.rptr[s] > ak.sptr[s + 1] - ak.sptr[s]]
+    root = max(cand, key=lambda s: sizes[s])
+    first = _subtree_of(ak, root)
+    L = _lib.lib()
+    opts = ssids.SSIDSOptions()
+    copt = opts.to_c()
+    flag = C.c_int(0)
+    symb = L.spral_ssids_hip_create_symbolic(0, ak.n, first + 1, root + 2, ak.sptr, ak.sparent,
+                                             ak.rptr, ak.rlist, ak.nptr, ak.nlist,
+                                             C.byref(copt), C.byref(flag))
+    assert symb and flag.value == 0
+    st = _lib.Stats()
+    num = L.spral_ssids_hip_factor(False, symb, m.val, m.val.size, None, C.byref(copt),
+                                   C.byref(st))
+    assert num and st.flag >= 0, st.flag
+    cn, ldv, nd, ldd = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    val, rl, dperm, dval = (C.POINTER(C.c_double)(), C.POINTER(C.c_int)(),
+                            C.POINTER(C.c_int)(), C.POINTER(C.c_double)())
+    L.spral_ssids_hip_get_contrib(num, C.byref(cn), C.byref(val), C.byref(ldv), C.byref(rl),
+                                  C.byref(nd), C.byref(dperm), C.byref(dval), C.byref(ldd))
+    assert nd.value == 0  # no delays leave this subtree
+    n_c = cn.value
+    assert n_c == (ak.rptr[root + 1] - ak.rptr[root]) - (ak.sptr[root + 1] - ak.sptr[root])
+    got = np.ctypeslib.as_array(val, shape=(n_c * ldv.value,)).reshape(n_c, ldv.value)[:, :n_c]
+    R = np.ctypeslib.as_array(rl, shape=(n_c,)).copy() - 1  # elimination indices
+    # the entries the subtree assembles: columns V = its eliminated variables
+    V = np.arange(ak.sptr[first] - 1, ak.sptr[root + 1] - 1)
+    idx = {int(v): k for k, v in enumerate(np.concatenate([V, R]))}
+    M = np.zeros((len(idx), len(idx)))
+    for s in range(first, root + 1):
+        blkm = ak.rptr[s + 1] - ak.rptr[s]
+        rows = ak.rlist[ak.rptr[s] - 1:ak.rptr[s + 1] - 1] - 1
+        for e in range(ak.nptr[s] - 1, ak.nptr[s + 1] - 1):
+            src, dst = ak.nlist[2 * e] - 1, ak.nlist[2 * e + 1] - 1
+            c, r = divmod(dst, blkm)
+            i, j = idx[int(rows[r])], idx[int(ak.sptr[s] - 1 + c)]
+            M[i, j] += m.val[src]
+            if i != j:
+                M[j, i] += m.val[src]
+    nv = len(V)
+    want = -M[nv:, :nv] @ np.linalg.solve(M[:nv, :nv], M[:nv, nv:])
+    assert np.abs(got - got.T).max() == 0.0
+    assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
+    L.spral_ssids_hip_free_contrib(num)
+    L.spral_ssids_hip_destroy_numeric(num)
+    L.spral_ssids_hip_destroy_symbolic(symb)
+    ssids.ssids_free(ak)
 
 
 def test_doc_example(lib):

@@ -6,7 +6,8 @@ import pytest
 
 from oracle.oracle_py import OracleFactor
 from spral_amd import matrices
-from helpers import analyse, brute_inertia, doc_example, rhs
+from helpers import (analyse, brute_inertia, check_ldlt_identities, dinv_blocks, doc_example,
+                     rhs, solve_operators)
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -96,3 +97,59 @@ def test_planted_mistake_is_caught(lib, oracle_lib):
     xt, b = rhs(m)
     x = of.solve(b)
     assert matrices.bwd_error(m.full, x, b).max() > 1e-10
+
+
+def _oracle_ops(of, n):
+    return solve_operators(lambda b, job: of.solve(b, job=job), n)
+
+
+@pytest.mark.parametrize("case", ["delays0", "delays1", "kkt", "doc"])
+def test_enquire_and_solve_jobs_pinned(lib, oracle_lib, case):
+    """oracle_enquire and oracle_solve jobs 1-4 against what A = Q^T L D L^T Q
+    fixes (helpers.check_ldlt_identities): the permuted forward operator is
+    unit lower triangular, the diagonal job is exactly the enquired D^{-1},
+    F A F^T = D, bwd = F^T, diag_bwd = F^T D^{-1}, all = A^{-1}."""
+    if case.startswith("delays"):
+        m = matrices.random_indef(90 + 40 * int(case[-1]), density=0.06, seed=40 + int(case[-1]))
+    elif case == "kkt":
+        m = matrices.kkt_laplace(grid=5, m_con=30)
+    else:
+        m, _ = doc_example()
+    ak, _ = analyse(m)
+    of = OracleFactor(ak, m.val)
+    assert of.stats["matrix_rank"] == m.n
+    if case.startswith("delays"):
+        assert of.stats["num_delay"] > 0 and of.stats["num_two"] > 0
+    piv, d = of.enquire()
+    A = m.full.toarray()
+    check_ldlt_identities(A, piv, d, _oracle_ops(of, m.n))
+    # inertia from the enquired blocks equals the factorization's count
+    ev = np.linalg.eigvalsh(dinv_blocks(piv, d))
+    assert int((ev < 0).sum()) == of.stats["num_neg"]
+
+
+def test_identity_check_catches_planted_mistakes(lib, oracle_lib):
+    """Each plausible mistake in what enquire or a solve job returns breaks
+    one of the identities: a wrong 2x2 off-diagonal sign, a swapped pair of
+    pivot positions, a transposed forward operator."""
+    m = matrices.random_indef(90, density=0.06, seed=40)
+    ak, _ = analyse(m)
+    of = OracleFactor(ak, m.val)
+    piv, d = of.enquire()
+    A = m.full.toarray()
+    ops = _oracle_ops(of, m.n)
+    check_ldlt_identities(A, piv, d, ops)
+    k = int(np.nonzero(d[1] != 0.0)[0][0])
+    bad = d.copy()
+    bad[1, k] = -bad[1, k]
+    with pytest.raises(AssertionError):
+        check_ldlt_identities(A, piv, bad, ops)
+    ones = np.nonzero(piv > 0)[0]
+    sw = piv.copy()
+    sw[ones[0]], sw[ones[1]] = piv[ones[1]], piv[ones[0]]
+    with pytest.raises(AssertionError):
+        check_ldlt_identities(A, sw, d, ops)
+    tr = dict(ops)
+    tr[1] = ops[1].T.copy()
+    with pytest.raises(AssertionError):
+        check_ldlt_identities(A, piv, d, tr)
