@@ -37,7 +37,10 @@ def _p(a):
 class OracleFactor:
     """Factorize with the oracle given the symbolic arrays of an analyse."""
 
-    def __init__(self, akeep, val, scaling=None, action=True, u=0.01, small=1e-20):
+    # u defaults to the reference's value: the single-precision literal 0.01
+    # widened to double (src/ssids/datatypes.f90:262)
+    def __init__(self, akeep, val, scaling=None, action=True, u=float(np.float32(0.01)),
+                 small=1e-20):
         L = lib()
         self.akeep = akeep
         self.val = np.ascontiguousarray(val, np.float64)

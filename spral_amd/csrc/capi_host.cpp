@@ -16,8 +16,10 @@ void spral_ssids_hip_default_options(struct spral_ssids_hip_options *opt) {
   opt->print_level = 0;
   opt->action = 1;
   opt->small = 1e-20;
-  opt->u = 0.01;
-  opt->multiplier = 1.1;
+  // u and multiplier are default-real (single precision) literals in the
+  // reference, widened to double: 0.01f and 1.1f, not 0.01 and 1.1
+  opt->u = (double)0.01f;
+  opt->multiplier = (double)1.1f;
   opt->pivot_method = 2;
   opt->failed_pivot_method = 1;
   opt->schur_precision = 0;

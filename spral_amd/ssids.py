@@ -56,8 +56,10 @@ class SSIDSOptions:
     action: bool = True
     pivot_method: int = 2
     small: float = 1e-20
-    u: float = 0.01
-    multiplier: float = 1.1
+    # default-real literals in the reference (src/ssids/datatypes.f90:262,268):
+    # single-precision 0.01 and 1.1 widened to double
+    u: float = float(np.float32(0.01))
+    multiplier: float = float(np.float32(1.1))
     failed_pivot_method: int = 1
     scaling: int = 0
     use_gpu: bool = True
