@@ -65,12 +65,22 @@ struct SolveNode {
   const int *ipos;   // pre-pivot -> post-pivot position (n entries)
   const int *rows;   // symbolic rows
   const int *rmap;
-  double *X;         // m x nrhs workspace (ld m)
-  double *P;         // backward GEMV partial sums (gemv workgroups x nrhs x nelim)
+  // The solve workspace of a call with nrhs right-hand sides is one array ws
+  // holding, per front, X (m x nrhs, ld m) and then the backward partial
+  // sums P (bwd row workgroups x nrhs x nelim): X = ws + nrhs * woff and
+  // P = X + nrhs * m, so the descriptors do not depend on nrhs.
+  int64_t woff;
   const int *ctab;   // per child, per 64-row tile of this front: first child
                      // trailing index whose position reaches the tile (stride T+1)
   int m, n, nd, sn, nelim, ldl, doff, cbeg, cend;
 };
+
+__host__ __device__ inline double *solve_X(double *ws, const SolveNode &Q, int nrhs) {
+  return ws + (size_t)nrhs * Q.woff;
+}
+__host__ __device__ inline double *solve_P(double *ws, const SolveNode &Q, int nrhs) {
+  return ws + (size_t)nrhs * Q.woff + (size_t)nrhs * Q.m;
+}
 
 // Global-address-space views of device pointers read from descriptors: loads
 // and stores through them compile to global (not flat) memory instructions.

@@ -45,17 +45,17 @@ void launch_pack_nodes(hipStream_t, const NodeDev *, int, int *);
 void launch_pack_dperm(hipStream_t, const NodeDev *, const int *, const int64_t *, int, double *,
                        int *);
 void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int *, double *,
-                    int, int);
+                    int, int, double *);
 void launch_fwd_rows(hipStream_t, int, const SolveNode *, const int *, const int *, int,
-                     const int *, int);
-void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
-void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int);
+                     const int *, int, double *);
+void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
+void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
 int big_tri_threshold();
 int fwd_rows_per_wg();
 int bwd_rows_per_wg();
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
-                const int *, int);
+                const int *, int, double *);
 
 void launch_residual(hipStream_t, int, const int64_t *, const int *, const int64_t *,
                      const double *, const double *, const double *, double *, int, int);
@@ -263,11 +263,11 @@ struct Numeric {
   std::vector<int> bwd_count;                 // per slot: backward row workgroups
   std::vector<std::vector<int>> big_of_level;  // slots whose triangle needs the launch sequence
   std::vector<SolveNode> sn_h;
-  double *d_X = nullptr;
+  double *d_X = nullptr;                  // solve workspace (solve_ws_per_rhs x nrhs)
   size_t d_X_cap = 0;
+  int64_t solve_ws_per_rhs = 0;           // doubles of workspace per right-hand side
   double *d_x = nullptr;
   size_t d_x_cap = 0;
-  int solve_nrhs_cap = 0;
   bool posdef = false;                    // A = L L^T (pivots in order, all > 0)
   spral_ssids_hip_stats stats{};
   double *d_aval = nullptr;               // copy of the factorized values
@@ -1099,49 +1099,48 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       N.bwd_total[L] = acc;
     }
     if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
+    N.d_gemv_tab = nullptr;
     HIPCHK(hipMalloc(&N.d_gemv_tab, std::max<size_t>(tab.size(), 1) * sizeof(int)));
     HIPCHK(hipMemcpy(N.d_gemv_tab, tab.data(), tab.size() * sizeof(int),
                      hipMemcpyHostToDevice));
   }
+  // per front, per right-hand side: X (m) and the backward partial sums
+  // (row workgroups x nelim, at least nelim: big fronts keep their
+  // triangle's solution there)
+  {
+    int64_t off = 0;
+    for (int sl = 0; sl < nn; ++sl) {
+      N.sn_h[sl].woff = off;
+      off += (int64_t)N.nodes[sl].m +
+             (int64_t)std::max(N.bwd_count[sl], 1) * std::max(N.nodes[sl].nelim, 1);
+    }
+    N.solve_ws_per_rhs = std::max<int64_t>(off, 1);
+  }
+  HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), nn * sizeof(SolveNode), hipMemcpyHostToDevice));
   N.big_of_level.assign(S.lvl_ptr.size() - 1, {});
   for (int L = 0; L + 1 < (int)S.lvl_ptr.size(); ++L)
     for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl)
       if (N.nodes[sl].nelim > big_tri_threshold()) N.big_of_level[L].push_back(sl);
-  N.solve_nrhs_cap = 0;  // X pointers need refreshing
 }
 
-static void ensure_solve_ws(Numeric &N, int nrhs) {
-  Symbolic &S = *N.symb;
-  if (nrhs <= N.solve_nrhs_cap) return;
-  // per front: X (m x nrhs), then the backward partial sums (row workgroups x
-  // nrhs x nelim), at least nrhs x nelim: big fronts keep their triangle's
-  // solution there
-  auto part_size = [&](int sl) {
-    return (size_t)std::max(N.bwd_count[sl], 1) * nrhs * std::max(N.nodes[sl].nelim, 1);
-  };
-  size_t tot = 0;
-  for (int sl = 0; sl < S.nnodes; ++sl) tot += (size_t)N.nodes[sl].m * nrhs + part_size(sl);
-  if (tot > N.d_X_cap) {
+// The handle's own solve workspace, grown to nrhs right-hand sides.
+static double *solve_ws(Numeric &N, int nrhs) {
+  const size_t need = (size_t)N.solve_ws_per_rhs * nrhs;
+  if (need > N.d_X_cap) {
     if (N.d_X) HIPCHK(hipFree(N.d_X));
-    HIPCHK(hipMalloc(&N.d_X, std::max(tot, size_t(1)) * sizeof(double)));
-    N.d_X_cap = tot;
+    N.d_X = nullptr;
+    N.d_X_cap = 0;
+    HIPCHK(hipMalloc(&N.d_X, need * sizeof(double)));
+    N.d_X_cap = need;
   }
-  size_t offx = 0;
-  for (int sl = 0; sl < S.nnodes; ++sl) {
-    N.sn_h[sl].X = N.d_X + offx;
-    offx += (size_t)N.nodes[sl].m * nrhs;
-    N.sn_h[sl].P = N.d_X + offx;
-    offx += part_size(sl);
-  }
-  HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), S.nnodes * sizeof(SolveNode),
-                   hipMemcpyHostToDevice));
-  N.solve_nrhs_cap = nrhs;
+  return N.d_X;
 }
 
-static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
-                         hipStream_t st) {
+// One solve job on device-resident x with workspace ws (at least
+// solve_ws_per_rhs x nrhs doubles).  Launches only: no allocation.
+static void solve_launch(Numeric &N, int job, int nrhs, double *d_x, int ldx, hipStream_t st,
+                         double *ws) {
   Symbolic &S = *N.symb;
-  ensure_solve_ws(N, nrhs);
   const int nlev = (int)S.lvl_ptr.size() - 1;
   // positive definite, as the CPU subtree (src/ssids/cpu/NumericSubtree.hxx:
   // 342-343): the diagonal solve alone is a no-op, diag + bwd is bwd
@@ -1157,11 +1156,11 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   if (fwd)
     for (int L = 0; L < nlev; ++L) {
       const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
-      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs);
+      launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs, ws);
       for (int sl : N.big_of_level[L])
-        launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
+        launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, ws);
       launch_fwd_rows(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, nrhs,
-                      N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L]);
+                      N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L], ws);
     }
   if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 0);
   if (half_after_fwd || half_before_bwd)
@@ -1169,11 +1168,16 @@ static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
   if (bwd)
     for (int L = nlev - 1; L >= 0; --L) {
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
-                 ldx, nrhs, N.d_gemv_tab + N.bwd_tab_off[L], N.bwd_total[L]);
+                 ldx, nrhs, N.d_gemv_tab + N.bwd_tab_off[L], N.bwd_total[L], ws);
       for (int sl : N.big_of_level[L])
-        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs);
+        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, ws);
     }
   HIPCHK(hipGetLastError());
+}
+
+static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
+                         hipStream_t st) {
+  solve_launch(N, job, nrhs, d_x, ldx, st, solve_ws(N, nrhs));
 }
 
 static int solve_host(Numeric &N, int job, int nrhs, double *x, int ldx) {

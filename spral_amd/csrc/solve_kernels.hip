@@ -2,7 +2,10 @@
 //
 // Positions of a front (after pivoting): [0, ne) eliminated, [ne, n) failed
 // fully-summed columns passed on to the parent, [n, m) the rows below.  X
-// (m x nrhs per front, ld m) holds the forward values of the front's positions.
+// (m x nrhs per front, ld m) holds the forward values of the front's positions;
+// X and the backward partial sums live in the call's workspace ws (solve_X,
+// solve_P).  Every kernel writes what it reads of ws earlier in the same
+// solve, so ws needs no clearing between calls.
 //
 // Forward (levels leaves -> root), two launches per level:
 //   k_fwd_tri : one workgroup per front: X[0:n] = own b + the children's
@@ -126,7 +129,7 @@ __device__ inline void stage_lower(gcdouble *PL, int ldl, int j0, int wb,
 // then the triangle X[0:ne] <- L11^{-1} X[0:ne] when ne <= kBigTri.
 __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const int *slots,
                                                       const int *child_list, double *x,
-                                                      int ldx, int nrhs) {
+                                                      int ldx, int nrhs, double *ws) {
   __shared__ double xs[kTriCap];
   __shared__ double yb[kBlk];
   __shared__ int nchunk[kGroups];
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
   gcint *Pipos = (gcint *)P.ipos;
   gcint *Prows = (gcint *)P.rows;
   gcint *Pctab = (gcint *)P.ctab;
-  gdouble *PX = (gdouble *)P.X;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
   const int cbeg = P.cbeg, cend = P.cend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -161,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
         cdoff = c.doff;
         cm = c.m;
         crmap = (gcint *)c.rmap;
-        cX = (gcdouble *)c.X;
+        cX = (gcdouble *)solve_X(ws, c, nrhs);
         gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
         kb = tab[0];
         ke = tab[Tn];
@@ -234,7 +237,7 @@ __device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "
 
 __global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, const int *slots,
                                                          const int *child_list, double *x,
-                                                         int ldx, int nrhs) {
+                                                         int ldx, int nrhs, double *ws) {
   __shared__ double xs[kGroups][kTriCap];
   __shared__ double ybs[kGroups][kBlk];
   const SolveNode &P = sn[slots[blockIdx.x]];
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, co
   gcint *Pipos = (gcint *)P.ipos;
   gcint *Prows = (gcint *)P.rows;
   gcint *Pctab = (gcint *)P.ctab;
-  gdouble *PX = (gdouble *)P.X;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
   const int cbeg = P.cbeg, cend = P.cend;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -276,7 +279,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, co
           const int k = kb + it * 64 + lane;
           if (k < ke) {
             const int pre = (k < c.n) ? c.doff + (k - c.nelim) : nd + c.rmap[k - c.n];
-            if (pre < n) acc[Pipos[pre]] += c.X[(size_t)r * c.m + k];
+            if (pre < n) acc[Pipos[pre]] += solve_X(ws, c, nrhs)[(size_t)r * c.m + k];
           }
           wave_fence();
         }
@@ -322,7 +325,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, co
 template <int RC>
 __global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, const int *slots,
                                                        const int *child_list,
-                                                       const int *wg_off, int nlvl, int nrhs) {
+                                                       const int *wg_off, int nlvl, int nrhs,
+                                                       double *ws) {
   __shared__ double xs[RC][kRows];
   __shared__ double ys[RC][kYChunk];
   __shared__ double red[kGroups][RC][kRows];
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, cons
   if (r0 >= r1) return;  // front without rows below
   gcdouble *PL = (gcdouble *)P.L;
   gcint *Pctab = (gcint *)P.ctab;
-  gdouble *PX = (gdouble *)P.X;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
   const int ldl = P.ldl, n = P.n, nd = P.nd, cbeg = P.cbeg, cend = P.cend;
   const int tid = threadIdx.x, lane = tid & 63, rr = lane, g = tid >> 6;
   const int row = imin(r0 + rr, r1 - 1);
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, cons
           const SolveNode &c = sn[child_list[ci]];
           const int cn = c.n, cne = c.nelim, cdoff = c.doff, cm = c.m;
           gcint *crmap = (gcint *)c.rmap;
-          gcdouble *cX = (gcdouble *)c.X;
+          gcdouble *cX = (gcdouble *)solve_X(ws, c, nrhs);
           gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
           const int kb = tab[I0], ke = tab[I1];
 #pragma unroll
@@ -462,14 +466,15 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
 template <int RC>
 __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, const int *slots,
                                                        const int *wg_off, int nlvl,
-                                                       const double *x, int ldx, int nrhs) {
+                                                       const double *x, int ldx, int nrhs,
+                                                       double *ws) {
   __shared__ double red[kGroups][RC][kBlk];
   const int li = find_node_s(wg_off, nlvl, blockIdx.x);
   const SolveNode &P = sn[slots[li]];
   gcdouble *PL = (gcdouble *)P.L;
   gcint *Pperm = (gcint *)P.perm;
   gcint *Prows = (gcint *)P.rows;
-  gdouble *part = (gdouble *)P.P;  // count x nrhs x ne
+  gdouble *part = (gdouble *)solve_P(ws, P, nrhs);  // count x nrhs x ne
   const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
   const int local = blockIdx.x - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -513,7 +518,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
 // L11^T X = X, 64-column blocks from the last, and x[perm[j]] = X[j].
 __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const int *slots,
                                                       const int *wg_off, double *x, int ldx,
-                                                      int nrhs) {
+                                                      int nrhs, double *ws) {
   __shared__ double xt[kBigTri];
   __shared__ double lb[kBlk][kBlk + 1];
   __shared__ double red[kGroups][kBlk];
@@ -522,8 +527,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
   const SolveNode &P = sn[slots[li]];
   gcdouble *PL = (gcdouble *)P.L;
   gcint *Pperm = (gcint *)P.perm;
-  gdouble *PX = (gdouble *)P.X;
-  gcdouble *part = (gcdouble *)P.P;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
+  gcdouble *part = (gcdouble *)solve_P(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, ne = P.nelim;
   const int count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -591,15 +596,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
 // k_bwd_tri gives.
 __global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, const int *slots,
                                                          const int *wg_off, double *x,
-                                                         int ldx, int nrhs) {
+                                                         int ldx, int nrhs, double *ws) {
   __shared__ double xts[kGroups][kBigTri];
   __shared__ double lb[kBlk][kBlk + 1];
   const int li = blockIdx.x;
   const SolveNode &P = sn[slots[li]];
   gcdouble *PL = (gcdouble *)P.L;
   gcint *Pperm = (gcint *)P.perm;
-  gdouble *PX = (gdouble *)P.X;
-  gcdouble *part = (gcdouble *)P.P;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
+  gcdouble *part = (gcdouble *)solve_P(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, ne = P.nelim;
   const int count = wg_off[li + 1] - wg_off[li];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -671,13 +676,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, co
 // the triangle's rows [j1 + 64q, ...) with the solution, and workgroup 0 keeps
 // the solution in Y (the front's partial-sum area, nrhs x ne).
 __global__ __launch_bounds__(kThreads) void k_big_fwd(const SolveNode *sn, int s, int j0,
-                                                      int nrhs) {
+                                                      int nrhs, double *ws) {
   __shared__ double ys[kRhsChunk][kBlk];
   __shared__ double red[kGroups][kRhsChunk][kRows];
   const SolveNode &P = sn[s];
   gcdouble *PL = (gcdouble *)P.L;
-  gdouble *PX = (gdouble *)P.X;
-  gdouble *Y = (gdouble *)P.P;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
+  gdouble *Y = (gdouble *)solve_P(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
   const int tid = threadIdx.x, lane = tid & 63, rr = lane, g = tid >> 6, wave = g;
   const int r0 = j1 + blockIdx.x * kRows, r1 = imin(ne, r0 + kRows);
@@ -722,11 +727,12 @@ __global__ __launch_bounds__(kThreads) void k_big_fwd(const SolveNode *sn, int s
 }
 
 // X[0:ne] = Y and x[perm[i]] = Y[i] after the last forward block
-__global__ void k_big_fwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
+__global__ void k_big_fwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs,
+                              double *ws) {
   const SolveNode &P = sn[s];
   gcint *Pperm = (gcint *)P.perm;
-  gdouble *PX = (gdouble *)P.X;
-  gcdouble *Y = (gcdouble *)P.P;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
+  gcdouble *Y = (gcdouble *)solve_P(ws, P, nrhs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim, m = P.m;
   if (i >= ne) return;
   const int v = Pperm[i];
@@ -743,13 +749,13 @@ __global__ void k_big_fwd_fin(const SolveNode *sn, int s, double *x, int ldx, in
 // the columns [64q, 64q + 64) of X[0:j0]:  X[c] -= L[j0:j1, c]^T x_b.
 // Workgroup 0 keeps the solution in Y (the front's partial-sum area).
 __global__ __launch_bounds__(kThreads) void k_big_bwd(const SolveNode *sn, int s, int j0,
-                                                      int nrhs) {
+                                                      int nrhs, double *ws) {
   __shared__ double lb[kBlk][kBlk + 1];
   __shared__ double xb[kRhsChunk][kBlk];
   const SolveNode &P = sn[s];
   gcdouble *PL = (gcdouble *)P.L;
-  gdouble *PX = (gdouble *)P.X;
-  gdouble *Y = (gdouble *)P.P;
+  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
+  gdouble *Y = (gdouble *)solve_P(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
   const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6, wave = g;
   const int c0 = blockIdx.x * kBlk, c1 = imin(j0, c0 + kBlk);  // columns of X to update
@@ -788,10 +794,11 @@ __global__ __launch_bounds__(kThreads) void k_big_bwd(const SolveNode *sn, int s
 }
 
 // x[perm[i]] = Y[i] after the first block
-__global__ void k_big_bwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs) {
+__global__ void k_big_bwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs,
+                              double *ws) {
   const SolveNode &P = sn[s];
   gcint *Pperm = (gcint *)P.perm;
-  gcdouble *Y = (gcdouble *)P.P;
+  gcdouble *Y = (gcdouble *)solve_P(ws, P, nrhs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim;
   if (i >= ne) return;
   const int v = Pperm[i];
@@ -804,43 +811,43 @@ int fwd_rows_per_wg() { return kRows; }
 int bwd_rows_per_wg() { return kThreads; }
 
 void launch_big_fwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
-                    int nrhs) {
+                    int nrhs, double *ws) {
   for (int j0 = 0; j0 < ne; j0 += kBlk) {
     const int rows = ne - std::min(j0 + kBlk, ne);
     hipLaunchKernelGGL(k_big_fwd, dim3(std::max(1, (rows + kRows - 1) / kRows)),
-                       dim3(kThreads), 0, st, sn, s, j0, nrhs);
+                       dim3(kThreads), 0, st, sn, s, j0, nrhs, ws);
   }
   hipLaunchKernelGGL(k_big_fwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
-                     nrhs);
+                     nrhs, ws);
 }
 void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
-                    int nrhs) {
+                    int nrhs, double *ws) {
   for (int j0 = (ne - 1) / kBlk * kBlk; j0 >= 0; j0 -= kBlk)
     hipLaunchKernelGGL(k_big_bwd, dim3(std::max(1, (j0 + kBlk - 1) / kBlk)), dim3(kThreads), 0,
-                       st, sn, s, j0, nrhs);
+                       st, sn, s, j0, nrhs, ws);
   hipLaunchKernelGGL(k_big_bwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
-                     nrhs);
+                     nrhs, ws);
 }
 
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                    const int *child_list, double *x, int ldx, int nrhs) {
+                    const int *child_list, double *x, int ldx, int nrhs, double *ws) {
   if (nlvl <= 0) return;
   if (nrhs == 1)
     hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
-                       ldx, nrhs);
+                       ldx, nrhs, ws);
   else  // a wave per right-hand side: the same sums, no barrier per right-hand side
     hipLaunchKernelGGL(k_fwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list,
-                       x, ldx, nrhs);
+                       x, ldx, nrhs, ws);
 }
 void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
-                     const int *child_list, int nrhs, const int *wg_off, int nwg) {
+                     const int *child_list, int nrhs, const int *wg_off, int nwg, double *ws) {
   if (nlvl <= 0 || nwg <= 0) return;
   if (nrhs == 1)
     hipLaunchKernelGGL(k_fwd_rows<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots, child_list,
-                       wg_off, nlvl, nrhs);
+                       wg_off, nlvl, nrhs, ws);
   else
     hipLaunchKernelGGL(k_fwd_rows<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
-                       child_list, wg_off, nlvl, nrhs);
+                       child_list, wg_off, nlvl, nrhs, ws);
 }
 void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, int ldx,
                    int nrhs, int half) {
@@ -850,22 +857,22 @@ void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, i
 }
 // rows_off / nrows: the level's 256-row workgroup table and its total
 void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots, double *x,
-                int ldx, int nrhs, const int *rows_off, int nrows) {
+                int ldx, int nrhs, const int *rows_off, int nrows, double *ws) {
   if (nlvl <= 0) return;
   if (nrows > 0) {
     if (nrhs == 1)
       hipLaunchKernelGGL(k_bwd_rows<1>, dim3(nrows), dim3(kThreads), 0, st, sn, slots, rows_off,
-                         nlvl, x, ldx, nrhs);
+                         nlvl, x, ldx, nrhs, ws);
     else
       hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nrows), dim3(kThreads), 0, st, sn, slots,
-                         rows_off, nlvl, x, ldx, nrhs);
+                         rows_off, nlvl, x, ldx, nrhs, ws);
   }
   if (nrhs == 1)
     hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs);
+                       ldx, nrhs, ws);
   else  // four right-hand sides at a time, one a wave
     hipLaunchKernelGGL(k_bwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs);
+                       ldx, nrhs, ws);
 }
 
 }  // namespace ssids_hip
