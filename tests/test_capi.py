@@ -28,6 +28,9 @@ def test_library_exports_all_declared(lib):
 def test_default_options(lib):
     from spral_amd import _lib
     o = _lib.default_options()
-    # src/ssids/datatypes.f90:188-284
-    assert o.u == 0.01 and o.small == 1e-20 and o.pivot_method == 2
-    assert o.failed_pivot_method == 1 and o.action == 1 and abs(o.multiplier - 1.1) < 1e-15
+    # src/ssids/datatypes.f90:188-284; u and multiplier are default-real
+    # literals there, single precision widened to double
+    import numpy as np
+    assert o.u == float(np.float32(0.01)) and o.small == 1e-20 and o.pivot_method == 2
+    assert o.failed_pivot_method == 1 and o.action == 1
+    assert o.multiplier == float(np.float32(1.1))
