@@ -360,6 +360,7 @@ def main():
         solve()
         sync()
         lat.append((time.perf_counter() - t0) * 1e3)
+    fp64_steps = its.value  # the mixed solves below reuse `its`
     x = d_x.download()
     xo = np.empty_like(x)
     xo[ak.invp] = x
@@ -439,7 +440,7 @@ def main():
         "solve_ms_p50": round(float(np.percentile(lat, 50)), 3),
         "solve_ms_p99": round(float(np.percentile(lat, 99)), 3),
         "bwd_err": bwd,
-        "refine_steps": its.value,
+        "refine_steps": fp64_steps,
         "num_neg": s.num_neg,
         "num_delay": s.num_delay,
         "num_two": s.num_two,

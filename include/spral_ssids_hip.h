@@ -213,11 +213,12 @@ int spral_ssids_hip_enquire_posdef(const void *numeric, double *d);
 /* Contribution block of the subtree root for a parent subtree held elsewhere;
  * same out-parameters as spral_ssids_cpu_subtree_get_contrib_dbl
  * (src/ssids/cpu/NumericSubtree.cxx:233-261).  Host memory owned by the
- * numeric handle until spral_ssids_hip_free_contrib. */
-void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val,
-                                 int *ldval, const int **rlist, int *ndelay,
-                                 const int **delay_perm,
-                                 const double **delay_val, int *lddelay);
+ * numeric handle until spral_ssids_hip_free_contrib.  Returns 0, or -51
+ * (device error) / -50 (host allocation) with every out-parameter empty. */
+int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val,
+                                int *ldval, const int **rlist, int *ndelay,
+                                const int **delay_perm,
+                                const double **delay_val, int *lddelay);
 void spral_ssids_hip_free_contrib(void *numeric);
 
 /* Diagnostic accessor (no reference counterpart; used by parity tests):

@@ -104,7 +104,7 @@ SIGNATURES = {
     "spral_ssids_hip_enquire_indef": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "spral_ssids_hip_alter": (C.c_int, [C.c_void_p, f64p]),
     "spral_ssids_hip_enquire_posdef": (C.c_int, [C.c_void_p, f64p]),
-    "spral_ssids_hip_get_contrib": (None, [C.c_void_p] + [C.c_void_p] * 8),
+    "spral_ssids_hip_get_contrib": (C.c_int, [C.c_void_p] + [C.c_void_p] * 8),
     "spral_ssids_hip_free_contrib": (None, [C.c_void_p]),
     "spral_ssids_hip_device_count": (C.c_int, []),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,

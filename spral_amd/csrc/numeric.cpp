@@ -1308,6 +1308,8 @@ static void set_error(spral_ssids_hip_stats *st, const HipError &e) {
 
 extern "C" {
 
+void spral_ssids_hip_free_contrib(void *numeric);
+
 int spral_ssids_hip_device_count(void) {
   int c = 0;
   if (hipGetDeviceCount(&c) != hipSuccess) return 0;
@@ -1644,9 +1646,9 @@ int spral_ssids_hip_alter(void *numeric, const double *d) {
   }
 }
 
-void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *ldval,
-                                 const int **rlist, int *ndelay, const int **delay_perm,
-                                 const double **delay_val, int *lddelay) {
+int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *ldval,
+                                const int **rlist, int *ndelay, const int **delay_perm,
+                                const double **delay_val, int *lddelay) {
   Numeric *N = static_cast<Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   *n = 0;
@@ -1658,49 +1660,60 @@ void spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int 
   *delay_val = nullptr;
   *lddelay = 0;
   // the subtree root is the last node in postorder
-  if (S.nnodes == 0) return;
-  const NodeDev &D = N->nodes[S.slot[S.nnodes - 1]];
-  int cm = D.m - D.n;
-  N->contrib_val.assign((size_t)cm * cm, 0.0);
-  if (cm > 0) {
-    if (N->c32) {  // mixed precision: the block is held in FP32
-      std::vector<float> f((size_t)cm * cm);
-      (void)hipMemcpy2D(f.data(), cm * sizeof(float), D.C, D.ldc * sizeof(float),
-                        cm * sizeof(float), cm, hipMemcpyDeviceToHost);
-      for (size_t k = 0; k < f.size(); ++k) N->contrib_val[k] = f[k];
-    } else {
-      (void)hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
-                        D.ldc * sizeof(double), cm * sizeof(double), cm,
-                        hipMemcpyDeviceToHost);
+  if (S.nnodes == 0) return 0;
+  try {
+    HIPCHK(hipSetDevice(S.device));
+    HIPCHK(hipStreamSynchronize(N->stream));
+    const NodeDev &D = N->nodes[S.slot[S.nnodes - 1]];
+    const int cm = D.m - D.n;
+    N->contrib_val.assign((size_t)cm * cm, 0.0);
+    if (cm > 0) {
+      if (N->c32) {  // mixed precision: the block is held in FP32
+        std::vector<float> f((size_t)cm * cm);
+        HIPCHK(hipMemcpy2D(f.data(), cm * sizeof(float), D.C, D.ldc * sizeof(float),
+                           cm * sizeof(float), cm, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < f.size(); ++k) N->contrib_val[k] = f[k];
+      } else {
+        HIPCHK(hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
+                           D.ldc * sizeof(double), cm * sizeof(double), cm,
+                           hipMemcpyDeviceToHost));
+      }
+      // symmetrise the lower triangle into the full square the reference passes
+      for (int j = 0; j < cm; ++j)
+        for (int i = 0; i < j; ++i)
+          N->contrib_val[(size_t)j * cm + i] = N->contrib_val[(size_t)i * cm + j];
     }
-    // symmetrise the lower triangle into the full square the reference passes
-    for (int j = 0; j < cm; ++j)
-      for (int i = 0; i < j; ++i)
-        N->contrib_val[(size_t)j * cm + i] = N->contrib_val[(size_t)i * cm + j];
+    N->contrib_rlist.resize(cm);
+    const int i_root = S.nnodes - 1;
+    for (int k = 0; k < cm; ++k)
+      N->contrib_rlist[k] = S.rows_h[S.row_off[i_root] + D.sn + k] + 1;
+    const int nf = D.n - D.nelim;
+    N->contrib_delay_perm.resize(nf);
+    N->contrib_delay_val.assign((size_t)nf * D.m, 0.0);
+    if (nf > 0) {
+      std::vector<int> ph(D.n);
+      HIPCHK(hipMemcpy(ph.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost));
+      for (int k = 0; k < nf; ++k) N->contrib_delay_perm[k] = ph[D.nelim + k] + 1;
+      HIPCHK(hipMemcpy2D(N->contrib_delay_val.data(), D.m * sizeof(double),
+                         D.L + (size_t)D.nelim * D.ldl, D.ldl * sizeof(double),
+                         D.m * sizeof(double), nf, hipMemcpyDeviceToHost));
+    }
+    *n = cm;
+    *val = N->contrib_val.data();
+    *ldval = cm;
+    *rlist = N->contrib_rlist.data();
+    *ndelay = nf;
+    *delay_perm = N->contrib_delay_perm.data();
+    *delay_val = nf ? N->contrib_delay_val.data() + D.nelim : nullptr;
+    *lddelay = D.m;
+    return 0;
+  } catch (HipError &) {
+    spral_ssids_hip_free_contrib(numeric);
+    return -51;
+  } catch (std::bad_alloc &) {
+    spral_ssids_hip_free_contrib(numeric);
+    return -50;
   }
-  N->contrib_rlist.resize(cm);
-  const int i_root = S.nnodes - 1;
-  for (int k = 0; k < cm; ++k)
-    N->contrib_rlist[k] = S.rows_h[S.row_off[i_root] + D.sn + k] + 1;
-  int nf = D.n - D.nelim;
-  N->contrib_delay_perm.resize(nf);
-  N->contrib_delay_val.assign((size_t)nf * D.m, 0.0);
-  if (nf > 0) {
-    std::vector<int> ph(D.n);
-    (void)hipMemcpy(ph.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost);
-    for (int k = 0; k < nf; ++k) N->contrib_delay_perm[k] = ph[D.nelim + k] + 1;
-    (void)hipMemcpy2D(N->contrib_delay_val.data(), D.m * sizeof(double),
-                      D.L + (size_t)D.nelim * D.ldl, D.ldl * sizeof(double),
-                      D.m * sizeof(double), nf, hipMemcpyDeviceToHost);
-  }
-  *n = cm;
-  *val = N->contrib_val.data();
-  *ldval = cm;
-  *rlist = N->contrib_rlist.data();
-  *ndelay = nf;
-  *delay_perm = N->contrib_delay_perm.data();
-  *delay_val = nf ? N->contrib_delay_val.data() + D.nelim : nullptr;
-  *lddelay = D.m;
 }
 
 int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, double *L,

@@ -115,13 +115,13 @@ module spral_ssids_gpu_subtree
        import :: C_PTR, C_INT
        type(C_PTR), value :: f, d
      end function hip_enquire_posdef
-     subroutine hip_get_contrib(f, n, val, ldval, rlist, ndelay, delay_perm, delay_val, &
-          lddelay) bind(C, name="spral_ssids_hip_get_contrib")
+     integer(C_INT) function hip_get_contrib(f, n, val, ldval, rlist, ndelay, delay_perm, &
+          delay_val, lddelay) bind(C, name="spral_ssids_hip_get_contrib")
        import :: C_PTR, C_INT
        type(C_PTR), value :: f
        integer(C_INT), intent(out) :: n, ldval, ndelay, lddelay
        type(C_PTR), intent(out) :: val, rlist, delay_perm, delay_val
-     end subroutine hip_get_contrib
+     end function hip_get_contrib
      subroutine hip_free_contrib(f) bind(C, name="spral_ssids_hip_free_contrib")
        import :: C_PTR
        type(C_PTR), value :: f
@@ -297,10 +297,16 @@ contains
   function get_contrib(this)
     type(contrib_type) :: get_contrib
     class(gpu_numeric_subtree), intent(in) :: this
-    integer(C_INT) :: n, ldval, ndelay, lddelay
+    integer(C_INT) :: n, ldval, ndelay, lddelay, flag
     type(C_PTR) :: val, rlist, delay_perm, delay_val
-    call hip_get_contrib(this%handle, n, val, ldval, rlist, ndelay, delay_perm, delay_val, &
-         lddelay)
+    flag = hip_get_contrib(this%handle, n, val, ldval, rlist, ndelay, delay_perm, &
+         delay_val, lddelay)
+    ! contrib_type has no error channel and the parent subtree spins on
+    ! %ready: a device failure while exporting the block stops the run
+    if (flag /= 0) then
+       write (0, "(a,i0)") "spral_ssids_hip_get_contrib failed, flag ", flag
+       error stop 1
+    end if
     get_contrib%n = n
     get_contrib%ldval = ldval
     get_contrib%ndelay = ndelay

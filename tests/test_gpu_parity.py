@@ -143,13 +143,22 @@ def test_laplacian_closed_form_inertia(lib, g):
 
 
 def test_laplacian_64_full_size(lib):
-    """The benchmark workload at full size: closed-form inertia 636."""
+    """The benchmark workload at full size: closed-form inertia 636; the raw
+    solve within the reference kernels' 1e-13 and FP64 refinement to 1e-14
+    (the 6112-column root front accumulates more rounding than the small
+    grids above)."""
     m = matrices.laplace3d(64, diag=5.7)
     ak, fk, finf, opts = gpu_factor(m, grid=(64, 64, 64))
     assert finf.num_neg == 636 and finf.matrix_rank == m.n
     xt, b = rhs(m, 1, 64)
     x = gpu_solve(ak, fk, opts, b)
-    assert matrices.bwd_error(m.full, x, b).max() <= BWD_TOL
+    raw = matrices.bwd_error(m.full, x, b).max()
+    print(f"laplace64 raw bwd {raw:.3e}", flush=True)
+    assert raw <= RAW_TOL
+    xr = np.array(b, dtype=np.float64, copy=True)
+    inf, bwd, its = ssids.ssids_solve_refine(xr, ak, fk, opts, maxit=5, tol=BWD_TOL)
+    assert inf.flag >= 0 and its <= 2
+    assert matrices.bwd_error(m.full, xr, b).max() <= BWD_TOL
     ssids.ssids_free(ak, fk)
 
 
@@ -303,8 +312,10 @@ def test_get_contrib_of_subtree(lib, case):
     cn, ldv, nd, ldd = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     val, rl, dperm, dval = (C.POINTER(C.c_double)(), C.POINTER(C.c_int)(),
                             C.POINTER(C.c_int)(), C.POINTER(C.c_double)())
-    L.spral_ssids_hip_get_contrib(num, C.byref(cn), C.byref(val), C.byref(ldv), C.byref(rl),
-                                  C.byref(nd), C.byref(dperm), C.byref(dval), C.byref(ldd))
+    f = L.spral_ssids_hip_get_contrib(num, C.byref(cn), C.byref(val), C.byref(ldv),
+                                      C.byref(rl), C.byref(nd), C.byref(dperm), C.byref(dval),
+                                      C.byref(ldd))
+    assert f == 0, f
     assert nd.value == 0  # no delays leave this subtree
     n_c = cn.value
     assert n_c == (ak.rptr[root + 1] - ak.rptr[root]) - (ak.sptr[root + 1] - ak.sptr[root])
