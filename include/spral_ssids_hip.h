@@ -173,6 +173,23 @@ int spral_ssids_hip_solve_bwd(void *numeric, int nrhs, double *x, int ldx);
 int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x,
                                  int ldx, void *stream);
 
+/* Allocation-free variant of spral_ssids_hip_solve_device (no counterpart in
+ * the reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; the solve workspace is a fixed 64 MiB device
+ * array inside the library (one per GPU, held until the process ends)
+ * instead of the handle's own buffer, and no allocating call is made.
+ * Returns SPRAL_SSIDS_HIP_ERROR_SCRATCH_TOO_SMALL, before any launch, when
+ * the solve needs more than SPRAL_SSIDS_HIP_SCRATCH_DOUBLES doubles
+ * (spral_ssids_hip_solve_workspace_doubles(numeric, nrhs) gives the need).
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (several streams or host threads) race. */
+#define SPRAL_SSIDS_HIP_SCRATCH_DOUBLES (8 << 20)
+#define SPRAL_SSIDS_HIP_ERROR_SCRATCH_TOO_SMALL (-61)
+int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, double *d_x,
+                                         int ldx, void *stream);
+int64_t spral_ssids_hip_solve_workspace_doubles(const void *numeric, int nrhs);
+
 /* FP64 iterative refinement (no reference counterpart: the reference solve has
  * none).  x = solve(b), then x += solve(b - A x) with A the factorized values
  * (scaled system when a scaling was given) until every column's scaled

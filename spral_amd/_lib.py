@@ -94,6 +94,9 @@ SIGNATURES = {
     "spral_ssids_hip_solve_bwd": (C.c_int, [C.c_void_p, C.c_int, f64p, C.c_int]),
     "spral_ssids_hip_solve_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                                C.c_void_p, C.c_int, C.c_void_p]),
+    "spral_ssids_hip_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
+                                                       C.c_void_p, C.c_int, C.c_void_p]),
+    "spral_ssids_hip_solve_workspace_doubles": (C.c_int64, [C.c_void_p, C.c_int]),
     "spral_ssids_hip_solve_refine": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p, C.c_int,
                                                C.c_int, C.c_double, f64p,
                                                C.POINTER(C.c_int)]),

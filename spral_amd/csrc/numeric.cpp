@@ -15,6 +15,7 @@
 #include <map>
 #include <set>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <vector>
@@ -1175,6 +1176,30 @@ static void solve_launch(Numeric &N, int job, int nrhs, double *d_x, int ldx, hi
   HIPCHK(hipGetLastError());
 }
 
+// Workspace of the allocation-free solve entry point: a file-scope device
+// array, one copy per GPU the library runs on, held until the process ends.
+// Its device address is looked up once per device (hipGetSymbolAddress with
+// that device current); host code passes only that address, never the
+// array's name, to kernels.  The solve kernels write every workspace entry
+// they read earlier in the same solve, so nothing needs clearing between
+// calls.  Calls that overlap on one GPU share the array and race.
+constexpr size_t kScratchDoubles = size_t(8) << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex scratch_mu;
+static double *scratch_addr[kMaxScratchDevices];
+
+static double *scratch_on(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) throw HipError(hipErrorInvalidDevice);
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_addr[device]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_addr[device] = static_cast<double *>(p);
+  }
+  return scratch_addr[device];
+}
+
 static void solve_device(Numeric &N, int job, int nrhs, double *d_x, int ldx,
                          hipStream_t st) {
   solve_launch(N, job, nrhs, d_x, ldx, st, solve_ws(N, nrhs));
@@ -1460,6 +1485,29 @@ int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x, 
   } catch (HipError &) {
     return -51;
   }
+}
+
+int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, double *d_x,
+                                         int ldx, void *stream) {
+  Numeric *N = static_cast<Numeric *>(numeric);
+  if (job < 0 || job > 4) return -11;
+  if (nrhs < 1 || ldx < N->symb->n) return -10;
+  if ((size_t)N->solve_ws_per_rhs * (size_t)nrhs > kScratchDoubles)
+    return SPRAL_SSIDS_HIP_ERROR_SCRATCH_TOO_SMALL;
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    double *ws = scratch_on(N->symb->device);
+    hipStream_t st = stream ? (hipStream_t)stream : N->stream;
+    solve_launch(*N, job, nrhs, d_x, ldx, st, ws);
+    return 0;
+  } catch (HipError &) {
+    return -51;
+  }
+}
+
+int64_t spral_ssids_hip_solve_workspace_doubles(const void *numeric, int nrhs) {
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  return N->solve_ws_per_rhs * (int64_t)nrhs;
 }
 
 int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,
