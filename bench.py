@@ -35,8 +35,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--solves", type=int, default=50, help="solve latency samples")
-    ap.add_argument("--cpu-grid", type=int, default=48,
-                    help="grid of the bounded CPU-baseline sample (oracle)")
+    ap.add_argument("--cpu-flops", type=float, default=9e10,
+                    help="flops of the bounded CPU-baseline sample (oracle)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead of the default workload: factor+solve this many independent "
@@ -74,20 +74,43 @@ def gather_traffic():
     return None
 
 
-def cpu_baseline(grid):
-    """Time the CPU oracle (single-threaded restatement of the reference TPP
-    path) on a bounded sample: the same stencil on a grid^3 box."""
+def node_flops(ak):
+    """Per-front flops of the analyse count (core_analyse.f90:881-896):
+    sum over j = 1..n of (m - n + j)^2, in postorder."""
+    sptr = np.asarray(ak.sptr, np.int64)
+    rptr = np.asarray(ak.rptr, np.int64)
+    n = np.diff(sptr)
+    m = np.diff(rptr)
+    # sum_{j=1..n} (m-n+j)^2 = S(m) - S(m-n), S(k) = k(k+1)(2k+1)/6
+    S = lambda k: k * (k + 1) * (2 * k + 1) // 6
+    return S(m) - S(m - n)
+
+
+def cpu_baseline(ak, m, budget_flops):
+    """Time the CPU oracle (single-threaded restatement of the reference's
+    TPP factorization) on a bounded sample of the same workload: the first
+    fronts of its assembly tree in postorder (a prefix of the same
+    factorization, each front the size the GPU factors) up to budget_flops."""
     from oracle.oracle_py import OracleFactor  # checker / baseline only
-    from spral_amd import matrices, ssids
-    m = matrices.laplace3d(grid, diag=5.7)
-    opts = ssids.SSIDSOptions(ordering=3, use_gpu=False)
-    ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, grid=(grid,) * 3)
+    fl = node_flops(ak)
+    cum = np.cumsum(fl)
+    k = int(np.searchsorted(cum, budget_flops, side="right"))
+    k = max(1, min(k, len(fl)))
+    done = float(cum[k - 1])
     t = time.perf_counter()
-    of = OracleFactor(ak, m.val)
+    of = OracleFactor(ak, m.val, max_nodes=k)
     dt = time.perf_counter() - t
-    return {"value": ak.num_flops / dt / 1e9, "unit": "GFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"oracle factor of the same stencil on {grid}^3 (n={m.n}, "
-                      f"{ak.num_flops:.3e} flops) in {dt:.2f} s"}
+    del of
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"value": done / dt / 1e9, "unit": "GFLOP/s", "cores": 1, "nproc": nproc,
+            "kind": "port",
+            "sample": f"oracle (single-threaded C++ restatement of the reference TPP "
+                      f"factorization) over the first {k} of {len(fl)} fronts in postorder "
+                      f"of this workload's assembly tree ({done:.3e} of "
+                      f"{float(cum[-1]):.3e} flops) in {dt:.2f} s"}
 
 
 def batch_bench(args):
@@ -451,7 +474,7 @@ def main():
         mixed["fp64_factor_plus_solve_ms"] = round(fac_mean + out["solve_ms_p50"], 3)
         out["mixed"] = mixed
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_grid)
+        out["cpu_baseline"] = cpu_baseline(ak, m, args.cpu_flops)
     L.spral_ssids_hip_destroy_numeric(num)
     ssids.ssids_free(ak)
     print(json.dumps(out), flush=True)

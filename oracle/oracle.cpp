@@ -256,6 +256,15 @@ void *oracle_factor(int n, int nnodes, const int *sptr, const int *sparent,
                     const int64_t *nlist, const double *aval,
                     const double *scaling, int action, double u, double small,
                     int64_t *stats) {
+  return oracle_factor_prefix(n, nnodes, sptr, sparent, rptr, rlist, nptr, nlist, aval,
+                              scaling, action, u, small, nnodes, stats);
+}
+
+void *oracle_factor_prefix(int n, int nnodes, const int *sptr, const int *sparent,
+                           const int64_t *rptr, const int *rlist, const int64_t *nptr,
+                           const int64_t *nlist, const double *aval,
+                           const double *scaling, int action, double u, double small,
+                           int max_nodes, int64_t *stats) {
   auto *o = new Oracle();
   o->n = n;
   o->nnodes = nnodes;
@@ -267,8 +276,9 @@ void *oracle_factor(int n, int nnodes, const int *sptr, const int *sparent,
   std::vector<int> map(n + 1, -1);
   int64_t num_delay = 0, not_first = 0, nfact = 0, nflops = 0;
   int maxfront = 0;
+  const int nfac = std::min(nnodes, std::max(0, max_nodes));
   try {
-    for (int s = 0; s < nnodes; ++s) {
+    for (int s = 0; s < nfac; ++s) {
       Node &nd = o->nodes[s];
       nd.sn = sptr[s + 1] - sptr[s];
       nd.sm = (int)(rptr[s + 1] - rptr[s]);
@@ -332,7 +342,7 @@ void *oracle_factor(int n, int nnodes, const int *sptr, const int *sparent,
     return o;
   }
   int64_t neg = 0, two = 0, zero = 0;
-  for (int s = 0; s < nnodes; ++s) {
+  for (int s = 0; s < nfac; ++s) {
     Node &nd = o->nodes[s];
     const double *d = nd.d.data();
     for (int i = 0; i < nd.nelim;) {

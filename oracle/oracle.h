@@ -29,6 +29,15 @@ void *oracle_factor(int n, int nnodes, const int *sptr, const int *sparent,
                     const int64_t *nlist, const double *aval,
                     const double *scaling, int action, double u, double small,
                     int64_t *stats);
+/* The same on the first max_nodes nodes in postorder only (every child
+ * precedes its parent, so this is a prefix of the same factorization): a
+ * bounded CPU sample of a large workload.  stats as above over those nodes;
+ * the handle is for timing only (no solves). */
+void *oracle_factor_prefix(int n, int nnodes, const int *sptr, const int *sparent,
+                           const int64_t *rptr, const int *rlist, const int64_t *nptr,
+                           const int64_t *nlist, const double *aval,
+                           const double *scaling, int action, double u, double small,
+                           int max_nodes, int64_t *stats);
 /* job: 0 all, 1 fwd, 2 diag, 3 bwd, 4 diag_bwd; x (ldx,nrhs) in elimination
  * order (as the subtree sees it). */
 void oracle_solve(void *h, int job, int nrhs, double *x, int ldx);

@@ -21,6 +21,9 @@ def lib():
         L.oracle_factor.restype = vp
         L.oracle_factor.argtypes = [C.c_int, C.c_int] + [vp] * 8 + [C.c_int, C.c_double,
                                                                     C.c_double, vp]
+        L.oracle_factor_prefix.restype = vp
+        L.oracle_factor_prefix.argtypes = ([C.c_int, C.c_int] + [vp] * 8 +
+                                           [C.c_int, C.c_double, C.c_double, C.c_int, vp])
         L.oracle_solve.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int]
         L.oracle_enquire.argtypes = [vp, vp, vp]
         L.oracle_node_info.argtypes = [vp, vp, vp]
@@ -40,17 +43,23 @@ class OracleFactor:
     # u defaults to the reference's value: the single-precision literal 0.01
     # widened to double (src/ssids/datatypes.f90:262)
     def __init__(self, akeep, val, scaling=None, action=True, u=float(np.float32(0.01)),
-                 small=1e-20):
+                 small=1e-20, max_nodes=None):
+        """max_nodes: factor only the first max_nodes fronts in postorder (a
+        prefix of the same factorization, for a bounded timing sample; such a
+        handle takes no solves)."""
         L = lib()
         self.akeep = akeep
         self.val = np.ascontiguousarray(val, np.float64)
         self.scaling = None if scaling is None else np.ascontiguousarray(
             np.asarray(scaling, np.float64)[akeep.invp])
         st = np.zeros(10, np.int64)
-        self.h = L.oracle_factor(akeep.n, akeep.nnodes, _p(akeep.sptr), _p(akeep.sparent),
-                                 _p(akeep.rptr), _p(akeep.rlist), _p(akeep.nptr),
-                                 _p(akeep.nlist), _p(self.val), _p(self.scaling),
-                                 1 if action else 0, u, small, _p(st))
+        args = (akeep.n, akeep.nnodes, _p(akeep.sptr), _p(akeep.sparent), _p(akeep.rptr),
+                _p(akeep.rlist), _p(akeep.nptr), _p(akeep.nlist), _p(self.val),
+                _p(self.scaling), 1 if action else 0, u, small)
+        if max_nodes is None:
+            self.h = L.oracle_factor(*args, _p(st))
+        else:
+            self.h = L.oracle_factor_prefix(*args, int(max_nodes), _p(st))
         keys = ["flag", "num_delay", "num_neg", "num_two", "num_zero", "maxfront",
                 "not_first_pass", "num_factor", "num_flops", "matrix_rank"]
         self.stats = {k: int(v) for k, v in zip(keys, st)}

@@ -49,20 +49,25 @@ void launch_fwd_tri(hipStream_t, int, const SolveNode *, const int *, const int 
                     int, int, double *);
 void launch_fwd_rows(hipStream_t, int, const SolveNode *, const int *, const int *, int,
                      const int *, int, double *);
-void launch_big_fwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
-void launch_big_bwd(hipStream_t, const SolveNode *, int, int, double *, int, int, double *);
+void launch_big_fwd_lvl(hipStream_t, const SolveNode *, const int *, int, int, double *, int,
+                        int, double *, int *);
+void launch_big_bwd_lvl(hipStream_t, const SolveNode *, const int *, int, int, double *, int,
+                        int, double *, int *);
+void launch_big_sync_clear(hipStream_t, int *, int);
 int big_tri_threshold();
 int fwd_rows_per_wg();
 int bwd_rows_per_wg();
+int bwd_cols_per_wg();
 void launch_dsolve(hipStream_t, int, const SolveNode *, double *, int, int, int);
 void launch_bwd(hipStream_t, int, const SolveNode *, const int *, double *, int, int,
-                const int *, int, double *);
+                const int *, const int *, int, double *, int);
 
 void launch_residual(hipStream_t, int, const int64_t *, const int *, const int64_t *,
                      const double *, const double *, const double *, double *, int, int);
 void launch_anorm(hipStream_t, int, const int64_t *, const int64_t *, const double *,
                   unsigned long long *);
-void launch_colmax(hipStream_t, int, const double *, int, int, int, unsigned long long *);
+void launch_colmax3(hipStream_t, int, const double *, const double *, const double *, int, int,
+                    unsigned long long *);
 void launch_axpy(hipStream_t, int, int, int, const double *, double *, const int *);
 void launch_scale_rows(hipStream_t, int, int, int, const double *, double *);
 
@@ -260,9 +265,15 @@ struct Numeric {
   // 64-row workgroups, backward: 256-row workgroups), one array
   int *d_gemv_tab = nullptr;
   std::vector<int> gemv_tab_off, gemv_total;  // forward: offset into d_gemv_tab, total
-  std::vector<int> bwd_tab_off, bwd_total;    // backward
+  std::vector<int> bwd_tab_off, bwd_total;    // backward: row workgroups per front
+  std::vector<int> bwdc_tab_off, bwdc_total;  // backward: row workgroups x column chunks
   std::vector<int> bwd_count;                 // per slot: backward row workgroups
-  std::vector<std::vector<int>> big_of_level;  // slots whose triangle needs the launch sequence
+  // per level: the fronts whose triangle is solved by the big-front level
+  // kernels, as a device table [block offsets | slots | sync offsets]
+  std::vector<std::vector<int>> big_of_level;
+  std::vector<int> bigtab_off, big_blocks;     // per level: table offset, blocks
+  int *d_bigtab = nullptr;
+  int64_t solve_sync_ints = 0;                // sync words after the workspace
   std::vector<SolveNode> sn_h;
   double *d_X = nullptr;                  // solve workspace (solve_ws_per_rhs x nrhs)
   size_t d_X_cap = 0;
@@ -293,6 +304,7 @@ struct Numeric {
     if (d_sn) (void)hipFree(d_sn);
     if (d_slots_all) (void)hipFree(d_slots_all);
     if (d_gemv_tab) (void)hipFree(d_gemv_tab);
+    if (d_bigtab) (void)hipFree(d_bigtab);
     if (d_X) (void)hipFree(d_X);
     if (d_x) (void)hipFree(d_x);
     if (d_aval) (void)hipFree(d_aval);
@@ -1079,6 +1091,8 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     N.bwd_tab_off.assign(nlev2, 0);
     N.bwd_total.assign(nlev2, 0);
     N.bwd_count.assign(nn, 0);
+    N.bwdc_tab_off.assign(nlev2, 0);
+    N.bwdc_total.assign(nlev2, 0);
     for (int L = 0; L < nlev2; ++L) {
       N.gemv_tab_off[L] = (int)tab.size();
       int acc = 0;
@@ -1098,6 +1112,14 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
       }
       tab.push_back(acc);
       N.bwd_total[L] = acc;
+      N.bwdc_tab_off[L] = (int)tab.size();
+      acc = 0;
+      for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) {
+        tab.push_back(acc);
+        acc += N.bwd_count[sl] * cdiv(std::max(N.nodes[sl].nelim, 1), bwd_cols_per_wg());
+      }
+      tab.push_back(acc);
+      N.bwdc_total[L] = acc;
     }
     if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
     N.d_gemv_tab = nullptr;
@@ -1122,11 +1144,50 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   for (int L = 0; L + 1 < (int)S.lvl_ptr.size(); ++L)
     for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl)
       if (N.nodes[sl].nelim > big_tri_threshold()) N.big_of_level[L].push_back(sl);
+  {
+    // per level [block offsets (nbig+1) | slots (nbig) | sync offsets (nbig)];
+    // sync words per big front: forward ticket + nb flags, backward the same
+    std::vector<int> tab;
+    const int nlev2 = (int)S.lvl_ptr.size() - 1;
+    N.bigtab_off.assign(nlev2, 0);
+    N.big_blocks.assign(nlev2, 0);
+    int64_t sync = 0;
+    for (int L = 0; L < nlev2; ++L) {
+      const auto &bl = N.big_of_level[L];
+      N.bigtab_off[L] = (int)tab.size();
+      int acc = 0;
+      for (int sl : bl) {
+        tab.push_back(acc);
+        acc += cdiv(N.nodes[sl].nelim, kBlk);
+      }
+      tab.push_back(acc);
+      N.big_blocks[L] = acc;
+      for (int sl : bl) tab.push_back(sl);
+      for (int sl : bl) {
+        tab.push_back((int)sync);
+        sync += 2 * (cdiv(N.nodes[sl].nelim, kBlk) + 1);
+      }
+    }
+    N.solve_sync_ints = sync;
+    if (N.d_bigtab) HIPCHK(hipFree(N.d_bigtab));
+    N.d_bigtab = nullptr;
+    if (!tab.empty()) {
+      HIPCHK(hipMalloc(&N.d_bigtab, tab.size() * sizeof(int)));
+      HIPCHK(hipMemcpy(N.d_bigtab, tab.data(), tab.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+    }
+  }
+}
+
+// Doubles of solve workspace for nrhs right-hand sides: the per-front X and
+// partial sums, then the big fronts' sync words.
+static size_t solve_ws_need(const Numeric &N, int nrhs) {
+  return (size_t)N.solve_ws_per_rhs * nrhs + (size_t)(N.solve_sync_ints + 1) / 2;
 }
 
 // The handle's own solve workspace, grown to nrhs right-hand sides.
 static double *solve_ws(Numeric &N, int nrhs) {
-  const size_t need = (size_t)N.solve_ws_per_rhs * nrhs;
+  const size_t need = solve_ws_need(N, nrhs);
   if (need > N.d_X_cap) {
     if (N.d_X) HIPCHK(hipFree(N.d_X));
     N.d_X = nullptr;
@@ -1154,24 +1215,30 @@ static void solve_launch(Numeric &N, int job, int nrhs, double *d_x, int ldx, hi
   // positive definite: L_chol = L D^{1/2}, so the forward (job 1) and
   // backward (job 3) halves of L L^T carry D^{-1/2} on their inner side
   const bool half_after_fwd = N.posdef && job == 1, half_before_bwd = N.posdef && job == 3;
+  int *sync = reinterpret_cast<int *>(ws + (size_t)N.solve_ws_per_rhs * nrhs);
+  if (fwd || bwd) launch_big_sync_clear(st, sync, (int)N.solve_sync_ints);
   if (fwd)
     for (int L = 0; L < nlev; ++L) {
       const int nl = S.lvl_ptr[L + 1] - S.lvl_ptr[L];
       launch_fwd_tri(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, d_x, ldx, nrhs, ws);
-      for (int sl : N.big_of_level[L])
-        launch_big_fwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, ws);
+      launch_big_fwd_lvl(st, N.d_sn, N.d_bigtab + N.bigtab_off[L], (int)N.big_of_level[L].size(),
+                         N.big_blocks[L], d_x, ldx, nrhs, ws, sync);
       launch_fwd_rows(st, nl, N.d_sn, slots + S.lvl_ptr[L], S.d_child, nrhs,
                       N.d_gemv_tab + N.gemv_tab_off[L], N.gemv_total[L], ws);
     }
-  if (diag) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 0);
+  // with a backward solve after it, D^{-1} is applied per front inside the
+  // backward kernels (a front's pivots only mix its own positions)
+  const int dfuse = (diag && bwd && !N.posdef) ? 1 : 0;
+  if (diag && !dfuse) launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 0);
   if (half_after_fwd || half_before_bwd)
     launch_dsolve(st, S.nnodes, N.d_sn, d_x, ldx, nrhs, 1);
   if (bwd)
     for (int L = nlev - 1; L >= 0; --L) {
       launch_bwd(st, S.lvl_ptr[L + 1] - S.lvl_ptr[L], N.d_sn, slots + S.lvl_ptr[L], d_x,
-                 ldx, nrhs, N.d_gemv_tab + N.bwd_tab_off[L], N.bwd_total[L], ws);
-      for (int sl : N.big_of_level[L])
-        launch_big_bwd(st, N.d_sn, sl, N.nodes[sl].nelim, d_x, ldx, nrhs, ws);
+                 ldx, nrhs, N.d_gemv_tab + N.bwd_tab_off[L], N.d_gemv_tab + N.bwdc_tab_off[L],
+                 N.bwdc_total[L], ws, dfuse);
+      launch_big_bwd_lvl(st, N.d_sn, N.d_bigtab + N.bigtab_off[L], (int)N.big_of_level[L].size(),
+                         N.big_blocks[L], d_x, ldx, nrhs, ws, sync);
     }
   HIPCHK(hipGetLastError());
 }
@@ -1281,9 +1348,7 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
   for (;;) {
     launch_residual(st, n, S.d_fptr, S.d_fcol, S.d_fsrc, N.d_aval, d_x, d_b, N.d_r, ld, nrhs);
     HIPCHK(hipMemsetAsync(N.d_norms, 0, (1 + 3 * (size_t)nrhs) * sizeof(unsigned long long), st));
-    launch_colmax(st, n, N.d_r, ld, nrhs, 0, N.d_norms + 1);
-    launch_colmax(st, n, d_x, ld, nrhs, 1, N.d_norms + 1);
-    launch_colmax(st, n, d_b, ld, nrhs, 2, N.d_norms + 1);
+    launch_colmax3(st, n, N.d_r, d_x, d_b, ld, nrhs, N.d_norms + 1);
     HIPCHK(hipMemcpyAsync(nb.data(), N.d_norms, nb.size() * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1492,7 +1557,7 @@ int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, doubl
   Numeric *N = static_cast<Numeric *>(numeric);
   if (job < 0 || job > 4) return -11;
   if (nrhs < 1 || ldx < N->symb->n) return -10;
-  if ((size_t)N->solve_ws_per_rhs * (size_t)nrhs > kScratchDoubles)
+  if (solve_ws_need(*N, nrhs) > kScratchDoubles)
     return SPRAL_SSIDS_HIP_ERROR_SCRATCH_TOO_SMALL;
   try {
     HIPCHK(hipSetDevice(N->symb->device));
@@ -1506,8 +1571,7 @@ int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, doubl
 }
 
 int64_t spral_ssids_hip_solve_workspace_doubles(const void *numeric, int nrhs) {
-  const Numeric *N = static_cast<const Numeric *>(numeric);
-  return N->solve_ws_per_rhs * (int64_t)nrhs;
+  return (int64_t)solve_ws_need(*static_cast<const Numeric *>(numeric), nrhs);
 }
 
 int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,

@@ -10,19 +10,22 @@
 
 namespace ssids_hip {
 
-// one wave per row (rows are short: stencils and KKT rows)
+// kResLanes lanes per row (stencil and KKT rows are short; longer rows are
+// strided over the group), the group's sums reduced by a fixed butterfly
+constexpr int kResLanes = 8;
 __global__ void k_residual(int n, const int64_t *rp, const int *ci, const int64_t *src,
                            const double *aval, const double *scale_unused, const double *x,
                            const double *b, double *r, int ld, int nrhs) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= n) return;
+  const int sub = threadIdx.x & (kResLanes - 1);
+  const int row = blockIdx.x * (blockDim.x / kResLanes) + threadIdx.x / kResLanes;
+  if (row >= n) return;  // whole groups of kResLanes lanes leave together
+  const int64_t e0 = rp[row], e1 = rp[row + 1];
   for (int k = 0; k < nrhs; ++k) {
     const double *xk = x + (size_t)k * ld;
     double acc = 0.0;
-    for (int64_t e = rp[row] + lane; e < rp[row + 1]; e += 64) acc += aval[src[e]] * xk[ci[e]];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) r[(size_t)k * ld + row] = b[(size_t)k * ld + row] - acc;
+    for (int64_t e = e0 + sub; e < e1; e += kResLanes) acc += aval[src[e]] * xk[ci[e]];
+    for (int off = kResLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kResLanes);
+    if (sub == 0) r[(size_t)k * ld + row] = b[(size_t)k * ld + row] - acc;
   }
 }
 
@@ -36,21 +39,34 @@ __global__ void k_anorm(int n, const int64_t *rp, const int64_t *src, const doub
   atomicMax(out, (unsigned long long)__double_as_longlong(s));
 }
 
-// per-column max |v| for nv vectors of length n (ld), into out[col*3 + slot]
-__global__ void k_colmax(int n, const double *v, int ld, int nrhs, int slot,
-                         unsigned long long *out) {
+// max |r|, max |x| and max |b| per column in one pass, into out[3k + 0..2]
+// (ordered bits; the maximum propagates NaN, and as an ordered bit pattern a
+// positive NaN sorts above +inf, so the atomic keeps it too); one atomic per
+// workgroup
+__global__ __launch_bounds__(256) void k_colmax3(int n, const double *rv, const double *xv,
+                                                 const double *bv, int ld, int nrhs,
+                                                 unsigned long long *out) {
+  __shared__ double red[3][4];
   const int k = blockIdx.y;
-  if (k >= nrhs) return;
-  // NaN-propagating maximum (fmax would drop a NaN): as an ordered bit
-  // pattern a positive NaN sorts above +inf, so the atomic keeps it too
   auto nmax = [](double a, double b) { return (b > a || b != b) ? b : a; };
-  double m = 0.0;
+  double m[3] = {0.0, 0.0, 0.0};
+  const double *v[3] = {rv + (size_t)k * ld, xv + (size_t)k * ld, bv + (size_t)k * ld};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    m = nmax(m, fabs(v[(size_t)k * ld + i]));
-  for (int off = 32; off > 0; off >>= 1) m = nmax(m, __shfl_xor(m, off));
-  m = fabs(m);  // a NaN with the sign bit clear
-  if ((threadIdx.x & 63) == 0)
-    atomicMax(&out[k * 3 + slot], (unsigned long long)__double_as_longlong(m));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) m[q] = nmax(m[q], fabs(v[q][i]));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    for (int off = 32; off > 0; off >>= 1) m[q] = nmax(m[q], __shfl_xor(m[q], off));
+    if (lane == 0) red[q][wave] = m[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int q = threadIdx.x;
+    double t = nmax(nmax(red[q][0], red[q][1]), nmax(red[q][2], red[q][3]));
+    t = fabs(t);  // a NaN with the sign bit clear
+    atomicMax(&out[k * 3 + q], (unsigned long long)__double_as_longlong(t));
+  }
 }
 
 __global__ void k_axpy(int n, int nrhs, int ld, const double *dx, double *x,
@@ -78,7 +94,7 @@ void launch_residual(hipStream_t st, int n, const int64_t *rp, const int *ci,
                      const int64_t *src, const double *aval, const double *x, const double *b,
                      double *r, int ld, int nrhs) {
   if (n <= 0) return;
-  int rows_per = 4;  // 4 waves per workgroup
+  const int rows_per = 256 / kResLanes;
   hipLaunchKernelGGL(k_residual, dim3((n + rows_per - 1) / rows_per), dim3(256), 0, st, n, rp,
                      ci, src, aval, nullptr, x, b, r, ld, nrhs);
 }
@@ -87,12 +103,12 @@ void launch_anorm(hipStream_t st, int n, const int64_t *rp, const int64_t *src,
   if (n <= 0) return;
   hipLaunchKernelGGL(k_anorm, dim3((n + 255) / 256), dim3(256), 0, st, n, rp, src, aval, out);
 }
-void launch_colmax(hipStream_t st, int n, const double *v, int ld, int nrhs, int slot,
-                   unsigned long long *out) {
+void launch_colmax3(hipStream_t st, int n, const double *r, const double *x, const double *b,
+                    int ld, int nrhs, unsigned long long *out) {
   if (n <= 0) return;
   int gx = (n + 255) / 256;
-  if (gx > 1024) gx = 1024;
-  hipLaunchKernelGGL(k_colmax, dim3(gx, nrhs), dim3(256), 0, st, n, v, ld, nrhs, slot, out);
+  if (gx > 256) gx = 256;
+  hipLaunchKernelGGL(k_colmax3, dim3(gx, nrhs), dim3(256), 0, st, n, r, x, b, ld, nrhs, out);
 }
 void launch_axpy(hipStream_t st, int n, int nrhs, int ld, const double *dx, double *x,
                  const int *active) {

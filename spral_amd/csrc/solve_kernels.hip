@@ -21,7 +21,8 @@
 //               finish (device-scope counter) reduces them in workgroup order
 //               and solves the triangle with L11^T.
 // Fronts with ne > kBigTri (separators at the top of the tree) solve their
-// triangle with one launch per 64-column block (k_big_fwd / k_big_bwd).
+// triangle in one launch per level over 64-column blocks, the blocks of a
+// front chained by flags (k_big_fwd_lvl / k_big_bwd_lvl).
 //
 // Loads of L are issued in batches of 16 independent addresses (column indices
 // clamped into the block instead of guarded, the surplus products multiplied
@@ -459,10 +460,13 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
 }
 
 // ---- backward -------------------------------------------------------------
-// 256-row workgroups over [ne, m) of every front of the level (a row per
-// thread):  part[wg][r][j] = sum over the workgroup's rows i of
-// L[i, j] x[var(i), r].  Columns go 16 at a time: the wave's 64 row products
-// are summed by a reduce-scatter, then the four waves' sums in wave order.
+// Workgroups over 256-row blocks of [ne, m) times kBwdCols-column chunks of
+// [0, ne), for every front of the level (a row per thread):
+// part[rb][r][j] = sum over row block rb's rows i of L[i, j] x[var(i), r].
+// Columns go 16 at a time: the wave's 64 row products are summed by a
+// reduce-scatter, then the four waves' sums in wave order.  Each (rb, j) is
+// written by exactly one workgroup, so the sums do not depend on the split.
+constexpr int kBwdCols = 64;
 template <int RC>
 __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, const int *slots,
                                                        const int *wg_off, int nlvl,
@@ -476,7 +480,10 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
   gcint *Prows = (gcint *)P.rows;
   gdouble *part = (gdouble *)solve_P(ws, P, nrhs);  // count x nrhs x ne
   const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
-  const int local = blockIdx.x - wg_off[li];
+  const int nchunk = idiv_up(imax(ne, 1), kBwdCols);
+  const int local_all = blockIdx.x - wg_off[li];
+  const int local = local_all / nchunk, chunk = local_all - local * nchunk;
+  const int jbeg = chunk * kBwdCols, jend = imin(ne, jbeg + kBwdCols);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = ne + local * kThreads, r1 = imin(m, r0 + kThreads);
   const int i = r0 + tid, row = imin(i, r1 - 1);
@@ -486,11 +493,11 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
     double xv[RC];
 #pragma unroll
     for (int q = 0; q < RC; ++q) xv[q] = (q < nr && var >= 0) ? x[(size_t)(rc + q) * ldx + var] : 0.0;
-    for (int j0 = 0; j0 < ne; j0 += kBlk) {
-      for (int u0 = 0; u0 < kBlk && j0 + u0 < ne; u0 += 16) {
+    for (int j0 = jbeg; j0 < jend; j0 += kBlk) {
+      for (int u0 = 0; u0 < kBlk && j0 + u0 < jend; u0 += 16) {
         double lv[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(j0 + u0 + u, ne - 1) * ldl + row];
+        for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(j0 + u0 + u, jend - 1) * ldl + row];
 #pragma unroll
         for (int q = 0; q < RC; ++q) {
           double a[16];
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
       __syncthreads();
       for (int idx = tid; idx < RC * kBlk; idx += kThreads) {
         const int q = idx / kBlk, c = idx % kBlk;
-        if (q < nr && j0 + c < ne)
+        if (q < nr && j0 + c < jend)
           part[((size_t)local * nrhs + rc + q) * ne + j0 + c] =
               ((red[0][q][c] + red[1][q][c]) + red[2][q][c]) + red[3][q][c];
       }
@@ -512,13 +519,24 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
   }
 }
 
+// Value of position j after x = D^{-1} x (the expressions of k_dsolve, so
+// the result is the same whether the diagonal solve runs on its own or
+// inside the backward solve)
+__device__ inline double dinv_at(gcdouble *d, gcint *perm, const double *xr, int j, int ne) {
+  const bool first2 = (j + 1 < ne) && !isfinite(d[2 * j + 2]);
+  const bool second2 = !isfinite(d[2 * j]);
+  if (first2) return d[2 * j] * xr[perm[j]] + d[2 * j + 1] * xr[perm[j + 1]];
+  if (second2) return d[2 * j - 1] * xr[perm[j - 1]] + d[2 * j + 1] * xr[perm[j]];
+  return xr[perm[j]] * d[2 * j];
+}
+
 // One workgroup per front of the level: X[j] = x[perm[j]] - sum of the row
 // workgroups' partials (wave w sums workgroups w, w+4, ...; the waves' sums
 // are subtracted in wave order), then for ne <= kBigTri the triangle
 // L11^T X = X, 64-column blocks from the last, and x[perm[j]] = X[j].
 __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const int *slots,
                                                       const int *wg_off, double *x, int ldx,
-                                                      int nrhs, double *ws) {
+                                                      int nrhs, double *ws, int dfuse) {
   __shared__ double xt[kBigTri];
   __shared__ double lb[kBlk][kBlk + 1];
   __shared__ double red[kGroups][kBlk];
@@ -526,6 +544,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
   const int li = blockIdx.x;
   const SolveNode &P = sn[slots[li]];
   gcdouble *PL = (gcdouble *)P.L;
+  gcdouble *Pd = (gcdouble *)P.D;
   gcint *Pperm = (gcint *)P.perm;
   gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
   gcdouble *part = (gcdouble *)solve_P(ws, P, nrhs);
@@ -554,7 +573,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
       __syncthreads();
       const int j = jc + tid;
       if (j < ne)
-        acc[j] = xr[Pperm[j]] - (((pr[0][tid] + pr[1][tid]) + pr[2][tid]) + pr[3][tid]);
+        acc[j] = (dfuse ? dinv_at(Pd, Pperm, xr, j, ne) : xr[Pperm[j]]) -
+                 (((pr[0][tid] + pr[1][tid]) + pr[2][tid]) + pr[3][tid]);
       __syncthreads();
     }
     if (ne > kBigTri) continue;  // the triangle is solved by the big-front launches
@@ -596,12 +616,14 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
 // k_bwd_tri gives.
 __global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, const int *slots,
                                                          const int *wg_off, double *x,
-                                                         int ldx, int nrhs, double *ws) {
+                                                         int ldx, int nrhs, double *ws,
+                                                         int dfuse) {
   __shared__ double xts[kGroups][kBigTri];
   __shared__ double lb[kBlk][kBlk + 1];
   const int li = blockIdx.x;
   const SolveNode &P = sn[slots[li]];
   gcdouble *PL = (gcdouble *)P.L;
+  gcdouble *Pd = (gcdouble *)P.D;
   gcint *Pperm = (gcint *)P.perm;
   gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
   gcdouble *part = (gcdouble *)solve_P(ws, P, nrhs);
@@ -628,7 +650,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, co
               if (j < ne) sum += part[((size_t)w * nrhs + r) * ne + j];
             sg[g] = sum;
           }
-          if (j < ne) acc[j] = xr[Pperm[j]] - (((sg[0] + sg[1]) + sg[2]) + sg[3]);
+          if (j < ne)
+            acc[j] = (dfuse ? dinv_at(Pd, Pperm, xr, j, ne) : xr[Pperm[j]]) -
+                     (((sg[0] + sg[1]) + sg[2]) + sg[3]);
         }
       }
     }
@@ -670,163 +694,206 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, co
   }
 }
 
-// ---- big fronts -------------------------------------------------------------
-// Forward, block [j0, j1): every workgroup solves the diagonal block from
-// X[j0:j1] (as left by the previous block's launch); workgroup q then updates
-// the triangle's rows [j1 + 64q, ...) with the solution, and workgroup 0 keeps
-// the solution in Y (the front's partial-sum area, nrhs x ne).
-__global__ __launch_bounds__(kThreads) void k_big_fwd(const SolveNode *sn, int s, int j0,
-                                                      int nrhs, double *ws) {
-  __shared__ double ys[kRhsChunk][kBlk];
-  __shared__ double red[kGroups][kRhsChunk][kRows];
-  const SolveNode &P = sn[s];
-  gcdouble *PL = (gcdouble *)P.L;
-  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
-  gdouble *Y = (gdouble *)solve_P(ws, P, nrhs);
-  const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
-  const int tid = threadIdx.x, lane = tid & 63, rr = lane, g = tid >> 6, wave = g;
-  const int r0 = j1 + blockIdx.x * kRows, r1 = imin(ne, r0 + kRows);
-  const int row = (r0 < r1) ? imin(r0 + rr, r1 - 1) : 0;
-  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
-    const int nr = imin(kRhsChunk, nrhs - rc);
-    for (int q = wave; q < kRhsChunk; q += kGroups) {
-      double v = 0.0;
-      if (q < nr) {
-        v = fwd_diag_wave(PL, ldl, j0, wb,
-                          (lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] : 0.0, lane);
-        if (blockIdx.x == 0 && lane < wb) Y[(size_t)(rc + q) * ne + j0 + lane] = v;
-      }
-      ys[q][lane] = v;
-    }
-    __syncthreads();
-    if (r0 < r1) {
-      double acc[kRhsChunk];
-#pragma unroll
-      for (int q = 0; q < kRhsChunk; ++q) acc[q] = 0.0;
-      double lv[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-        lv[u] = PL[(size_t)(j0 + imin(g + kGroups * u, wb - 1)) * ldl + row];
-#pragma unroll
-      for (int u = 0; u < 16; ++u)
-#pragma unroll
-        for (int q = 0; q < kRhsChunk; ++q) acc[q] += lv[u] * ys[q][g + kGroups * u];
-#pragma unroll
-      for (int q = 0; q < kRhsChunk; ++q) red[g][q][rr] = acc[q];
-    }
-    __syncthreads();
-    if (r0 < r1)
-      for (int idx = tid; idx < kRhsChunk * kRows; idx += kThreads) {
-        const int q = idx / kRows, i = idx % kRows;
-        if (q < nr && r0 + i < r1)
-          PX[(size_t)(rc + q) * m + r0 + i] -=
-              ((red[0][q][i] + red[1][q][i]) + red[2][q][i]) + red[3][q][i];
-      }
-    __syncthreads();
-  }
+// ---- big fronts: one launch per tree level ---------------------------------
+// Fronts with ne > kBigTri solve their triangle, L11 y = X[0:ne] (forward) or
+// L11^T x = X[0:ne] (backward), in one launch per tree level over 64-column
+// blocks: a workgroup per block of every big front of the level.  Within a
+// front, workgroups take their blocks by ticket (an atomic counter; forward
+// in increasing, backward in decreasing block order), so a workgroup waits
+// only for blocks whose workgroups took their tickets earlier and are
+// running: every wait ends.  Block b's workgroup accumulates the products of
+// the blocks it depends on in block order (not arrival order), solves its
+// diagonal block on one wave per right-hand side, stores the block's
+// solution into X and x, and raises the block's flag.  The flag is a
+// device-scope release store after a workgroup barrier; readers poll it with
+// relaxed device-scope loads and read the published solutions with
+// device-scope loads (coherent across the XCDs' L2 caches).  The sync words
+// (per big front: forward ticket and one flag per block, then the same for
+// the backward solve) live after the call's workspace and are cleared by
+// k_big_sync_clear at the start of every solve.
+__device__ inline double load_dev(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void wait_flag(const int *f) {
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __builtin_amdgcn_s_sleep(1);
+  // the solution loads that follow stay behind the poll (a compiler barrier;
+  // the hardware issues them only after the poll's value has returned)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ inline void raise_flag(int *f) {
+  __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// X[0:ne] = Y and x[perm[i]] = Y[i] after the last forward block
-__global__ void k_big_fwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs,
-                              double *ws) {
-  const SolveNode &P = sn[s];
-  gcint *Pperm = (gcint *)P.perm;
-  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
-  gcdouble *Y = (gcdouble *)solve_P(ws, P, nrhs);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim, m = P.m;
-  if (i >= ne) return;
-  const int v = Pperm[i];
-  for (int r = 0; r < nrhs; ++r) {
-    const double y = Y[(size_t)r * ne + i];
-    PX[(size_t)r * m + i] = y;
-    x[(size_t)r * ldx + v] = y;
-  }
+__global__ void k_big_sync_clear(int *sync, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sync[i] = 0;
 }
 
-// Backward, block [j0, j1) (blocks from the last): every workgroup solves the
-// diagonal block with L_bb^T from X[j0:j1] (complete: every later block has
-// already pushed its contribution); workgroup q then pushes the solution into
-// the columns [64q, 64q + 64) of X[0:j0]:  X[c] -= L[j0:j1, c]^T x_b.
-// Workgroup 0 keeps the solution in Y (the front's partial-sum area).
-__global__ __launch_bounds__(kThreads) void k_big_bwd(const SolveNode *sn, int s, int j0,
-                                                      int nrhs, double *ws) {
-  __shared__ double lb[kBlk][kBlk + 1];
-  __shared__ double xb[kRhsChunk][kBlk];
-  const SolveNode &P = sn[s];
-  gcdouble *PL = (gcdouble *)P.L;
-  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
-  gdouble *Y = (gdouble *)solve_P(ws, P, nrhs);
-  const int ldl = P.ldl, m = P.m, ne = P.nelim, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
-  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6, wave = g;
-  const int c0 = blockIdx.x * kBlk, c1 = imin(j0, c0 + kBlk);  // columns of X to update
-  stage_lower(PL, ldl, j0, wb, lb);
+// forward: thread (lane = row of block b, wave = a quarter of every earlier
+// block's columns); RC right-hand sides per pass
+template <int RC>
+__global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, const int *boff,
+                                                          const int *bslot, const int *bsync,
+                                                          int nbig, double *x, int ldx,
+                                                          int nrhs, double *ws, int *sync) {
+  __shared__ double red[kGroups][RC][kBlk];
+  __shared__ int s_b;
+  const int li = table_find(boff, nbig, blockIdx.x);
+  if (li < 0) return;
+  const SolveNode &P = sn[bslot[li]];
+  int *tick = sync + bsync[li];
+  int *flag = tick + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_b = atomicAdd(tick, 1);
   __syncthreads();
-  for (int rc = 0; rc < nrhs; rc += kRhsChunk) {
-    const int nr = imin(kRhsChunk, nrhs - rc);
-    for (int q = wave; q < kRhsChunk; q += kGroups) {
-      double v = 0.0;
-      if (q < nr) {
-        v = bwd_diag_wave(lb, (lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] : 0.0, lane);
-        if (lane >= wb) v = 0.0;
-        if (blockIdx.x == 0 && lane < wb) Y[(size_t)(rc + q) * ne + j0 + lane] = v;
-      }
-      xb[q][lane] = v;
-    }
-    __syncthreads();
-    if (c0 < c1) {
-      // lane = row t of the block, wave g = columns c0 + g + 4u
+  const int b = s_b;
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  double *PX = solve_X(ws, P, nrhs);
+  const int ldl = P.ldl, m = P.m, ne = P.nelim;
+  const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+  const int row = j0 + imin(lane, wb - 1);
+  for (int rc = 0; rc < nrhs; rc += RC) {
+    const int nr = imin(RC, nrhs - rc);
+    double acc[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+    for (int j = 0; j < b; ++j) {  // every block before b is a full block
+      const int c0 = j * kBlk + 16 * wave;
       double lv[16];
-      const int t = j0 + imin(lane, wb - 1);
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(c0 + g + kGroups * u, c1 - 1) * ldl + t];
-      const int c = c0 + g + kGroups * (lane >> 2);
-      for (int q = 0; q < nr; ++q) {
-        double a[16];
-        const double xt = xb[q][lane];
+      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)(c0 + u) * ldl + row];
+      if (rc == 0) wait_flag(flag + j);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) a[u] = lv[u] * xt;
-        const double sum = wave_reduce_scatter<16>(a, lane);
-        if ((lane & 3) == 0 && c < c1) PX[(size_t)(rc + q) * m + c] -= sum;
+      for (int q = 0; q < RC; ++q) {
+        if (q < nr) {
+          const double *y = PX + (size_t)(rc + q) * m + c0;
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc[q] += lv[u] * load_dev(y + u);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q) red[wave][q][lane] = acc[q];
+    __syncthreads();
+    for (int q = wave; q < nr; q += kGroups) {
+      double *X = PX + (size_t)(rc + q) * m;
+      double v = 0.0;
+      if (lane < wb)
+        v = X[j0 + lane] - (((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
+                            red[3][q][lane]);
+      v = fwd_diag_wave(PL, ldl, j0, wb, v, lane);
+      if (lane < wb) {
+        X[j0 + lane] = v;
+        x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
       }
     }
     __syncthreads();
   }
+  if (tid == 0) raise_flag(flag + b);
 }
 
-// x[perm[i]] = Y[i] after the first block
-__global__ void k_big_bwd_fin(const SolveNode *sn, int s, double *x, int ldx, int nrhs,
-                              double *ws) {
-  const SolveNode &P = sn[s];
+// backward: thread (lane = row of a later block, wave = a quarter of block
+// b's columns); per-lane sums over the later blocks, then one reduce over
+// the lanes
+template <int RC>
+__global__ __launch_bounds__(kThreads) void k_big_bwd_lvl(const SolveNode *sn, const int *boff,
+                                                          const int *bslot, const int *bsync,
+                                                          int nbig, double *x, int ldx,
+                                                          int nrhs, double *ws, int *sync) {
+  __shared__ double lb[kBlk][kBlk + 1];
+  __shared__ double red[RC][kBlk];
+  __shared__ int s_b;
+  const int li = table_find(boff, nbig, blockIdx.x);
+  if (li < 0) return;
+  const SolveNode &P = sn[bslot[li]];
+  const int ne = P.nelim, nb = idiv_up(ne, kBlk);
+  int *tick = sync + bsync[li] + nb + 1;
+  int *flag = tick + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) s_b = nb - 1 - atomicAdd(tick, 1);
+  __syncthreads();
+  const int b = s_b;
+  gcdouble *PL = (gcdouble *)P.L;
   gcint *Pperm = (gcint *)P.perm;
-  gcdouble *Y = (gcdouble *)solve_P(ws, P, nrhs);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, ne = P.nelim;
-  if (i >= ne) return;
-  const int v = Pperm[i];
-  for (int r = 0; r < nrhs; ++r) x[(size_t)r * ldx + v] = Y[(size_t)r * ne + i];
+  double *PX = solve_X(ws, P, nrhs);
+  const int ldl = P.ldl, m = P.m;
+  const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
+  stage_lower(PL, ldl, j0, wb, lb);
+  // the wave's 16 columns of block b (clamped; the surplus sums are dropped)
+  const int c0 = j0 + 16 * wave;
+  for (int rc = 0; rc < nrhs; rc += RC) {
+    const int nr = imin(RC, nrhs - rc);
+    double acc[RC][16];
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[q][u] = 0.0;
+    for (int j = nb - 1; j > b; --j) {
+      const int r0 = j * kBlk, rows = imin(kBlk, ne - r0);
+      const int i = r0 + imin(lane, rows - 1);
+      double lv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(c0 + u, j1 - 1) * ldl + i];
+      if (rc == 0) wait_flag(flag + j);
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        const double xi =
+            (q < nr && lane < rows) ? load_dev(PX + (size_t)(rc + q) * m + i) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[q][u] += lv[u] * xi;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      const double s = wave_reduce_scatter<16>(acc[q], lane);
+      if ((lane & 3) == 0) red[q][16 * wave + (lane >> 2)] = s;
+    }
+    __syncthreads();
+    for (int q = wave; q < nr; q += kGroups) {
+      double *X = PX + (size_t)(rc + q) * m;
+      double v = (lane < wb) ? X[j0 + lane] - red[q][lane] : 0.0;
+      v = bwd_diag_wave(lb, v, lane);
+      if (lane < wb) {
+        X[j0 + lane] = v;
+        x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) raise_flag(flag + b);
 }
 
 // ---- launchers ----------------------------------------------------------------
 int big_tri_threshold() { return kBigTri; }
 int fwd_rows_per_wg() { return kRows; }
 int bwd_rows_per_wg() { return kThreads; }
+int bwd_cols_per_wg() { return kBwdCols; }
 
-void launch_big_fwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
-                    int nrhs, double *ws) {
-  for (int j0 = 0; j0 < ne; j0 += kBlk) {
-    const int rows = ne - std::min(j0 + kBlk, ne);
-    hipLaunchKernelGGL(k_big_fwd, dim3(std::max(1, (rows + kRows - 1) / kRows)),
-                       dim3(kThreads), 0, st, sn, s, j0, nrhs, ws);
-  }
-  hipLaunchKernelGGL(k_big_fwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
-                     nrhs, ws);
+// tab: the level's big fronts, [boff (nbig+1) | slots (nbig) | sync offsets
+// (nbig)]; nwg = boff[nbig], the level's 64-column blocks
+void launch_big_fwd_lvl(hipStream_t st, const SolveNode *sn, const int *tab, int nbig, int nwg,
+                        double *x, int ldx, int nrhs, double *ws, int *sync) {
+  if (nbig <= 0) return;
+  if (nrhs == 1)
+    hipLaunchKernelGGL(k_big_fwd_lvl<1>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
+                       tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
+  else
+    hipLaunchKernelGGL(k_big_fwd_lvl<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
+                       tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
 }
-void launch_big_bwd(hipStream_t st, const SolveNode *sn, int s, int ne, double *x, int ldx,
-                    int nrhs, double *ws) {
-  for (int j0 = (ne - 1) / kBlk * kBlk; j0 >= 0; j0 -= kBlk)
-    hipLaunchKernelGGL(k_big_bwd, dim3(std::max(1, (j0 + kBlk - 1) / kBlk)), dim3(kThreads), 0,
-                       st, sn, s, j0, nrhs, ws);
-  hipLaunchKernelGGL(k_big_bwd_fin, dim3((ne + 255) / 256), dim3(256), 0, st, sn, s, x, ldx,
-                     nrhs, ws);
+void launch_big_bwd_lvl(hipStream_t st, const SolveNode *sn, const int *tab, int nbig, int nwg,
+                        double *x, int ldx, int nrhs, double *ws, int *sync) {
+  if (nbig <= 0) return;
+  // one right-hand side per pass: the per-lane sums of 16 columns fill the
+  // registers already
+  hipLaunchKernelGGL(k_big_bwd_lvl<1>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
+                     tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
+}
+void launch_big_sync_clear(hipStream_t st, int *sync, int n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_big_sync_clear, dim3((n + 255) / 256), dim3(256), 0, st, sync, n);
 }
 
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
@@ -855,24 +922,26 @@ void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, i
     hipLaunchKernelGGL(k_dsolve, dim3(nslots), dim3(64), 0, st, sn, nslots, x, ldx, nrhs,
                        half);
 }
-// rows_off / nrows: the level's 256-row workgroup table and its total
+// rows_off: the level's 256-row workgroup table (per front); chunk_off /
+// nwg: the row workgroups x column chunks table of k_bwd_rows and its total
 void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots, double *x,
-                int ldx, int nrhs, const int *rows_off, int nrows, double *ws) {
+                int ldx, int nrhs, const int *rows_off, const int *chunk_off, int nwg,
+                double *ws, int dfuse) {
   if (nlvl <= 0) return;
-  if (nrows > 0) {
+  if (nwg > 0) {
     if (nrhs == 1)
-      hipLaunchKernelGGL(k_bwd_rows<1>, dim3(nrows), dim3(kThreads), 0, st, sn, slots, rows_off,
+      hipLaunchKernelGGL(k_bwd_rows<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots, chunk_off,
                          nlvl, x, ldx, nrhs, ws);
     else
-      hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nrows), dim3(kThreads), 0, st, sn, slots,
-                         rows_off, nlvl, x, ldx, nrhs, ws);
+      hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
+                         chunk_off, nlvl, x, ldx, nrhs, ws);
   }
   if (nrhs == 1)
     hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs, ws);
+                       ldx, nrhs, ws, dfuse);
   else  // four right-hand sides at a time, one a wave
     hipLaunchKernelGGL(k_bwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs, ws);
+                       ldx, nrhs, ws, dfuse);
 }
 
 }  // namespace ssids_hip

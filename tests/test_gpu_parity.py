@@ -175,6 +175,29 @@ def test_multiple_rhs(lib, nrhs):
     ssids.ssids_free(ak, fk)
 
 
+@pytest.mark.parametrize("nrhs", [1, 3, 9])
+def test_big_front_triangles(lib, nrhs):
+    """Fronts with more than 256 eliminated columns (the top separators of a
+    24^3 grid) solve their triangles in the flag-chained level kernels: every
+    job composes to the full solve, each column equals the single-right-hand-side solve bit for
+    bit, and the raw backward error meets the reference kernels' 1e-13."""
+    m = matrices.laplace3d(24, diag=5.7)
+    ak, fk, finf, opts = gpu_factor(m, grid=(24, 24, 24))
+    ne_big = [node_factors(fk, k)["nelim"] for k in range(len(ak.sptr) - 1)]
+    assert max(ne_big) > 256, max(ne_big)
+    xt, b = rhs(m, nrhs, 40 + nrhs)
+    print(f"big-front solve nrhs={nrhs} n={m.n} max nelim={max(ne_big)}", flush=True)
+    x = gpu_solve(ak, fk, opts, b)
+    assert matrices.bwd_error(m.full, x, b).max() <= RAW_TOL
+    for k in range(nrhs):
+        assert np.array_equal(gpu_solve(ak, fk, opts, b[:, k]), x[:, k]), k
+    y = gpu_solve(ak, fk, opts, b, job=ssids.SSIDS_SOLVE_JOB_FWD)
+    y = gpu_solve(ak, fk, opts, y, job=ssids.SSIDS_SOLVE_JOB_DIAG)
+    y = gpu_solve(ak, fk, opts, y, job=ssids.SSIDS_SOLVE_JOB_BWD)
+    assert np.allclose(y, x, rtol=1e-12, atol=1e-12)
+    ssids.ssids_free(ak, fk)
+
+
 def test_solve_jobs_compose(lib):
     m = matrices.random_indef(250, density=0.03, seed=5)
     ak, fk, finf, opts = gpu_factor(m)
