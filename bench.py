@@ -51,6 +51,9 @@ def parse():
                     help="instead of the default workload: the mixed-precision KKT system "
                          "(27-point H on this grid, grid^3 * 88000/512000 constraint rows with "
                          "4 entries) factored and solved in FP64 and mixed precision")
+    ap.add_argument("--kkt-ordering", type=int, default=2,
+                    help="--kkt27 ordering: 2 matching-based (with its scaling, the "
+                         "reference's choice for saddle-point systems), 4 AMD")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -59,13 +62,14 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC = ROOT / "profiles" / "k_gather_laplace64_traffic.txt"
+TRAFFIC = ROOT / "profiles" / "k_gather2_laplace64_traffic.md"
 
 
 def gather_traffic():
-    """HBM bytes per k_gather launch from the committed PMC summary (FETCH_SIZE
-    + WRITE_SIZE over the factorization's gather launches, scaled by the same
-    counters on a copy of known size), or None when no summary is committed."""
+    """HBM bytes per rest-of-level gather launch (k_gather<2, double>) from the
+    committed PMC summary (FETCH_SIZE + WRITE_SIZE over the factorization's
+    launches, each scaled by the same counter on a copy of known size), or
+    None when no summary is committed."""
     if not TRAFFIC.exists():
         return None
     for line in TRAFFIC.read_text().splitlines():
@@ -176,15 +180,21 @@ def batch_bench(args):
 def kkt27_bench(args):
     """FP64 against mixed precision (FP32 Schur updates) on the KKT system
     [H A^T; A 0] with H a 27-point stencil (N(0,1) couplings, diagonal
-    27 + |N(0,1)|) and A with 4 N(0,1) entries per row, AMD order.  A step
-    is a refactor on the warm handle plus a solve refined in FP64 to 1e-14.
-    One JSON line; `value` is the FP64 factor GFLOP/s."""
+    27 + |N(0,1)|) and A with 4 N(0,1) entries per row, in the
+    matching-based order with its scaling (--kkt-ordering 2, default) or the
+    AMD order (4).  A step is a refactor on the warm handle plus a solve
+    refined in FP64 to 1e-14.  One JSON line; `value` is the FP64 factor
+    GFLOP/s."""
     from spral_amd import devmem, matrices, ssids
     g = args.kkt27
     mc = int(round(g ** 3 * 88000 / 512000))
     m = matrices.kkt_stencil27(g, mc)
-    opts = ssids.SSIDSOptions(ordering=4)
-    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts)
+    ordname = {2: "matching-based order + its scaling", 4: "AMD order"}[args.kkt_ordering]
+    opts = ssids.SSIDSOptions(ordering=args.kkt_ordering,
+                              scaling=3 if args.kkt_ordering == 2 else 0)
+    t_an = time.perf_counter()
+    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, val=m.val)
+    t_an = time.perf_counter() - t_an
     assert ainf.flag >= 0, ainf
     rng = np.random.Generator(np.random.PCG64(20261202 + 3))
     x_true = rng.uniform(-1, 1, m.n)
@@ -224,11 +234,11 @@ def kkt27_bench(args):
         "warmup": args.warmup, "ms_per_step": round(f64["factor_ms"] + f64["solve_ms"], 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"KKT: 27-point H on {g}^3, {mc} constraint rows x 4, AMD order, "
-                               "1 RHS, host buffers",
+        "config": {"workload": f"KKT: 27-point H on {g}^3, {mc} constraint rows x 4, "
+                               f"{ordname}, 1 RHS, host buffers",
                    "n": m.n, "nz": m.nz, "num_flops": int(ak.num_flops),
                    "num_factor": int(ak.num_factor), "maxfront": int(ainf.maxfront),
-                   "parallelism": "single-gpu"},
+                   "analyse_s": round(t_an, 2), "parallelism": "single-gpu"},
         "fp64": f64, "mixed": res[1]}), flush=True)
 
 
@@ -388,6 +398,26 @@ def main():
     xo = np.empty_like(x)
     xo[ak.invp] = x
     bwd = float(matrices.bwd_error(m.full, xo, b)[0])
+    # one unrefined solve (forward, D, backward) against its HBM bytes:
+    # L read once per direction + D + x read/written (3 x 8 n per right-hand side)
+    raw = []
+    d_t = devmem.DeviceArray(m.n)
+    for _ in range(args.solves):
+        devmem.check(devmem.hip().hipMemcpy(d_t.ptr, b2.ptr, m.n * 8, devmem.D2D))
+        sync()
+        t0 = time.perf_counter()
+        f = L.spral_ssids_hip_solve_device(num, 0, 1, d_t.ptr, m.n, None)
+        sync()
+        raw.append((time.perf_counter() - t0) * 1e3)
+        assert f == 0, f
+    d_t.free()
+    raw_ms = float(np.percentile(raw, 50))
+    sbytes = 2 * 8.0 * ak.num_factor + 16.0 * m.n + 24.0 * m.n
+    solve_roof = {"bound": "hbm", "achieved": round(sbytes / (raw_ms * 1e-3) / 1e9, 1),
+                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(sbytes / (raw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                  "bytes": sbytes, "raw_solve_ms_p50": round(raw_ms, 3),
+                  "what": "one unrefined solve (all jobs) on the resident factors"}
     # profiled factorization: events around every Schur-update launch
     sp = refactor(timing=1)
     sync()
@@ -403,7 +433,9 @@ def main():
         roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": PEAK_FP64_MFMA_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_MFMA_TFLOPS, 4)}
     roof.update({"traffic": gather_traffic(),
-                 "kernel": "k_gather (extend-add + lazy Schur update, FP64 MFMA)",
+                 "kernel": "k_gather<2, double>: rest-of-level gather (extend-add + lazy "
+                           "Schur update on FP64 MFMA), timed with events on its own stream "
+                           "inside the production launch sequence",
                  "launches": sp.kernel_launches,
                  "avg_launch_us": round(sp.kernel_ms * 1e3 / max(1, sp.kernel_launches), 2),
                  "bytes_per_launch": sp.kernel_bytes / max(1, sp.kernel_launches),
@@ -468,6 +500,7 @@ def main():
         "num_delay": s.num_delay,
         "num_two": s.num_two,
         "roofline": roof,
+        "solve_roofline": solve_roof,
     }
     if mixed is not None:
         mixed["factor_plus_solve_ms"] = round(mixed["factor_ms"] + mixed["solve_ms_p50"], 3)

@@ -64,7 +64,11 @@ struct spral_ssids_hip_stats {
   int hip_error;           /* hipError_t of the first failing call, else 0  */
   int num_levels;          /* tree levels walked                            */
   double factor_ms;        /* device time of the numeric factorization      */
-  double kernel_ms;        /* summed device time of the gather launches     */
+  double kernel_ms;        /* options.timing: summed device time of the     *
+                            * gather launches (static plan: the rest-of-    *
+                            * level gathers, tiles outside the first 64     *
+                            * columns, timed in the production sequence;    *
+                            * level-by-level path: whole-level gathers)     */
   double kernel_flops;     /* algorithmic flops of those launches           */
   double kernel_bytes;     /* algorithmic HBM bytes of those launches       */
   int kernel_launches;     /* number of gather launches timed               */
@@ -226,6 +230,22 @@ int spral_ssids_hip_alter(void *numeric, const double *d);
  * (SSIDS_ERROR_NOT_LLT) for an indefinite factorization; enquire_indef and
  * alter return -14 (SSIDS_ERROR_NOT_LDLT) for a positive-definite one. */
 int spral_ssids_hip_enquire_posdef(const void *numeric, double *d);
+
+/* Serialised factors (no reference counterpart: the reference's only on-disk
+ * state is rb_dump of the matrix, src/ssids/ssids.f90:883-896).  save writes
+ * the resident factors of `numeric` (L, D^{-1}, pivot order, the factorized
+ * values and scaling kept for refinement, stats) to `path`; load rebuilds a
+ * numeric handle on `symbolic`, which must be an analyse of the same pattern
+ * with the same ordering (checked by a fingerprint of the row lists), whose
+ * solves, enquiries and refinements then equal the original's bit for bit.
+ * save returns 0 or -1 (I/O); load returns NULL with *flag -1 (I/O or a
+ * malformed file), -2 (another analyse), -51 (device) or -50 (host memory). */
+int spral_ssids_hip_save_numeric(const void *numeric, const char *path);
+void *spral_ssids_hip_load_numeric(void *symbolic, const char *path,
+                                   struct spral_ssids_hip_stats *stats, int *flag);
+/* The scaling a numeric handle factorized with (n, elimination order) into s:
+ * returns 1, or 0 when it has none (-51 on a device error). */
+int spral_ssids_hip_numeric_scaling(const void *numeric, double *s);
 
 /* Contribution block of the subtree root for a parent subtree held elsewhere;
  * same out-parameters as spral_ssids_cpu_subtree_get_contrib_dbl

@@ -13,7 +13,8 @@ equilibrated in one step to 1/sqrt|a_ii|; a 2x2 matrix has a closed-form
 fixed point) and against the reference test's property (every scaled row's
 infinity norm within 0.05 of 1, tests/scaling.f90:455-470).
 
-ref_equilib_scale_sym / ref_auction_scale_sym call the reference routines
+ref_equilib_scale_sym / ref_auction_scale_sym / ref_hungarian_scale_sym call
+the reference routines
 through the shim oracle/ref_scaling_shim.f90 when oracle/_ref is built.
 """
 import ctypes as C
@@ -67,6 +68,7 @@ def _reflib():
         vp = C.c_void_p
         L.ref_equilib_scale_sym.argtypes = [C.c_int, vp, vp, vp, vp, C.c_int, C.c_float, vp, vp]
         L.ref_auction_scale_sym.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ref_hungarian_scale_sym.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_int, vp, vp]
         _ref = L
     return _ref
 
@@ -83,3 +85,19 @@ def ref_equilib_scale_sym(n, ptr, row, val, max_iterations=10, tol=np.float32(1e
                             s.ctypes.data, max_iterations, float(np.float32(tol)),
                             C.byref(it), C.byref(fl))
     return s[:n], it.value, fl.value
+
+
+def ref_hungarian_scale_sym(n, ptr, row, val, scale_if_singular=False):
+    """The reference's hungarian_scale_sym itself: (scaling, match, matched,
+    flag); match 1-based, 0 unmatched."""
+    L = _reflib()
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    row = np.ascontiguousarray(row, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    s = np.zeros(max(n, 1))
+    mt = np.zeros(max(n, 1), np.int32)
+    nm, fl = C.c_int(0), C.c_int(0)
+    L.ref_hungarian_scale_sym(n, ptr.ctypes.data, row.ctypes.data, val.ctypes.data,
+                              s.ctypes.data, mt.ctypes.data, 1 if scale_if_singular else 0,
+                              C.byref(nm), C.byref(fl))
+    return s[:n], mt[:n], nm.value, fl.value

@@ -94,6 +94,14 @@ SIGNATURES = {
     "spral_ssids_hip_solve_bwd": (C.c_int, [C.c_void_p, C.c_int, f64p, C.c_int]),
     "spral_ssids_hip_solve_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                                C.c_void_p, C.c_int, C.c_void_p]),
+    "spral_hungarian_scale_sym": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int,
+                                            C.POINTER(C.c_int)]),  # raw pointers
+    "spral_match_order": (C.c_int, [C.c_int, i64p, i32p, f64p, i32p, f64p]),
+    "spral_ssids_hip_save_numeric": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "spral_ssids_hip_load_numeric": (C.c_void_p, [C.c_void_p, C.c_char_p, C.c_void_p,
+                                                  C.POINTER(C.c_int)]),
+    "spral_ssids_hip_numeric_scaling": (C.c_int, [C.c_void_p, f64p]),
     "spral_ssids_hip_solve_device_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
                                                        C.c_void_p, C.c_int, C.c_void_p]),
     "spral_ssids_hip_solve_workspace_doubles": (C.c_int64, [C.c_void_p, C.c_int]),

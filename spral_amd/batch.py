@@ -2,8 +2,8 @@
 
 The systems are put together as one block-diagonal matrix. Each block keeps
 its own fill-reducing order (exact minimum degree, or approximate minimum
-degree for large blocks), and the blocks are contiguous in the elimination
-order. The assembly tree is then a forest with one tree per system, and every
+degree for large blocks, or the matching-based order with its scaling), and
+the blocks are contiguous in the elimination order. The assembly tree is then a forest with one tree per system, and every
 level's launches cover the fronts of all systems at once: one launch sequence
 per tree level for the whole batch, instead of one per level per system.
 
@@ -30,6 +30,7 @@ class Batch:
     fkeep: ssids.FKeep = None
     options: ssids.SSIDSOptions = None
     orders: list = field(default_factory=list)   # per-system order (1-based)
+    scale: np.ndarray = None     # matching scalings of ordering 2 + scaling 3, concatenated
 
 
 def block_diagonal(mats):
@@ -52,29 +53,41 @@ def block_diagonal(mats):
 
 
 def system_order(m, options):
-    """Fill-reducing order of one system, as ssids_analyse would pick it."""
+    """Fill-reducing order of one system, as ssids_analyse would pick it
+    (ordering 2: the matching-based order, returned with its scaling)."""
     L = _lib.lib()
     o = np.zeros(m.n, np.int32)
     ptr = np.ascontiguousarray(m.ptr, np.int64)
     row = np.ascontiguousarray(m.row, np.int32)
+    if options.ordering == 2:
+        sc = np.zeros(m.n)
+        L.spral_match_order(m.n, ptr, row, np.ascontiguousarray(m.val, np.float64), o, sc)
+        return o, sc
     if options.ordering == 4 or m.n > ssids.AMD_THRESHOLD:
         L.spral_ssids_hip_order_amd(m.n, ptr, row, o)
     else:
         L.spral_ssids_hip_order_min_degree(m.n, ptr, row, o)
-    return o
+    return o, None
 
 
 def batch_analyse(mats, options=None):
     """One analyse of the block-diagonal pattern; returns (Batch, inform)."""
     options = options or ssids.SSIDSOptions()
     n, ptr, row, val, sizes, offsets = block_diagonal(mats)
-    orders = [system_order(m, options) for m in mats]
+    res = [system_order(m, options) for m in mats]
+    orders = [o for o, _ in res]
     order = np.concatenate([o + int(off) for o, off in zip(orders, offsets[:-1])]) \
         if orders else np.zeros(0, np.int32)
-    opts = ssids.SSIDSOptions(**{**options.__dict__, "ordering": 0})
+    scale = None
+    over = {"ordering": 0}
+    if options.ordering == 2 and options.scaling == 3:
+        # each system's matching scaling, passed as a user scaling of the whole
+        scale = np.concatenate([sc for _, sc in res]) if res else np.zeros(0)
+        over["scaling"] = 0
+    opts = ssids.SSIDSOptions(**{**options.__dict__, **over})
     ak, inf = ssids.ssids_analyse(False, n, ptr, row, opts, order=order)
     return Batch(sizes=sizes, offsets=offsets, akeep=ak, options=opts,
-                 orders=orders), inf
+                 orders=orders, scale=scale), inf
 
 
 def batch_factor(batch, mats_or_vals, posdef=False):
@@ -85,7 +98,8 @@ def batch_factor(batch, mats_or_vals, posdef=False):
         val = mats_or_vals
     else:
         val = np.concatenate([np.asarray(m.val, np.float64) for m in mats_or_vals])
-    fk, inf = ssids.ssids_factor(posdef, val, batch.akeep, batch.options, fkeep=batch.fkeep)
+    fk, inf = ssids.ssids_factor(posdef, val, batch.akeep, batch.options, fkeep=batch.fkeep,
+                                 scale=None if batch.scale is None else batch.scale.copy())
     batch.fkeep = fk
     if inf.flag < 0:
         return inf, None, None

@@ -189,6 +189,7 @@ struct Symbolic {
   std::vector<int> cbeg, cend;            // per node (node-indexed)
   std::vector<int64_t> row_off, rmap_off, acol_off;
   std::vector<int> rows_h;                // 0-based elimination indices
+  std::vector<int> rmap_h;                // host copy of the child -> parent row map
   int64_t naval_min = 0;                  // max A index referenced + 1
   int64_t num_flops = 0;
   // device
@@ -438,6 +439,7 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   HIPCHK(hipSetDevice(device));
   S->d_rows = dev_upload(S->rows_h, S->owned);
   S->d_rmap = dev_upload(rmap, S->owned);
+  S->rmap_h = rmap;
   S->d_asrc = dev_upload(asrc, S->owned);
   S->d_arow = dev_upload(arow, S->owned);
   S->d_acix = dev_upload(acix, S->owned);
@@ -674,16 +676,12 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   // block and the rows below it) on `st`, the other tiles on `st2`.  The
   // first step's pivot block and apply touch only the first 64 columns, so
   // they overlap the rest of the gather; `st` joins `st2` before the first
-  // trailing update.  A timing run keeps one whole-level gather per level
-  // (measured alone).
-  const bool split = !opt.timing;
-  hipStream_t st2 = nullptr;
-  if (split) {
-    if (!N.stream2) HIPCHK(hipStreamCreate(&N.stream2));
-    if (!N.ev_level) HIPCHK(hipEventCreateWithFlags(&N.ev_level, hipEventDisableTiming));
-    if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
-    st2 = N.stream2;
-  }
+  // trailing update.  A timing run (opt.timing) launches the same sequence
+  // eagerly, with events on `st2` around each level's rest-of-level gather.
+  if (!N.stream2) HIPCHK(hipStreamCreate(&N.stream2));
+  if (!N.ev_level) HIPCHK(hipEventCreateWithFlags(&N.ev_level, hipEventDisableTiming));
+  if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
+  hipStream_t st2 = N.stream2;
   // the whole launch sequence (no host round trip, no memset: the counters
   // are cleared before it), captured once as a graph when not timing
   auto enqueue = [&]() {
@@ -698,29 +696,23 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     const int *d_slots = N.d_slots_all + s0;
     launch_level_setup(st, N.d_nodes, s0, nl, S.d_child, fp.pivot_tpp, ms, tab, overflow);
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
-    bool joined = true;
-    if (split) {
-      HIPCHK(hipEventRecord(N.ev_level, st));
-      HIPCHK(hipStreamWaitEvent(st2, N.ev_level, 0));
-      // the first tile column, and step 0's pivot blocks in its diagonal tiles
-      launch_gather_first(st, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
-                          S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp);
-      launch_gather(st2, 2, lv.cap[2 * ms + 4], N.d_nodes, d_slots, toff(2 * ms + 4), nl,
-                    S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
-      HIPCHK(hipEventRecord(N.ev_rest, st2));
-      joined = false;
-    } else {
-      if (opt.timing) tev.emplace_back(timed_event(st));
-      launch_gather(st, 0, lv.cap[0], N.d_nodes, d_slots, toff(0), nl, S.d_child, N.d_aval,
-                    S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
-      if (opt.timing) tev.emplace_back(timed_event(st));
-    }
+    HIPCHK(hipEventRecord(N.ev_level, st));
+    HIPCHK(hipStreamWaitEvent(st2, N.ev_level, 0));
+    // the first tile column, and step 0's pivot blocks in its diagonal tiles
+    launch_gather_first(st, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
+                        S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp);
+    if (opt.timing) tev.emplace_back(timed_event(st2));
+    launch_gather(st2, 2, lv.cap[2 * ms + 4], N.d_nodes, d_slots, toff(2 * ms + 4), nl,
+                  S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
+    if (opt.timing) tev.emplace_back(timed_event(st2));
+    HIPCHK(hipEventRecord(N.ev_rest, st2));
+    bool joined = false;
     auto join = [&]() {
       if (!joined) HIPCHK(hipStreamWaitEvent(st, N.ev_rest, 0));
       joined = true;
     };
     for (int stp = 0; stp < ms; ++stp) {
-      if (stp > 0 || !split) launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      if (stp > 0) launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
       launch_apply(st, lv.cap[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
       // one step (every n <= 64): the update stays inside the first tile
       // column too; with more steps it reaches the later columns
@@ -739,7 +731,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   }
   launch_pack_nodes(st, N.d_nodes, nn, P.d_pack);
   };
-  if (split && !graphs_disabled()) {
+  if (!opt.timing && !graphs_disabled()) {
     // a graph is valid for one plan, one set of pivot parameters and the
     // same value and scaling buffers
     const std::vector<double> key = {(double)P.gen, fp.u, fp.small, (double)fp.action,
@@ -908,141 +900,12 @@ static bool plan_disabled() {
   return e && *e && *e != '0';
 }
 
-static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
-                        const spral_ssids_hip_options &opt, int64_t naval) {
+// Solve descriptors and launch tables of a factorization (the factors in
+// N.nodes): per-front solve views, row-workgroup tables, the workspace
+// layout and the big-front level tables.
+static void build_solve_tables(Numeric &N) {
   Symbolic &S = *N.symb;
-  // Schur-update precision: 0 FP64, 1 FP32 (mixed); FP16 is not built
-  if (opt.schur_precision < 0 || opt.schur_precision > 1) {
-    N.stats.flag = -98;  // SSIDS_ERROR_UNIMPLEMENTED
-    return;
-  }
-  N.c32 = (opt.schur_precision == 1);
-  // keep the values (and scaling) for refinement residuals
-  if ((size_t)naval > N.d_aval_cap) {
-    if (N.d_aval) HIPCHK(hipFree(N.d_aval));
-    HIPCHK(hipMalloc(&N.d_aval, std::max<int64_t>(naval, 1) * sizeof(double)));
-    N.d_aval_cap = naval;
-  }
-  if (naval > 0 && d_aval != N.d_aval)
-    HIPCHK(hipMemcpyAsync(N.d_aval, d_aval, naval * sizeof(double), hipMemcpyDeviceToDevice,
-                          N.stream));
-  if (d_scaling) {
-    if (!N.d_scal) HIPCHK(hipMalloc(&N.d_scal, std::max(S.n, 1) * sizeof(double)));
-    HIPCHK(hipMemcpyAsync(N.d_scal, d_scaling, S.n * sizeof(double), hipMemcpyDeviceToDevice,
-                          N.stream));
-  } else if (N.d_scal) {
-    HIPCHK(hipFree(N.d_scal));
-    N.d_scal = nullptr;
-  }
-  N.anorm = -1.0;
   const int nn = S.nnodes;
-  hipStream_t st = N.stream;
-  FactorParams fp;
-  fp.u = opt.u;
-  fp.small = opt.small;
-  fp.action = opt.action;
-  fp.tpp_failed = (opt.failed_pivot_method != 2);
-  fp.pivot_tpp = (opt.pivot_method == 3) && !N.posdef;
-  fp.posdef = N.posdef;
-  fp.c32 = N.c32;
-  if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
-  if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
-  if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
-  if (!N.d_slots_all) {
-    std::vector<int> iota(std::max(nn, 1));
-    for (int i = 0; i < nn; ++i) iota[i] = i;
-    HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
-  }
-  HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
-  std::vector<hipEvent_t> tev;  // pairs around the gather launches
-  hipEvent_t ev0, ev1;
-  HIPCHK(hipEventCreate(&ev0));
-  HIPCHK(hipEventCreate(&ev1));
-  HIPCHK(hipEventRecord(ev0, st));
-  bool done = false;
-  if (!plan_disabled()) {
-    if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp || N.plan.c32 != N.c32)
-      plan_build(N, fp.pivot_tpp, opt.multiplier);
-    done = factor_static(N, fp, opt, tev);
-    if (!done) {
-      for (auto e : tev) (void)hipEventDestroy(e);
-      tev.clear();
-      plan_release(N);  // rebuilt with the delays seen, on the next factorization
-    }
-  }
-  if (!done) factor_dynamic(N, fp, opt, tev);
-  const int nlev = (int)S.lvl_ptr.size() - 1;
-  int maxfront = 0;
-  for (int sl = 0; sl < nn; ++sl) maxfront = std::max(maxfront, N.nodes[sl].m);
-  launch_stats(st, N.d_nodes, nn, N.d_counters, fp);
-  int cnt[16];
-  HIPCHK(hipMemcpyAsync(cnt, N.d_counters, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipEventRecord(ev1, st));
-  HIPCHK(hipStreamSynchronize(st));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-  HIPCHK(hipEventDestroy(ev0));
-  HIPCHK(hipEventDestroy(ev1));
-  spral_ssids_hip_stats &o = N.stats;
-  if (o.flag >= 0) {
-    o.num_neg = cnt[0];
-    o.num_two = cnt[1];
-    o.num_zero = cnt[2];
-    o.num_delay = cnt[3];
-    o.not_first_pass = cnt[4];
-    o.not_second_pass = cnt[7];
-    o.flag = (cnt[2] > 0) ? 7 : 0;  // SSIDS_WARNING_FACT_SINGULAR
-    if (cnt[5] > 0) o.flag = -5;
-    if (cnt[6] > 0) o.flag = -6;  // SSIDS_ERROR_NOT_POS_DEF
-  }
-  int64_t nf = 0, fl = 0;
-  int maxsn = 0;
-  for (int sl = 0; sl < nn; ++sl) {
-    const NodeDev &D = N.nodes[sl];
-    for (int64_t j = D.m; j >= D.m - D.nelim + 1; --j) {
-      nf += j;
-      fl += j * j;
-    }
-    maxsn = std::max(maxsn, S.sn[S.node_of[sl]]);
-  }
-  o.num_factor = nf;
-  o.num_flops = fl;
-  o.maxfront = maxfront;
-  o.maxsupernode = maxsn;
-  o.num_levels = nlev;
-  o.factor_ms = ms;
-  o.pool_peak_bytes = (int64_t)N.pool.peak();
-  o.static_plan = done ? 1 : 0;
-  o.kernel_ms = 0.0;
-  o.kernel_launches = 0;
-  for (size_t k = 0; k + 1 < tev.size(); k += 2) {
-    float e = 0.f;
-    HIPCHK(hipEventElapsedTime(&e, tev[k], tev[k + 1]));
-    o.kernel_ms += e;
-    o.kernel_launches++;
-  }
-  for (auto e : tev) (void)hipEventDestroy(e);
-  // algorithmic work of the gather kernels: each child's Schur complement
-  // C_c - L21 (L21 D)^T on the lower triangle (K*N*(N+1) flops, K = nelim,
-  // N = m-n), and HBM bytes = parent fronts written once (lower part) +
-  // children's contribution blocks (lower), L21 and L21 D panels and
-  // delayed columns read once.
-  double gf = 0.0, gb = 0.0;
-  for (int sl = 0; sl < nn; ++sl) {
-    const NodeDev &D = N.nodes[sl];
-    double m = D.m, n = D.n;
-    gb += 8.0 * (n * (n + 1) / 2 + (m - n) * n + (m - n) * (m - n + 1) / 2);  // written
-    if (D.parent >= 0) {
-      double K = D.nelim, Nc = m - n;
-      gf += K * Nc * (Nc + 1);
-      gb += 8.0 * (Nc * (Nc + 1) / 2 + 2 * Nc * K + (n - D.nelim) * m);  // read
-    }
-  }
-  o.kernel_flops = gf;
-  o.kernel_bytes = gb;
-  // solve descriptors
   N.sn_h.assign(nn, SolveNode{});
   std::vector<int> doff(nn, 0);
   for (int i = 0; i < nn; ++i) {
@@ -1177,6 +1040,170 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
                        hipMemcpyHostToDevice));
     }
   }
+}
+
+static void factor_impl(Numeric &N, const double *d_aval, const double *d_scaling,
+                        const spral_ssids_hip_options &opt, int64_t naval) {
+  Symbolic &S = *N.symb;
+  // Schur-update precision: 0 FP64, 1 FP32 (mixed); FP16 is not built
+  if (opt.schur_precision < 0 || opt.schur_precision > 1) {
+    N.stats.flag = -98;  // SSIDS_ERROR_UNIMPLEMENTED
+    return;
+  }
+  N.c32 = (opt.schur_precision == 1);
+  // keep the values (and scaling) for refinement residuals
+  if ((size_t)naval > N.d_aval_cap) {
+    if (N.d_aval) HIPCHK(hipFree(N.d_aval));
+    HIPCHK(hipMalloc(&N.d_aval, std::max<int64_t>(naval, 1) * sizeof(double)));
+    N.d_aval_cap = naval;
+  }
+  if (naval > 0 && d_aval != N.d_aval)
+    HIPCHK(hipMemcpyAsync(N.d_aval, d_aval, naval * sizeof(double), hipMemcpyDeviceToDevice,
+                          N.stream));
+  if (d_scaling) {
+    if (!N.d_scal) HIPCHK(hipMalloc(&N.d_scal, std::max(S.n, 1) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(N.d_scal, d_scaling, S.n * sizeof(double), hipMemcpyDeviceToDevice,
+                          N.stream));
+  } else if (N.d_scal) {
+    HIPCHK(hipFree(N.d_scal));
+    N.d_scal = nullptr;
+  }
+  N.anorm = -1.0;
+  const int nn = S.nnodes;
+  hipStream_t st = N.stream;
+  FactorParams fp;
+  fp.u = opt.u;
+  fp.small = opt.small;
+  fp.action = opt.action;
+  fp.tpp_failed = (opt.failed_pivot_method != 2);
+  fp.pivot_tpp = (opt.pivot_method == 3) && !N.posdef;
+  fp.posdef = N.posdef;
+  fp.c32 = N.c32;
+  if (!N.d_nodes) HIPCHK(hipMalloc(&N.d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
+  if (!N.d_counters) HIPCHK(hipMalloc(&N.d_counters, 16 * sizeof(int)));
+  if (!N.d_scratch_n) HIPCHK(hipMalloc(&N.d_scratch_n, std::max(S.n, 1) * sizeof(int)));
+  if (!N.d_slots_all) {
+    std::vector<int> iota(std::max(nn, 1));
+    for (int i = 0; i < nn; ++i) iota[i] = i;
+    HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
+  std::vector<hipEvent_t> tev;  // pairs around the gather launches
+  hipEvent_t ev0, ev1;
+  HIPCHK(hipEventCreate(&ev0));
+  HIPCHK(hipEventCreate(&ev1));
+  HIPCHK(hipEventRecord(ev0, st));
+  bool done = false;
+  if (!plan_disabled()) {
+    if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp || N.plan.c32 != N.c32)
+      plan_build(N, fp.pivot_tpp, opt.multiplier);
+    done = factor_static(N, fp, opt, tev);
+    if (!done) {
+      for (auto e : tev) (void)hipEventDestroy(e);
+      tev.clear();
+      plan_release(N);  // rebuilt with the delays seen, on the next factorization
+    }
+  }
+  if (!done) factor_dynamic(N, fp, opt, tev);
+  const int nlev = (int)S.lvl_ptr.size() - 1;
+  int maxfront = 0;
+  for (int sl = 0; sl < nn; ++sl) maxfront = std::max(maxfront, N.nodes[sl].m);
+  launch_stats(st, N.d_nodes, nn, N.d_counters, fp);
+  int cnt[16];
+  HIPCHK(hipMemcpyAsync(cnt, N.d_counters, 16 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ev1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+  HIPCHK(hipEventDestroy(ev0));
+  HIPCHK(hipEventDestroy(ev1));
+  spral_ssids_hip_stats &o = N.stats;
+  if (o.flag >= 0) {
+    o.num_neg = cnt[0];
+    o.num_two = cnt[1];
+    o.num_zero = cnt[2];
+    o.num_delay = cnt[3];
+    o.not_first_pass = cnt[4];
+    o.not_second_pass = cnt[7];
+    o.flag = (cnt[2] > 0) ? 7 : 0;  // SSIDS_WARNING_FACT_SINGULAR
+    if (cnt[5] > 0) o.flag = -5;
+    if (cnt[6] > 0) o.flag = -6;  // SSIDS_ERROR_NOT_POS_DEF
+  }
+  int64_t nf = 0, fl = 0;
+  int maxsn = 0;
+  for (int sl = 0; sl < nn; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    for (int64_t j = D.m; j >= D.m - D.nelim + 1; --j) {
+      nf += j;
+      fl += j * j;
+    }
+    maxsn = std::max(maxsn, S.sn[S.node_of[sl]]);
+  }
+  o.num_factor = nf;
+  o.num_flops = fl;
+  o.maxfront = maxfront;
+  o.maxsupernode = maxsn;
+  o.num_levels = nlev;
+  o.factor_ms = ms;
+  o.pool_peak_bytes = (int64_t)N.pool.peak();
+  o.static_plan = done ? 1 : 0;
+  o.kernel_ms = 0.0;
+  o.kernel_launches = 0;
+  for (size_t k = 0; k + 1 < tev.size(); k += 2) {
+    float e = 0.f;
+    HIPCHK(hipEventElapsedTime(&e, tev[k], tev[k + 1]));
+    o.kernel_ms += e;
+    o.kernel_launches++;
+  }
+  for (auto e : tev) (void)hipEventDestroy(e);
+  // algorithmic work of the gather kernels: each child's Schur complement
+  // C_c - L21 (L21 D)^T on the lower triangle (K*N*(N+1) flops, K = nelim,
+  // N = m-n), and HBM bytes = parent fronts written once (lower part) +
+  // children's contribution blocks (lower), L21 and L21 D panels and
+  // delayed columns read once.
+  // On the static plan the timed launches are the rest-of-level gathers
+  // (k_gather part 2: every parent tile outside the first 64-column tile
+  // column), so only that part is counted: a child's contribution entry
+  // (i, j), i >= j, lands there when its parent column P(j) >= 64 (P is
+  // monotone, so that is every j from the first such j*).
+  double gf = 0.0, gb = 0.0;
+  const bool part2 = done;
+  for (int sl = 0; sl < nn; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    double m = D.m, n = D.n;
+    if (!part2) {
+      gb += 8.0 * (n * (n + 1) / 2 + (m - n) * n + (m - n) * (m - n + 1) / 2);  // written
+    } else if (m > kBlk) {
+      gb += 8.0 * (m - kBlk) * (m - kBlk + 1) / 2;  // written, columns >= 64
+    }
+    if (D.parent < 0) continue;
+    const double K = D.nelim;
+    if (!part2) {
+      const double Nc = m - n;
+      gf += K * Nc * (Nc + 1);
+      gb += 8.0 * (Nc * (Nc + 1) / 2 + 2 * Nc * K + (n - D.nelim) * m);  // read
+      continue;
+    }
+    const int node = S.node_of[sl], pn = N.nodes[D.parent].nd;
+    const int Nc = D.m - D.n;
+    int js = Nc;  // first contribution index with parent column >= 64
+    for (int j = 0; j < Nc; ++j)
+      if (pn + S.rmap_h[S.rmap_off[node] + j] >= kBlk) {
+        js = j;
+        break;
+      }
+    const double r = Nc - js;
+    gf += K * r * (r + 1);
+    gb += 8.0 * (r * (r + 1) / 2 + 2 * r * K);  // C and the L21 / L21 D rows read
+    int ndl = 0;  // delayed columns landing at parent column >= 64 (copied)
+    for (int k = D.nelim; k < D.n; ++k) ndl += (D.doff + (k - D.nelim) >= kBlk) ? 1 : 0;
+    gb += 8.0 * ndl * m;
+  }
+  o.kernel_flops = gf;
+  o.kernel_bytes = gb;
+  build_solve_tables(N);
 }
 
 // Doubles of solve workspace for nrhs right-hand sides: the per-front X and
@@ -1383,6 +1410,269 @@ static int refine_device(Numeric &N, int nrhs, const double *d_b, double *d_x, i
   for (int k = 0; k < nrhs; ++k) converged = converged && active[k] == 0;
   if (nonfinite) return SPRAL_SSIDS_HIP_ERROR_NONFINITE;
   return converged ? 0 : SPRAL_SSIDS_HIP_WARNING_NOT_CONVERGED;
+}
+
+// ---- serialised factors ------------------------------------------------------
+// File layout (native endianness): magic "SSIDSHIP", version, the handle's
+// shape (n, nodes, posdef, FP32 contribution blocks, scaling present, values
+// count), a fingerprint of the symbolic structure (row lists), the stats,
+// the factorized values and scaling (for refinement residuals), then per
+// front in slot order its sizes and state, L (ldl x n), D^{-1} (2n), the
+// pivoted variable list, the pre-pivot -> post-pivot positions, the forward
+// solve's child tile table and, for a root that keeps one, its contribution
+// block.
+namespace {
+constexpr char kMagic[8] = {'S', 'S', 'I', 'D', 'S', 'H', 'I', 'P'};
+constexpr int kVersion = 1;
+
+uint64_t symbolic_fingerprint(const Symbolic &S) {
+  uint64_t h = 1469598103934665603ull;  // FNV-1a over the row lists and sizes
+  auto mix = [&](uint64_t v) {
+    for (int b = 0; b < 8; ++b) {
+      h ^= (v >> (8 * b)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  mix((uint64_t)S.n);
+  mix((uint64_t)S.nnodes);
+  for (int i = 0; i < S.nnodes; ++i) {
+    mix((uint64_t)S.sn[i]);
+    mix((uint64_t)S.sm[i]);
+  }
+  for (int v : S.rows_h) mix((uint64_t)(uint32_t)v);
+  return h;
+}
+
+struct FileW {
+  FILE *f;
+  bool ok = true;
+  template <class T> void put(const T *p, size_t n) {
+    if (ok && n && fwrite(p, sizeof(T), n, f) != n) ok = false;
+  }
+  template <class T> void put1(const T &v) { put(&v, 1); }
+};
+struct FileR {
+  FILE *f;
+  bool ok = true;
+  template <class T> void get(T *p, size_t n) {
+    if (ok && n && fread(p, sizeof(T), n, f) != n) ok = false;
+  }
+  template <class T> T get1() {
+    T v{};
+    get(&v, 1);
+    return v;
+  }
+};
+
+size_t ctab_ints(const Symbolic &S, int sl, int m) {
+  const int i = S.node_of[sl];
+  return (size_t)(cdiv(m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
+}
+}  // namespace
+
+static int save_numeric(const Numeric &N, const char *path) {
+  const Symbolic &S = *N.symb;
+  HIPCHK(hipSetDevice(S.device));
+  HIPCHK(hipStreamSynchronize(N.stream));
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return -1;
+  FileW w{f};
+  w.put(kMagic, 8);
+  w.put1(kVersion);
+  const int hdr[5] = {S.n, S.nnodes, N.posdef ? 1 : 0, N.c32, N.d_scal ? 1 : 0};
+  w.put(hdr, 5);
+  const int64_t naval = (int64_t)N.d_aval_cap;
+  w.put1(naval);
+  w.put1(symbolic_fingerprint(S));
+  w.put1(N.stats);
+  std::vector<double> hv(std::max<int64_t>(naval, S.n));
+  if (naval > 0) {
+    HIPCHK(hipMemcpy(hv.data(), N.d_aval, naval * sizeof(double), hipMemcpyDeviceToHost));
+    w.put(hv.data(), naval);
+  }
+  if (N.d_scal) {
+    HIPCHK(hipMemcpy(hv.data(), N.d_scal, S.n * sizeof(double), hipMemcpyDeviceToHost));
+    w.put(hv.data(), S.n);
+  }
+  std::vector<double> dbuf;
+  std::vector<int> ibuf;
+  std::vector<char> cbuf;
+  for (int sl = 0; sl < S.nnodes && w.ok; ++sl) {
+    const NodeDev &D = N.nodes[sl];
+    const int st[10] = {D.m, D.n, D.nd, D.nelim, D.nelim_app, D.flag, D.nzero, D.nsteps, D.ldl,
+                        D.ldc};
+    w.put(st, 10);
+    dbuf.resize((size_t)D.ldl * D.n + 2 * (size_t)D.n);
+    if (D.n > 0) {
+      HIPCHK(hipMemcpy(dbuf.data(), D.L, (size_t)D.ldl * D.n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(dbuf.data() + (size_t)D.ldl * D.n, D.D, 2 * (size_t)D.n * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      w.put(dbuf.data(), dbuf.size());
+      ibuf.resize(2 * (size_t)D.n);
+      HIPCHK(hipMemcpy(ibuf.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ibuf.data() + D.n, D.ipos, D.n * sizeof(int), hipMemcpyDeviceToHost));
+      w.put(ibuf.data(), ibuf.size());
+    }
+    const size_t tn = ctab_ints(S, sl, D.m);
+    ibuf.resize(tn);
+    HIPCHK(hipMemcpy(ibuf.data(), D.ctab, tn * sizeof(int), hipMemcpyDeviceToHost));
+    w.put(ibuf.data(), tn);
+    const int cm = D.m - D.n;
+    const size_t cb = (D.is_root && cm > 0 && D.C)
+                          ? (size_t)D.ldc * cm * (N.c32 ? sizeof(float) : sizeof(double))
+                          : 0;
+    w.put1((int64_t)cb);
+    if (cb) {
+      cbuf.resize(cb);
+      HIPCHK(hipMemcpy(cbuf.data(), D.C, cb, hipMemcpyDeviceToHost));
+      w.put(cbuf.data(), cb);
+    }
+  }
+  const bool ok = w.ok && std::fclose(f) == 0;
+  return ok ? 0 : -1;
+}
+
+static Numeric *load_numeric(Symbolic *Sp, const char *path, int *flag) {
+  Symbolic &S = *Sp;
+  *flag = 0;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) {
+    *flag = -1;
+    return nullptr;
+  }
+  FileR r{f};
+  char magic[8];
+  r.get(magic, 8);
+  const int ver = r.get1<int>();
+  int hdr[5];
+  r.get(hdr, 5);
+  const int64_t naval = r.get1<int64_t>();
+  const uint64_t fp = r.get1<uint64_t>();
+  if (!r.ok || std::memcmp(magic, kMagic, 8) != 0 || ver != kVersion) {
+    std::fclose(f);
+    *flag = -1;
+    return nullptr;
+  }
+  if (hdr[0] != S.n || hdr[1] != S.nnodes || fp != symbolic_fingerprint(S)) {
+    std::fclose(f);
+    *flag = -2;  // factors of another analyse
+    return nullptr;
+  }
+  auto *N = new Numeric();
+  N->symb = &S;
+  N->posdef = hdr[2] != 0;
+  N->c32 = hdr[3];
+  try {
+    HIPCHK(hipSetDevice(S.device));
+    HIPCHK(hipStreamCreate(&N->stream));
+    N->stats = r.get1<spral_ssids_hip_stats>();
+    std::vector<double> hv(std::max<int64_t>(naval, S.n));
+    HIPCHK(hipMalloc(&N->d_aval, std::max<int64_t>(naval, 1) * sizeof(double)));
+    N->d_aval_cap = naval;
+    if (naval > 0) {
+      r.get(hv.data(), naval);
+      HIPCHK(hipMemcpy(N->d_aval, hv.data(), naval * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (hdr[4]) {
+      r.get(hv.data(), S.n);
+      HIPCHK(hipMalloc(&N->d_scal, std::max(S.n, 1) * sizeof(double)));
+      HIPCHK(hipMemcpy(N->d_scal, hv.data(), S.n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const int nn = S.nnodes;
+    N->nodes.assign(nn, NodeDev{});
+    N->nd_seen.assign(nn, 0);
+    auto keep = [&](size_t bytes) {
+      void *p = N->pool.alloc(std::max<size_t>(bytes, 1));
+      N->persistent.push_back({p, std::max<size_t>(bytes, 1)});
+      return p;
+    };
+    std::vector<double> dbuf;
+    std::vector<int> ibuf;
+    std::vector<char> cbuf;
+    for (int sl = 0; sl < nn && r.ok; ++sl) {
+      NodeDev &D = N->nodes[sl];
+      node_static(S, sl, D);
+      int st[10];
+      r.get(st, 10);
+      D.m = st[0];
+      D.n = st[1];
+      D.nd = st[2];
+      D.nelim = st[3];
+      D.nelim_app = st[4];
+      D.flag = st[5];
+      D.nzero = st[6];
+      D.nsteps = st[7];
+      D.ldl = st[8];
+      D.ldc = st[9];
+      if (D.m < D.n || D.n < 0 || D.ldl < D.m || D.nelim > D.n || D.n - D.nd != D.sn ||
+          D.m - D.nd != D.sm) {
+        r.ok = false;
+        break;
+      }
+      N->nd_seen[sl] = D.nd;
+      D.L = (double *)keep((size_t)D.ldl * D.n * sizeof(double));
+      D.D = (double *)keep(2 * (size_t)std::max(D.n, 1) * sizeof(double));
+      D.perm = (int *)keep(std::max(D.n, 1) * sizeof(int));
+      D.ipos = (int *)keep(std::max(D.n, 1) * sizeof(int));
+      if (D.n > 0) {
+        dbuf.resize((size_t)D.ldl * D.n + 2 * (size_t)D.n);
+        r.get(dbuf.data(), dbuf.size());
+        ibuf.resize(2 * (size_t)D.n);
+        r.get(ibuf.data(), ibuf.size());
+        if (!r.ok) break;
+        HIPCHK(hipMemcpy(D.L, dbuf.data(), (size_t)D.ldl * D.n * sizeof(double),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.D, dbuf.data() + (size_t)D.ldl * D.n,
+                         2 * (size_t)D.n * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.perm, ibuf.data(), D.n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.ipos, ibuf.data() + D.n, D.n * sizeof(int), hipMemcpyHostToDevice));
+      }
+      const size_t tn = ctab_ints(S, sl, D.m);
+      D.ctab = (int *)keep(tn * sizeof(int));
+      ibuf.resize(tn);
+      r.get(ibuf.data(), tn);
+      if (!r.ok) break;
+      HIPCHK(hipMemcpy(D.ctab, ibuf.data(), tn * sizeof(int), hipMemcpyHostToDevice));
+      const int64_t cb = r.get1<int64_t>();
+      const int cm = D.m - D.n;
+      const size_t cbmax =
+          (size_t)std::max(D.ldc, 0) * std::max(cm, 0) * (N->c32 ? sizeof(float) : sizeof(double));
+      if (cb < 0 || (size_t)cb > cbmax) {
+        r.ok = false;
+        break;
+      }
+      if (cb > 0) {
+        D.C = (double *)keep((size_t)cb);
+        cbuf.resize(cb);
+        r.get(cbuf.data(), cb);
+        if (!r.ok) break;
+        HIPCHK(hipMemcpy(D.C, cbuf.data(), cb, hipMemcpyHostToDevice));
+      }
+    }
+    std::fclose(f);
+    f = nullptr;
+    if (!r.ok) {
+      delete N;
+      *flag = -1;
+      return nullptr;
+    }
+    if (!N->d_nodes) HIPCHK(hipMalloc(&N->d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
+    HIPCHK(hipMemcpy(N->d_nodes, N->nodes.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice));
+    if (!N->d_counters) HIPCHK(hipMalloc(&N->d_counters, 16 * sizeof(int)));
+    if (!N->d_scratch_n) HIPCHK(hipMalloc(&N->d_scratch_n, std::max(S.n, 1) * sizeof(int)));
+    build_solve_tables(*N);
+    return N;
+  } catch (HipError &) {
+    if (f) std::fclose(f);
+    delete N;
+    *flag = -51;
+  } catch (std::bad_alloc &) {
+    if (f) std::fclose(f);
+    delete N;
+    *flag = -50;
+  }
+  return nullptr;
 }
 
 }  // namespace ssids_hip
@@ -1756,6 +2046,38 @@ int spral_ssids_hip_alter(void *numeric, const double *d) {
   } catch (HipError &) {
     return -51;
   }
+}
+
+int spral_ssids_hip_save_numeric(const void *numeric, const char *path) {
+  try {
+    return save_numeric(*static_cast<const Numeric *>(numeric), path);
+  } catch (HipError &) {
+    return -51;
+  } catch (std::bad_alloc &) {
+    return -50;
+  }
+}
+
+void *spral_ssids_hip_load_numeric(void *symbolic, const char *path,
+                                   struct spral_ssids_hip_stats *stats, int *flag) {
+  Numeric *N = load_numeric(static_cast<Symbolic *>(symbolic), path, flag);
+  if (stats) {
+    if (N) *stats = N->stats;
+    else std::memset(stats, 0, sizeof(*stats));
+  }
+  return N;
+}
+
+int spral_ssids_hip_numeric_scaling(const void *numeric, double *s) {
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  if (!N->d_scal) return 0;
+  try {
+    HIPCHK(hipSetDevice(N->symb->device));
+    HIPCHK(hipMemcpy(s, N->d_scal, N->symb->n * sizeof(double), hipMemcpyDeviceToHost));
+  } catch (HipError &) {
+    return -51;
+  }
+  return 1;
 }
 
 int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *ldval,

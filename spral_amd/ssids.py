@@ -12,8 +12,12 @@ CPU fallback.  Matrices are lower-triangle CSC with 1-based ptr/row as in the
 reference.  Orderings: options.ordering = 0 takes the user's order (as the
 reference); 1 (the reference's METIS) uses this package's exact minimum degree
 up to n = AMD_THRESHOLD and its approximate minimum degree above, since METIS is
-not available; 3 is geometric nested dissection for an nx*ny*nz grid (pass
-grid=(nx,ny,nz)); 4 is the approximate minimum degree at any size.
+not available; 2 is the reference's matching-based order (maximum-product
+matching, matched pairs ordered together; needs val) with approximate minimum
+degree on the compressed graph; 3 is geometric nested dissection for an
+nx*ny*nz grid (pass grid=(nx,ny,nz)); 4 is the approximate minimum degree at
+any size.  Scalings: options.scaling 1 is the matching (Hungarian) scaling, 3
+the scaling of ordering 2, >= 4 norm equilibration on the device.
 """
 import ctypes as C
 from dataclasses import dataclass, field
@@ -30,15 +34,18 @@ SSIDS_ERROR_A_PTR = -3
 SSIDS_ERROR_SINGULAR = -5
 SSIDS_ERROR_NOT_POS_DEF = -6
 SSIDS_ERROR_ORDER = -8
+SSIDS_ERROR_VAL = -9
 SSIDS_ERROR_X_SIZE = -10
 SSIDS_ERROR_JOB_OOR = -11
 SSIDS_ERROR_NOT_LLT = -13
 SSIDS_ERROR_NOT_LDLT = -14
+SSIDS_ERROR_NO_SAVED_SCALING = -15
 SSIDS_ERROR_ALLOCATION = -50
 SSIDS_ERROR_CUDA_UNKNOWN = -51
 SSIDS_ERROR_UNIMPLEMENTED = -98
 SSIDS_WARNING_ANAL_SINGULAR = 6
 SSIDS_WARNING_FACT_SINGULAR = 7
+SSIDS_WARNING_MATCH_ORD_NO_SCALE = 8
 
 SSIDS_SOLVE_JOB_ALL = 0
 SSIDS_SOLVE_JOB_FWD = 1
@@ -119,6 +126,7 @@ class AKeep:
     symbolic: int = None       # C handle of the device symbolic subtree
     inform: SSIDSInform = field(default_factory=SSIDSInform)
     nz: int = 0
+    scaling: np.ndarray = None  # matching scaling of ordering = 2 (user order)
 
 
 @dataclass
@@ -143,8 +151,10 @@ def _as_i32(a):
 AMD_THRESHOLD = 20000
 
 
-def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None):
-    """Analyse the sparsity pattern; returns (akeep, inform)."""
+def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None, val=None):
+    """Analyse the sparsity pattern; returns (akeep, inform).  val is needed
+    for the matching-based order (options.ordering = 2), as in the
+    reference."""
     options = options or SSIDSOptions()
     L = _lib.lib()
     inform = SSIDSInform()
@@ -156,12 +166,30 @@ def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None):
     ptr = _as_i64(ptr)
     row = _as_i32(row)
     ak.ptr, ak.row, ak.nz = ptr, row, int(ptr[-1] - 1)
+    if options.ordering == 2 and val is None:
+        inform.flag = SSIDS_ERROR_VAL
+        ak.inform = inform
+        return ak, inform
     if options.ordering == 0:
         if order is None:
             inform.flag = SSIDS_ERROR_ORDER
             ak.inform = inform
             return ak, inform
         ordv = _as_i32(np.abs(order)).copy()
+    elif options.ordering == 2:
+        # matching-based order (src/ssids/ssids.f90:318-340): the matching's
+        # pairs take consecutive positions; its scaling is kept for
+        # options.scaling = 3 in the factorization
+        ordv = np.zeros(n, np.int32)
+        ak.scaling = np.zeros(n)
+        mf = L.spral_match_order(n, ptr, row, np.ascontiguousarray(val, np.float64), ordv,
+                                 ak.scaling)
+        if mf < 0:
+            inform.flag = SSIDS_ERROR_ALLOCATION
+            ak.inform = inform
+            return ak, inform
+        if mf == 1:
+            inform.flag = SSIDS_WARNING_ANAL_SINGULAR
     elif options.ordering == 3:
         nx, ny, nz = grid
         ordv = np.zeros(n, np.int32)
@@ -280,10 +308,36 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
         if scale is not None:
             scale[:] = s
         scale = s
-    elif options.scaling in (1, 2, 3):
+    elif options.scaling == 1:
+        # maximum-product matching scaling (src/ssids/ssids.f90:922-950),
+        # scale_if_singular = options.action
+        s = np.zeros(akeep.n)
+        nm = C.c_int(0)
+        hp, hr = _as_i64(akeep.ptr), _as_i32(akeep.row)
+        hf = L.spral_hungarian_scale_sym(akeep.n, hp.ctypes.data, hr.ctypes.data,
+                                         val.ctypes.data, s.ctypes.data, None,
+                                         1 if options.action else 0, C.byref(nm))
+        if hf == -2:
+            inf = SSIDSInform(flag=SSIDS_ERROR_SINGULAR)
+            return FKeep(inform=inf), inf
+        if hf == -1:
+            inf = SSIDSInform(flag=SSIDS_ERROR_ALLOCATION)
+            return FKeep(inform=inf), inf
+        if scale is not None:
+            scale[:] = s
+        scale = s
+    elif options.scaling == 3:
+        # the scaling of the matching-based order (src/ssids/ssids.f90:979-987)
+        if getattr(akeep, "scaling", None) is None:
+            inf = SSIDSInform(flag=SSIDS_ERROR_NO_SAVED_SCALING)
+            return FKeep(inform=inf), inf
+        if scale is not None:
+            scale[:] = akeep.scaling
+        scale = akeep.scaling
+    elif options.scaling == 2:
         raise NotImplementedError(
-            f"options.scaling={options.scaling} (matching-based) is not built; use 0 with "
-            "a user scaling or >= 4 (equilibration)")
+            "options.scaling=2 (auction) is not built; use 1 (matching), 3, 0 with a user "
+            "scaling or >= 4 (equilibration)")
     if scale is not None:
         # scaling is applied in elimination order inside the subtree
         sc = np.ascontiguousarray(np.asarray(scale, np.float64)[akeep.invp])
@@ -313,6 +367,8 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
     num = L.spral_ssids_hip_factor(bool(posdef), akeep.symbolic, val, val.size, scp,
                                    C.byref(opts), C.byref(st))
     inf = _inform_from_stats(st, akeep)
+    if num and inf.flag == SSIDS_SUCCESS and akeep.scaling is not None and options.scaling != 3:
+        inf.flag = SSIDS_WARNING_MATCH_ORD_NO_SCALE  # src/ssids/ssids.f90:915-918
     fk = FKeep(numeric=num, pos_def=bool(posdef), scaling=sc, inform=inf,
                symbolic=akeep.symbolic)
     if num and inf.flag == SSIDS_WARNING_FACT_SINGULAR and not options.action:
@@ -499,11 +555,12 @@ _AKEEP_ARRAYS = ("ptr", "row", "order", "invp", "sptr", "sparent", "rptr", "rlis
 
 def ssids_save_akeep(akeep, path):
     """Writes the analyse result (the arrays of akeep and its inform sizes)."""
+    extra = {} if akeep.scaling is None else {"scaling": akeep.scaling}
     np.savez(path, n=akeep.n, nnodes=akeep.nnodes, num_factor=akeep.num_factor,
              num_flops=akeep.num_flops, nz=akeep.nz, flag=akeep.inform.flag,
              maxfront=akeep.inform.maxfront, maxsupernode=akeep.inform.maxsupernode,
              matrix_rank=akeep.inform.matrix_rank,
-             **{k: getattr(akeep, k) for k in _AKEEP_ARRAYS})
+             **{k: getattr(akeep, k) for k in _AKEEP_ARRAYS}, **extra)
 
 
 def ssids_load_akeep(path, options=None):
@@ -516,6 +573,8 @@ def ssids_load_akeep(path, options=None):
         setattr(ak, k, z[k])
     ak.nnodes, ak.nz = int(z["nnodes"]), int(z["nz"])
     ak.num_factor, ak.num_flops = int(z["num_factor"]), int(z["num_flops"])
+    if "scaling" in z.files:
+        ak.scaling = z["scaling"]
     inf = SSIDSInform(flag=int(z["flag"]), maxfront=int(z["maxfront"]),
                       maxsupernode=int(z["maxsupernode"]), matrix_rank=int(z["matrix_rank"]),
                       num_factor=ak.num_factor, num_flops=ak.num_flops, num_sup=ak.nnodes)
@@ -531,6 +590,40 @@ def ssids_load_akeep(path, options=None):
             inf.flag = f.value
     ak.inform = inf
     return ak, inf
+
+
+def ssids_save_fkeep(fkeep, path):
+    """Writes the resident factors of fkeep (no reference counterpart: the
+    reference keeps factors in memory only).  Returns 0, or -1 on an I/O
+    error."""
+    if fkeep is None or not fkeep.numeric:
+        return SSIDS_ERROR_CALL_SEQUENCE
+    return _lib.lib().spral_ssids_hip_save_numeric(fkeep.numeric, str(path).encode())
+
+
+def ssids_load_fkeep(akeep, path):
+    """Rebuilds resident factors saved by ssids_save_fkeep on akeep (an
+    analyse of the same pattern and ordering: its device symbolic handle must
+    exist).  Returns (fkeep, inform); inform.flag < 0 when the file is
+    unreadable (-1), belongs to another analyse (-2) or the device failed."""
+    L = _lib.lib()
+    if akeep is None or akeep.symbolic is None:
+        inf = SSIDSInform(flag=SSIDS_ERROR_CALL_SEQUENCE)
+        return FKeep(inform=inf), inf
+    st = _lib.Stats()
+    f = C.c_int(0)
+    num = L.spral_ssids_hip_load_numeric(akeep.symbolic, str(path).encode(), C.byref(st),
+                                         C.byref(f))
+    if not num:
+        inf = SSIDSInform(flag=f.value)
+        return FKeep(inform=inf), inf
+    inf = _inform_from_stats(st, akeep)
+    sc = np.zeros(akeep.n)
+    has = L.spral_ssids_hip_numeric_scaling(num, sc)
+    en = _lib.lib().spral_ssids_hip_enquire_posdef(num, np.zeros(max(akeep.n, 1)))
+    fk = FKeep(numeric=num, pos_def=(en == 0), scaling=(sc if has == 1 else None), inform=inf,
+               symbolic=akeep.symbolic)
+    return fk, inf
 
 
 class AnalyseCache:
