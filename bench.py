@@ -52,8 +52,8 @@ def parse():
                          "(27-point H on this grid, grid^3 * 88000/512000 constraint rows with "
                          "4 entries) factored and solved in FP64 and mixed precision")
     ap.add_argument("--kkt-ordering", type=int, default=2,
-                    help="--kkt27 ordering: 2 matching-based (with its scaling, the "
-                         "reference's choice for saddle-point systems), 4 AMD")
+                    help="--kkt27 / --batch ordering: 2 matching-based (with its scaling, "
+                         "the reference's choice for saddle-point systems), 4 AMD")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision (FP32 Schur update) measurement")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -125,7 +125,9 @@ def batch_bench(args):
     rng = np.random.Generator(np.random.PCG64(20261202 + 4))
     sizes = rng.integers(200, 5001, count)
     mats = [matrices.kkt_random(int(n), seed=20261202 + 4 + i) for i, n in enumerate(sizes)]
-    b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=4))
+    kord = args.kkt_ordering
+    b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=kord,
+                                                          scaling=3 if kord == 2 else 0))
     assert inf.flag >= 0, inf
     rhs = [m.full @ rng.uniform(-1, 1, m.n) for m in mats]
     val = np.concatenate([m.val for m in mats])
@@ -151,8 +153,9 @@ def batch_bench(args):
         singles.append((ak, o))
     devmem.synchronize()
     t0 = time.perf_counter()
-    for (ak, o), m, r in zip(singles, mats, rhs):
-        fk, fi = ssids.ssids_factor(False, m.val, ak, o)
+    for i, ((ak, o), m, r) in enumerate(zip(singles, mats, rhs)):
+        sc = None if b.scale is None else b.scale[b.offsets[i]:b.offsets[i + 1]].copy()
+        fk, fi = ssids.ssids_factor(False, m.val, ak, o, scale=sc)
         x = r.copy()
         ssids.ssids_solve(x, ak, fk, o)
         ssids.ssids_free(None, fk)
@@ -168,7 +171,9 @@ def batch_bench(args):
         "ms_per_step": round(tb * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{count} KKT systems, n uniform in [200, 5000], 20% "
-                               "constraint rows, AMD order each, 1 RHS each",
+                               "constraint rows, " + ("matching-based order + its scaling"
+                                                      if kord == 2 else "AMD order") +
+                               " each, 1 RHS each",
                    "n_total": n_total, "num_flops": flops, "fronts": fronts,
                    "parallelism": "single-gpu"},
         "factor_ms_device": round(finf.factor_ms, 3),
@@ -252,10 +257,12 @@ def serve_bench(args):
     pats = {f"kkt{g}": matrices.kkt_laplace(grid=g, m_con=g ** 3 // 8, seed=20261202 + 5 + g)
             for g in grids}
     nh = {f"kkt{g}": g ** 3 for g in grids}
+    # matching-based order from the unperturbed values, and the matching
+    # scaling recomputed for every refactor's barrier-perturbed values
     svc = serve.SolverService(hbm_budget_bytes=200 << 30,
-                              options=ssids.SSIDSOptions(ordering=4))
+                              options=ssids.SSIDSOptions(ordering=2, scaling=1))
     for name, m in pats.items():
-        svc.register(name, m.n, m.ptr, m.row)
+        svc.register(name, m.n, m.ptr, m.row, val=m.val)
     svc.start()
     # H's diagonal entries (rows < nh on the diagonal) get the barrier terms
     diag_pos = {}
@@ -295,9 +302,10 @@ def serve_bench(args):
         "config": {"workload": f"Poisson stream at {args.rate}/s for {args.serve} s on "
                                f"{len(pats)} resident KKT patterns (7-pt H of grid^3 + "
                                "grid^3/8 constraint rows, grids " + str(list(grids)) +
-                               "), AMD orders, 30% refactors with log-uniform [1e-8, 1e8] barrier "
-                               "terms on H's diagonal, 70% solves with k in {1,4,16,64}, "
-                               "FP64 refinement to 1e-14",
+                               "), matching-based orders, matching scaling per refactor, 30% "
+                               "refactors with log-uniform [1e-8, 1e8] barrier terms on H's "
+                               "diagonal, 70% solves with k in {1,4,16,64}, FP64 refinement to "
+                               "1e-14, refactors and solves on two lanes",
                    "n": [int(m.n) for m in pats.values()], "parallelism": "single-gpu"},
         "refactor_ms": lr, "solve_ms": ls, "rounds": rounds,
         "requests": {"refactor": kinds.count("refactor"), "solve": kinds.count("solve")}}),

@@ -26,6 +26,21 @@ int spral_hungarian_scale_sym(int n, const int64_t *ptr, const int *row, const d
                               double *scaling, int *match, int scale_if_singular,
                               int *matched);
 
+/* Replaces auction_scale_sym (src/scaling.f90:269-309; auction_match
+ * :1481-1586, auction_match_core :1328-1466): the approximate matching of
+ * the auction algorithm, one column bidding at a time, and the symmetric
+ * scaling from its prices.  max_iterations, max_unchanged (3),
+ * min_proportion (3, single precision) and eps_initial (single precision)
+ * are auction_options (defaults 30000, {10, 100, 100}, {0.90, 0.0, 0.0},
+ * 0.01).  match (n, may be NULL): matched column of each row, 1-based, 0
+ * unmatched; *matched, *iterations and *unmatchable as auction_inform.
+ * Returns 0, -1 (allocation) or -3 (n < 0). */
+int spral_auction_scale_sym(int n, const int64_t *ptr, const int *row, const double *val,
+                            double *scaling, int *match, int max_iterations,
+                            const int *max_unchanged, const float *min_proportion,
+                            float eps_initial, int *matched, int *iterations,
+                            int *unmatchable);
+
 /* Matching-based elimination order, replacing match_order_metis
  * (src/match_order.f90:135-208; mo_match :495-629, mo_split :220-396) as
  * ssids_analyse calls it for options%ordering = 2

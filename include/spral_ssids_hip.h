@@ -265,6 +265,13 @@ void spral_ssids_hip_free_contrib(void *numeric);
 int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, double *L,
                                  double *D, int *perm);
 
+/* Diagnostic accessor (no reference counterpart; used by parity tests): per
+ * node (0-based postorder, arrays of nnodes), the pivots the a posteriori
+ * pass accepted (nelim_app; the rest of nelim came from threshold partial
+ * pivoting on the failed columns) and the delayed columns that arrived from
+ * the children.  Returns 0. */
+int spral_ssids_hip_node_pivots(const void *numeric, int *nelim_app, int *ndelay_in);
+
 /* Number of visible HIP devices (replaces the cudaGetDeviceCount probe of
  * src/hw_topology/guess_topology.cxx:65). */
 int spral_ssids_hip_device_count(void);

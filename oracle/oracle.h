@@ -12,6 +12,13 @@
  * (src/ssids/cpu/NumericSubtree.hxx:246-280), solves
  * (src/ssids/cpu/NumericSubtree.hxx:286-406, ldlt_tpp.cxx:272-317) and
  * enquire (src/ssids/cpu/NumericSubtree.hxx:424-468).
+ *
+ * oracle_factor_app restates the device factorization's pivoting instead:
+ * the front layout the device assembles (delayed columns first) and a
+ * posteriori pivoting in 64-column blocks (complete pivoting inside a block,
+ * src/ssids/cpu/kernels/block_ldlt.hxx:289-414; the |l| <= 1/u test of
+ * src/ssids/cpu/kernels/ldlt_app.cxx:304-322), then threshold partial
+ * pivoting on the failed columns.
  */
 #ifndef SSIDS_ORACLE_H
 #define SSIDS_ORACLE_H
@@ -38,6 +45,15 @@ void *oracle_factor_prefix(int n, int nnodes, const int *sptr, const int *sparen
                            const int64_t *nlist, const double *aval,
                            const double *scaling, int action, double u, double small,
                            int max_nodes, int64_t *stats);
+/* The same factorization with the device's pivoting (APP, then TPP on the
+ * failed columns) and front layout; stats as oracle_factor, not_first_pass
+ * counting the columns APP failed. */
+void *oracle_factor_app(int n, int nnodes, const int *sptr, const int *sparent,
+                        const int64_t *rptr, const int *rlist, const int64_t *nptr,
+                        const int64_t *nlist, const double *aval, const double *scaling,
+                        int action, double u, double small, int64_t *stats);
+/* per-node pivots accepted by APP (nnodes; equal to nelim without APP) */
+void oracle_node_app(void *h, int *nelim_app);
 /* job: 0 all, 1 fwd, 2 diag, 3 bwd, 4 diag_bwd; x (ldx,nrhs) in elimination
  * order (as the subtree sees it). */
 void oracle_solve(void *h, int job, int nrhs, double *x, int ldx);

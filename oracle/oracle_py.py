@@ -24,6 +24,9 @@ def lib():
         L.oracle_factor_prefix.restype = vp
         L.oracle_factor_prefix.argtypes = ([C.c_int, C.c_int] + [vp] * 8 +
                                            [C.c_int, C.c_double, C.c_double, C.c_int, vp])
+        L.oracle_factor_app.restype = vp
+        L.oracle_factor_app.argtypes = L.oracle_factor.argtypes
+        L.oracle_node_app.argtypes = [vp, vp]
         L.oracle_solve.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int]
         L.oracle_enquire.argtypes = [vp, vp, vp]
         L.oracle_node_info.argtypes = [vp, vp, vp]
@@ -43,10 +46,12 @@ class OracleFactor:
     # u defaults to the reference's value: the single-precision literal 0.01
     # widened to double (src/ssids/datatypes.f90:262)
     def __init__(self, akeep, val, scaling=None, action=True, u=float(np.float32(0.01)),
-                 small=1e-20, max_nodes=None):
+                 small=1e-20, max_nodes=None, app=False):
         """max_nodes: factor only the first max_nodes fronts in postorder (a
         prefix of the same factorization, for a bounded timing sample; such a
-        handle takes no solves)."""
+        handle takes no solves).  app: the device's pivoting (a posteriori
+        pivoting in 64-column blocks, then TPP on the failed columns) and
+        front layout (delayed columns first) instead of TPP throughout."""
         L = lib()
         self.akeep = akeep
         self.val = np.ascontiguousarray(val, np.float64)
@@ -56,7 +61,10 @@ class OracleFactor:
         args = (akeep.n, akeep.nnodes, _p(akeep.sptr), _p(akeep.sparent), _p(akeep.rptr),
                 _p(akeep.rlist), _p(akeep.nptr), _p(akeep.nlist), _p(self.val),
                 _p(self.scaling), 1 if action else 0, u, small)
-        if max_nodes is None:
+        if app:
+            assert max_nodes is None
+            self.h = L.oracle_factor_app(*args, _p(st))
+        elif max_nodes is None:
             self.h = L.oracle_factor(*args, _p(st))
         else:
             self.h = L.oracle_factor_prefix(*args, int(max_nodes), _p(st))
@@ -95,6 +103,11 @@ class OracleFactor:
         nd = np.zeros(nn, np.int32)
         lib().oracle_node_info(self.h, _p(ne), _p(nd))
         return ne, nd
+
+    def node_app(self):
+        ne = np.zeros(self.akeep.nnodes, np.int32)
+        lib().oracle_node_app(self.h, _p(ne))
+        return ne
 
     def node_factors(self, node):
         info = np.zeros(6, np.int32)

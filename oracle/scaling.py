@@ -101,3 +101,19 @@ def ref_hungarian_scale_sym(n, ptr, row, val, scale_if_singular=False):
                               s.ctypes.data, mt.ctypes.data, 1 if scale_if_singular else 0,
                               C.byref(nm), C.byref(fl))
     return s[:n], mt[:n], nm.value, fl.value
+
+
+def ref_auction_scale_sym(n, ptr, row, val):
+    """The reference's auction_scale_sym itself with default auction_options:
+    (scaling, match, matched, iterations, flag); match 1-based, 0 unmatched."""
+    L = _reflib()
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    row = np.ascontiguousarray(row, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    s = np.zeros(max(n, 1))
+    mt = np.zeros(max(n, 1), np.int32)
+    nm, it, fl = C.c_int(0), C.c_int(0), C.c_int(0)
+    L.ref_auction_scale_sym(n, ptr.ctypes.data, row.ctypes.data, val.ctypes.data,
+                            s.ctypes.data, mt.ctypes.data, C.byref(nm), C.byref(it),
+                            C.byref(fl))
+    return s[:n], mt[:n], nm.value, it.value, fl.value

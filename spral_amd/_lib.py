@@ -97,6 +97,10 @@ SIGNATURES = {
     "spral_hungarian_scale_sym": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.c_int,
                                             C.POINTER(C.c_int)]),  # raw pointers
+    "spral_auction_scale_sym": (C.c_int, [C.c_int] + [C.c_void_p] * 5 +
+                                [C.c_int, C.c_void_p, C.c_void_p, C.c_float,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
     "spral_match_order": (C.c_int, [C.c_int, i64p, i32p, f64p, i32p, f64p]),
     "spral_ssids_hip_save_numeric": (C.c_int, [C.c_void_p, C.c_char_p]),
     "spral_ssids_hip_load_numeric": (C.c_void_p, [C.c_void_p, C.c_char_p, C.c_void_p,
@@ -118,6 +122,7 @@ SIGNATURES = {
     "spral_ssids_hip_get_contrib": (C.c_int, [C.c_void_p] + [C.c_void_p] * 8),
     "spral_ssids_hip_free_contrib": (None, [C.c_void_p]),
     "spral_ssids_hip_device_count": (C.c_int, []),
+    "spral_ssids_hip_node_pivots": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
     "spral_rb_peek": (C.c_int, [C.c_char_p] + [C.c_void_p] * 9),

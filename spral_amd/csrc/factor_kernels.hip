@@ -64,10 +64,14 @@ __device__ inline const double *child_src(const NodeDev &c, int kr, int k) {
 // rows [vR, vR+R) (R = 64/NW) of column j of the symmetric w x w block in
 // registers; pivots are tracked in a uniform bit mask (no physical swaps).
 // Per pivot step:
-//  * search: every lane scans its live rows for max |a(i,j)| (rows in
-//    ascending order, first maximum kept); each wave reduces to its
-//    (value, smallest column, that column's row) with DPP and a ballot, and
-//    the waves' candidates are merged through LDS in the same order;
+//  * search: every lane scans its live rows i >= j for max |a(i,j)| (the
+//    lower triangle, as block_ldlt.hxx:77-101 searches it: each symmetric
+//    pair once, in one orientation, so a maximum off the diagonal gives the
+//    pair (column, row) with column < row whatever the rounding of the two
+//    mirrored entries; rows ascending, first maximum kept); each wave
+//    reduces to its (value, smallest column, that column's row) with DPP
+//    and a ballot, and the waves' candidates are merged through LDS in the
+//    same order;
 //  * the owners of the pivot column(s) broadcast them through LDS;
 //  * rank-1 / rank-2 update of every column's rows in registers;
 //  * the L column(s) of the step go to LDS, indexed by elimination step.
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
 #pragma unroll
       for (int r = 0; r < R; ++r) {  // NaN never wins; all-zero keeps br = 64
         const double av = fabs(a[r]);
-        const bool take = !((elim >> (r0 + r)) & 1ull) && av > bv;
+        const bool take = r0 + r >= lane && !((elim >> (r0 + r)) & 1ull) && av > bv;
         bv = take ? av : bv;
         br = take ? r0 + r : br;
       }
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
       brow = best & 127;
       par ^= 1;
       const double vmax = __longlong_as_double((long long)kmax);
-      if (!(vmax >= fp.small) || bc >= 64) {
+      if (!(vmax >= fp.small) || bc >= 64 || brow >= 64) {
         zero_rest(-5);
         break;
       }
@@ -290,8 +294,10 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
       c1p = cs[0];
     } else {
       const double a11 = uni(cs[0][bc]), a22 = uni(cs[1][brow]), a21 = uni(cs[0][brow]);
+      // explicit fma throughout the block (no contraction left to the
+      // compiler), so the oracle's restatement can round the same way
       const double detscale = 1.0 / fabs(a21);
-      const double detpiv = (a11 * detscale) * a22 - fabs(a21);
+      const double detpiv = fma(a11 * detscale, a22, -fabs(a21));
       if (fabs(detpiv) >= fabs(a21) / 2) {
         size = 2;
         p1 = bc;
@@ -312,7 +318,7 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
       elim |= 1ull << p1;
       const double lj = c1p[lane] * di;  // L(lane, k)
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[r] -= c1p[r0 + r] * lj;
+      for (int r = 0; r < R; ++r) a[r] = fma(-c1p[r0 + r], lj, a[r]);
       if (wv == 0) t[k][lane] = lj;
       if (tid == 0) {
         dinv[2 * k] = di;
@@ -323,9 +329,9 @@ __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl
     } else {
       elim |= (1ull << p1) | (1ull << p2);
       const double x = c1p[lane], y = c2p[lane];
-      const double l1 = d11 * x + d21 * y, l2 = d21 * x + d22 * y;
+      const double l1 = fma(d11, x, d21 * y), l2 = fma(d21, x, d22 * y);
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[r] -= c1p[r0 + r] * l1 + c2p[r0 + r] * l2;
+      for (int r = 0; r < R; ++r) a[r] = fma(-c1p[r0 + r], l1, fma(-c2p[r0 + r], l2, a[r]));
       if (wv == 0) {
         t[k][lane] = l1;
         t[k + 1][lane] = l2;

@@ -9,6 +9,10 @@
 // tie, the order in which equal-distance rows leave the queue) are the
 // reference's, so the matching and the scaling are the same bit for bit.
 //
+// The auction scaling (auction_scale_sym, :269-309; auction_match
+// :1481-1586, auction_match_core :1328-1466) is the same restated for the
+// approximate matching of Bertsekas's auction, as the reference runs it.
+//
 // The same matching drives the matching-based elimination order of
 // src/match_order.f90 (match_order_metis :135-208, mo_match :495-629,
 // mo_split :220-396): matched pairs of the 1- and 2-cycle split are merged,
@@ -508,6 +512,123 @@ int spral_hungarian_scale_sym(int n, const int64_t *ptr, const int *row, const d
     if (match)
       for (int i = 0; i < n; ++i) match[i] = rm[i] >= 0 ? rm[i] + 1 : 0;
     return flag;
+  } catch (std::bad_alloc &) {
+    return -1;
+  }
+}
+
+int spral_auction_scale_sym(int n, const int64_t *ptr, const int *row, const double *val,
+                            double *scaling, int *match, int max_iterations,
+                            const int *max_unchanged, const float *min_proportion,
+                            float eps_initial, int *matched, int *iterations,
+                            int *unmatchable) {
+  if (n < 0) return -3;
+  *matched = *iterations = *unmatchable = 0;
+  if (n == 0) return 0;
+  try {
+    // weights: 2 max + 1 - (column max of log|a| - log|a|), so that a
+    // larger matching always pays (auction_match, src/scaling.f90:1481-1586)
+    Graph g = expand_lower(n, ptr, row, val, true);
+    std::vector<double> cmax(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+      if (g.ptr[j + 1] <= g.ptr[j]) continue;
+      double mx = -kInf;
+      for (int64_t k = g.ptr[j]; k < g.ptr[j + 1]; ++k) mx = std::max(mx, g.val[k]);
+      cmax[j] = mx;
+      for (int64_t k = g.ptr[j]; k < g.ptr[j + 1]; ++k) g.val[k] = mx - g.val[k];
+    }
+    double maxentry = -kInf;
+    for (double x : g.val) maxentry = std::max(maxentry, x);
+    maxentry = 2 * maxentry + 1;
+    for (double &x : g.val) x = maxentry - x;
+    // auction (src/scaling.f90:1328-1466): one column bids at a time for the
+    // row of its best net value, paying the margin over its second best
+    // plus eps; outbid columns rejoin the next pass
+    std::vector<double> u(n, 0.0), v(n);
+    for (int j = 0; j < n; ++j) v[j] = -cmax[j];
+    std::vector<int> colmatch(n, 0), owner(n, -1), next(n);
+    for (int j = 0; j < n; ++j) next[j] = j;
+    int tail = n, unmatched = n, prev = -1, nunchanged = 0, itr;
+    double eps = (double)eps_initial;
+    int unmatchable_count = 0;
+    for (itr = 1; itr <= max_iterations; ++itr) {
+      if (unmatched == 0) break;
+      if (unmatched != prev) nunchanged = 0;
+      prev = unmatched;
+      ++nunchanged;
+      const float prop = (float)(n - unmatched) / (float)n;  // default-real arithmetic
+      bool stop = false;
+      for (int t = 0; t < 3; ++t)
+        if (nunchanged >= max_unchanged[t] && prop >= min_proportion[t]) stop = true;
+      if (stop) break;
+      eps = std::min(1.0, eps + 1.0 / (n + 1));
+      int insert = 0;
+      for (int cp = 0; cp < tail; ++cp) {
+        const int col = next[cp];
+        if (colmatch[col] != 0) continue;  // matched (row + 1) or ineligible (-1)
+        if (g.ptr[col] == g.ptr[col + 1]) continue;
+        int64_t k = g.ptr[col];
+        int bestr = g.row[k];
+        double bestu = g.val[k] - u[bestr], bestv = -kInf;
+        for (++k; k < g.ptr[col + 1]; ++k) {
+          const double x = g.val[k] - u[g.row[k]];
+          if (x > bestu) {
+            bestv = bestu;
+            bestr = g.row[k];
+            bestu = x;
+          } else if (x > bestv) {
+            bestv = x;
+          }
+        }
+        if (bestv == -kInf) bestv = 0.0;
+        if (bestu > 0) {
+          u[bestr] = u[bestr] + bestu - bestv + eps;
+          v[col] = bestv - eps;
+          colmatch[col] = bestr + 1;
+          --unmatched;
+          const int kk = owner[bestr];
+          owner[bestr] = col;
+          if (kk >= 0) {
+            colmatch[kk] = 0;
+            ++unmatched;
+            next[insert++] = kk;
+          }
+        } else {
+          colmatch[col] = -1;
+          --unmatched;
+          ++unmatchable_count;
+        }
+      }
+      tail = insert;
+    }
+    *iterations = itr - 1;
+    *unmatchable = unmatchable_count;
+    int nm = 0;
+    std::vector<int> rowmatch(n, 0);
+    for (int j = 0; j < n; ++j)
+      if (colmatch[j] > 0) {
+        ++nm;
+        rowmatch[colmatch[j] - 1] = j + 1;
+      }
+    *matched = nm;
+    std::vector<double> rs(n), cs(n);
+    for (int i = 0; i < n; ++i) rs[i] = -u[i] + maxentry;
+    for (int j = 0; j < n; ++j) cs[j] = -v[j] - cmax[j];
+    // square post-processing (match_postproc): equal mean row and column logs
+    double ravg = 0.0, cavg = 0.0;
+    for (int i = 0; i < n; ++i) ravg += rs[i];
+    for (int i = 0; i < n; ++i) cavg += cs[i];
+    ravg /= n;
+    cavg /= n;
+    const double adjust = (ravg - cavg) / 2;
+    for (int i = 0; i < n; ++i) {
+      rs[i] -= adjust;
+      cs[i] += adjust;
+    }
+    for (int i = 0; i < n; ++i) scaling[i] = std::exp((rs[i] + cs[i]) / 2);
+    if (match)
+      for (int i = 0; i < n; ++i) match[i] = rowmatch[i];
+    return 0;
   } catch (std::bad_alloc &) {
     return -1;
   }

@@ -16,11 +16,13 @@
 #include <set>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <new>
 #include <stdexcept>
 #include <vector>
 
 #include "../../include/spral_ssids_hip.h"
+#include "api_lock.hpp"
 #include "dev.hpp"
 
 namespace ssids_hip {
@@ -236,6 +238,24 @@ struct Plan {
   }
 };
 
+static std::shared_mutex api_mu;
+static thread_local int api_depth = 0;
+
+ApiShared::ApiShared() {
+  if (api_depth++ == 0) api_mu.lock_shared();
+}
+ApiShared::~ApiShared() {
+  if (--api_depth == 0) api_mu.unlock_shared();
+}
+CaptureExclusive::CaptureExclusive() : nested_(api_depth > 0) {
+  if (nested_) api_mu.unlock_shared();
+  api_mu.lock();
+}
+CaptureExclusive::~CaptureExclusive() {
+  api_mu.unlock();
+  if (nested_) api_mu.lock_shared();
+}
+
 struct Numeric {
   Symbolic *symb = nullptr;
   int c32 = 0;  // contribution blocks in FP32 (schur_precision 1, mixed precision)
@@ -243,6 +263,11 @@ struct Numeric {
   // second stream of the static plan: a level's gather beyond its first tile
   // column runs there while the first pivot step runs on `stream`
   hipStream_t stream2 = nullptr;
+  // graph capture records on this stream only.  It and `stream2` are
+  // non-blocking: an operation on the legacy null stream from another host
+  // thread (a synchronous copy) would otherwise depend on the capturing
+  // stream, which invalidates the capture
+  hipStream_t stream_cap = nullptr;
   hipEvent_t ev_level = nullptr, ev_rest = nullptr;
   // the static-plan factorization captured as a graph, replayed by later
   // refactors with the same plan, parameters and buffers
@@ -318,6 +343,7 @@ struct Numeric {
     if (ev_level) (void)hipEventDestroy(ev_level);
     if (ev_rest) (void)hipEventDestroy(ev_rest);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream_cap) (void)hipStreamDestroy(stream_cap);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -678,13 +704,13 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   // they overlap the rest of the gather; `st` joins `st2` before the first
   // trailing update.  A timing run (opt.timing) launches the same sequence
   // eagerly, with events on `st2` around each level's rest-of-level gather.
-  if (!N.stream2) HIPCHK(hipStreamCreate(&N.stream2));
+  if (!N.stream2) HIPCHK(hipStreamCreateWithFlags(&N.stream2, hipStreamNonBlocking));
   if (!N.ev_level) HIPCHK(hipEventCreateWithFlags(&N.ev_level, hipEventDisableTiming));
   if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
   hipStream_t st2 = N.stream2;
   // the whole launch sequence (no host round trip, no memset: the counters
   // are cleared before it), captured once as a graph when not timing
-  auto enqueue = [&]() {
+  auto enqueue = [&](hipStream_t st) {
   HIPCHK(hipMemcpyAsync(N.d_nodes, P.d_tmpl, nn * sizeof(NodeDev), hipMemcpyDeviceToDevice,
                         st));
   for (int L = 0; L < nlev; ++L) {
@@ -738,20 +764,52 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
                                      (double)fp.tpp_failed, (double)fp.pivot_tpp,
                                      (double)fp.posdef, (double)fp.c32,
                                      (double)(uintptr_t)N.d_aval, (double)(uintptr_t)N.d_scal};
+    bool captured = true;
     if (!N.gexec || N.gkey != key) {
       if (N.gexec) HIPCHK(hipGraphExecDestroy(N.gexec));
       if (N.graph) HIPCHK(hipGraphDestroy(N.graph));
       N.gexec = nullptr;
       N.graph = nullptr;
-      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
-      enqueue();
-      HIPCHK(hipStreamEndCapture(st, &N.graph));
-      HIPCHK(hipGraphInstantiate(&N.gexec, N.graph, nullptr, nullptr, 0));
-      N.gkey = key;
+      // thread-local, on the private non-blocking stream, while no other
+      // host thread is inside the library; the graph is launched on `st`
+      if (!N.stream_cap) HIPCHK(hipStreamCreateWithFlags(&N.stream_cap, hipStreamNonBlocking));
+      hipStream_t cs = N.stream_cap;
+      CaptureExclusive only_caller;
+      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      hipError_t ce = hipSuccess;
+      try {
+        enqueue(cs);
+      } catch (HipError &e) {
+        ce = e.code;
+      } catch (...) {
+        hipGraph_t g = nullptr;  // leave capture mode before reporting
+        (void)hipStreamEndCapture(cs, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t ee = hipStreamEndCapture(cs, &g);
+      if (ce == hipSuccess) ce = ee;
+      if (ce == hipSuccess) {
+        N.graph = g;
+        HIPCHK(hipGraphInstantiate(&N.gexec, N.graph, nullptr, nullptr, 0));
+        N.gkey = key;
+      } else {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        // a HIP call made outside the library by another host thread can
+        // invalidate a capture: nothing captured has run, so this call
+        // launches the sequence eagerly and the next one captures again
+        if (ce != hipErrorStreamCaptureInvalidated && ce != hipErrorStreamCaptureImplicit &&
+            ce != hipErrorStreamCaptureUnsupported)
+          throw HipError(ce);
+        captured = false;
+      }
     }
-    HIPCHK(hipGraphLaunch(N.gexec, st));
+    if (captured) HIPCHK(hipGraphLaunch(N.gexec, st));
+    else enqueue(st);
   } else {
-    enqueue();
+    enqueue(st);
   }
   HIPCHK(hipGetLastError());
   std::vector<int> pack(8 * (size_t)nn);
@@ -1691,6 +1749,7 @@ extern "C" {
 void spral_ssids_hip_free_contrib(void *numeric);
 
 int spral_ssids_hip_device_count(void) {
+  ApiShared api_guard;
   int c = 0;
   if (hipGetDeviceCount(&c) != hipSuccess) return 0;
   return c;
@@ -1702,6 +1761,7 @@ void *spral_ssids_hip_create_symbolic(int device, int n, int sa, int en, const i
                                       const int64_t *nlist,
                                       const struct spral_ssids_hip_options *options,
                                       int *flag) {
+  ApiShared api_guard;
   (void)options;
   *flag = 0;
   try {
@@ -1715,6 +1775,7 @@ void *spral_ssids_hip_create_symbolic(int device, int n, int sa, int en, const i
 }
 
 void spral_ssids_hip_destroy_symbolic(void *symbolic) {
+  ApiShared api_guard;
   delete static_cast<Symbolic *>(symbolic);
 }
 
@@ -1722,6 +1783,7 @@ void *spral_ssids_hip_factor_device(bool posdef, void *symbolic, const double *d
                                     int64_t naval, const double *d_scaling,
                                     const struct spral_ssids_hip_options *options,
                                     struct spral_ssids_hip_stats *stats) {
+  ApiShared api_guard;
   Symbolic *S = static_cast<Symbolic *>(symbolic);
   spral_ssids_hip_options opt;
   if (options) opt = *options;
@@ -1757,6 +1819,7 @@ void *spral_ssids_hip_factor(bool posdef, void *symbolic, const double *aval, in
                              const double *scaling,
                              const struct spral_ssids_hip_options *options,
                              struct spral_ssids_hip_stats *stats) {
+  ApiShared api_guard;
   Symbolic *S = static_cast<Symbolic *>(symbolic);
   double *d_a = nullptr, *d_s = nullptr;
   void *res = nullptr;
@@ -1782,6 +1845,7 @@ int spral_ssids_hip_refactor_device(void *numeric, const double *d_aval,
                                     const double *d_scaling,
                                     const struct spral_ssids_hip_options *options,
                                     struct spral_ssids_hip_stats *stats) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   spral_ssids_hip_options opt;
   if (options) opt = *options;
@@ -1802,7 +1866,10 @@ int spral_ssids_hip_refactor_device(void *numeric, const double *d_aval,
   }
 }
 
-void spral_ssids_hip_destroy_numeric(void *numeric) { delete static_cast<Numeric *>(numeric); }
+void spral_ssids_hip_destroy_numeric(void *numeric) {
+  ApiShared api_guard;
+  delete static_cast<Numeric *>(numeric);
+}
 
 static int solve_entry(void *numeric, int job, int nrhs, double *x, int ldx) {
   try {
@@ -1815,20 +1882,25 @@ static int solve_entry(void *numeric, int job, int nrhs, double *x, int ldx) {
 }
 
 int spral_ssids_hip_solve_fwd(void *numeric, int nrhs, double *x, int ldx) {
+  ApiShared api_guard;
   return solve_entry(numeric, 1, nrhs, x, ldx);
 }
 int spral_ssids_hip_solve_diag(void *numeric, int nrhs, double *x, int ldx) {
+  ApiShared api_guard;
   return solve_entry(numeric, 2, nrhs, x, ldx);
 }
 int spral_ssids_hip_solve_diag_bwd(void *numeric, int nrhs, double *x, int ldx) {
+  ApiShared api_guard;
   return solve_entry(numeric, 4, nrhs, x, ldx);
 }
 int spral_ssids_hip_solve_bwd(void *numeric, int nrhs, double *x, int ldx) {
+  ApiShared api_guard;
   return solve_entry(numeric, 3, nrhs, x, ldx);
 }
 
 int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x, int ldx,
                                  void *stream) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   if (job < 0 || job > 4) return -11;
   if (nrhs < 1 || ldx < N->symb->n) return -10;
@@ -1844,6 +1916,7 @@ int spral_ssids_hip_solve_device(void *numeric, int job, int nrhs, double *d_x, 
 
 int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, double *d_x,
                                          int ldx, void *stream) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   if (job < 0 || job > 4) return -11;
   if (nrhs < 1 || ldx < N->symb->n) return -10;
@@ -1861,12 +1934,14 @@ int spral_ssids_hip_solve_device_noalloc(void *numeric, int job, int nrhs, doubl
 }
 
 int64_t spral_ssids_hip_solve_workspace_doubles(const void *numeric, int nrhs) {
+  ApiShared api_guard;
   return (int64_t)solve_ws_need(*static_cast<const Numeric *>(numeric), nrhs);
 }
 
 int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d_b,
                                         double *d_x, int ldx, int maxit, double tol,
                                         double *bwd, int *iters, void *stream) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   if (nrhs < 1 || ldx < N->symb->n) return -10;
   try {
@@ -1880,6 +1955,7 @@ int spral_ssids_hip_solve_refine_device(void *numeric, int nrhs, const double *d
 
 int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, double *x, int ldx,
                                  int maxit, double tol, double *bwd, int *iters) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   const int n = N->symb->n;
   if (nrhs < 1 || ldx < n) return -10;
@@ -1904,6 +1980,7 @@ int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, doubl
 }
 
 int spral_ssids_hip_enquire_posdef(const void *numeric, double *d) {
+  ApiShared api_guard;
   const Numeric *N = static_cast<const Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   if (!N->posdef) return -13;  // SSIDS_ERROR_NOT_LLT
@@ -1927,6 +2004,7 @@ int spral_ssids_hip_enquire_posdef(const void *numeric, double *d) {
 }
 
 int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d) {
+  ApiShared api_guard;
   const Numeric *N = static_cast<const Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   if (N->posdef) return -14;  // SSIDS_ERROR_NOT_LDLT
@@ -2015,6 +2093,7 @@ int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d
 }
 
 int spral_ssids_hip_alter(void *numeric, const double *d) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   if (N->posdef) return -14;  // SSIDS_ERROR_NOT_LDLT
@@ -2049,6 +2128,7 @@ int spral_ssids_hip_alter(void *numeric, const double *d) {
 }
 
 int spral_ssids_hip_save_numeric(const void *numeric, const char *path) {
+  ApiShared api_guard;
   try {
     return save_numeric(*static_cast<const Numeric *>(numeric), path);
   } catch (HipError &) {
@@ -2060,6 +2140,7 @@ int spral_ssids_hip_save_numeric(const void *numeric, const char *path) {
 
 void *spral_ssids_hip_load_numeric(void *symbolic, const char *path,
                                    struct spral_ssids_hip_stats *stats, int *flag) {
+  ApiShared api_guard;
   Numeric *N = load_numeric(static_cast<Symbolic *>(symbolic), path, flag);
   if (stats) {
     if (N) *stats = N->stats;
@@ -2069,6 +2150,7 @@ void *spral_ssids_hip_load_numeric(void *symbolic, const char *path,
 }
 
 int spral_ssids_hip_numeric_scaling(const void *numeric, double *s) {
+  ApiShared api_guard;
   const Numeric *N = static_cast<const Numeric *>(numeric);
   if (!N->d_scal) return 0;
   try {
@@ -2083,6 +2165,7 @@ int spral_ssids_hip_numeric_scaling(const void *numeric, double *s) {
 int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *ldval,
                                 const int **rlist, int *ndelay, const int **delay_perm,
                                 const double **delay_val, int *lddelay) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   *n = 0;
@@ -2152,6 +2235,7 @@ int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *
 
 int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, double *L,
                                  double *D, int *perm) {
+  ApiShared api_guard;
   const Numeric *N = static_cast<const Numeric *>(numeric);
   const Symbolic &S = *N->symb;
   if (node < 0 || node >= S.nnodes) return -1;
@@ -2175,7 +2259,20 @@ int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, doubl
   return 0;
 }
 
+int spral_ssids_hip_node_pivots(const void *numeric, int *nelim_app, int *ndelay_in) {
+  ApiShared api_guard;
+  const Numeric *N = static_cast<const Numeric *>(numeric);
+  const Symbolic &S = *N->symb;
+  for (int node = 0; node < S.nnodes; ++node) {
+    const NodeDev &Dn = N->nodes[S.slot[node]];
+    if (nelim_app) nelim_app[node] = Dn.nelim_app;
+    if (ndelay_in) ndelay_in[node] = Dn.nd;
+  }
+  return 0;
+}
+
 void spral_ssids_hip_free_contrib(void *numeric) {
+  ApiShared api_guard;
   Numeric *N = static_cast<Numeric *>(numeric);
   std::vector<double>().swap(N->contrib_val);
   std::vector<double>().swap(N->contrib_delay_val);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/spral_ssids_hip.h"
+#include "api_lock.hpp"
 
 namespace ssids_hip {
 
@@ -152,6 +153,7 @@ extern "C" {
 int spral_ssids_hip_equilib_scale_sym_device(int n, const int64_t *d_ptr, const int *d_row,
                                              const double *d_val, double *d_scaling,
                                              int max_iterations, float tol, int *iterations) {
+  ssids_hip::ApiShared api_guard;
   return ssids_hip::equilib_run(n, d_ptr, d_row, d_val, d_scaling, max_iterations, tol,
                                 iterations);
 }
@@ -172,6 +174,7 @@ int spral_ssids_hip_equilib_scale_sym(int n, const int64_t *ptr, const int *row,
   const int64_t nz = ptr[n] - 1;
   for (int64_t e = 0; e < nz; ++e)
     if (row[e] < 1 || row[e] > n) return -1;
+  ssids_hip::ApiShared api_guard;
   int64_t *d_ptr = nullptr;
   int *d_row = nullptr;
   double *d_val = nullptr, *d_s = nullptr;

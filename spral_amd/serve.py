@@ -1,24 +1,34 @@
 """A resident solver service: refactor and solve requests on patterns whose
-analyse and factors stay in HBM.
+analyse and factors stay in HBM, with refactors and solves overlapping on
+the device.
 
-* Each pattern is analysed once (ssids.AnalyseCache), and its factorization
-  handle is reused by every refactor of new values (its pool, static plan and
-  the delays seen so far persist).
-* Admission by HBM footprint: a pattern's factors stay resident while the
-  sum of resident footprints (the pool's peak bytes) fits the budget; else
-  the least recently used pattern is evicted (its last values are kept on the
-  host, and it is refactored when next needed).
-* Continuous batching: a worker thread takes every request that has arrived,
-  keeps each pattern's requests in arrival order, and merges each run of
-  consecutive solves into one multi-RHS solve; requests that arrive while
-  the GPU works join the next round. A multi-RHS solve gives each column
-  bitwise what a solve of that column alone gives, so a result does not
-  depend on what it was batched with.
+* Each pattern is analysed once (ssids.AnalyseCache; the matching-based
+  order takes the values given at registration), and its factorization
+  handles are reused by every refactor of new values (their pools, static
+  plans and the delays seen so far persist).
+* Versions: every refactor submitted for a pattern starts a new version of
+  its factors; a solve is answered with the version current when it was
+  submitted.  So a result does not depend on what else is in flight.
+* Two lanes, each a host thread driving its own HIP streams (every
+  factorization handle owns its streams): the factor lane runs refactors,
+  the solve lane runs solves.  A pattern holds up to two factorization
+  handles, so a refactor builds version v+1 in one while solves of version v
+  still run on the other; refactors of one pattern and solves of another
+  always overlap.  A solve whose version is still being factorized waits,
+  and the solve lane serves other work meanwhile.
+* Continuous batching: the solve lane takes every solve that has arrived and
+  merges those of one pattern and version into one multi-RHS solve; a
+  multi-RHS solve gives each column bitwise what a solve of that column
+  alone gives.
+* Admission by HBM footprint: a new factorization is admitted while the sum
+  of resident footprints fits the budget; otherwise the least recently used
+  idle patterns are evicted (their latest values stay on the host, and they
+  are refactorized when next needed).
 * Latency (submission to result) is recorded per request kind for p50/p99.
 """
-import queue
 import threading
 import time
+from collections import Counter, deque
 from concurrent.futures import Future
 from dataclasses import dataclass, field
 
@@ -29,20 +39,25 @@ from . import ssids
 
 @dataclass
 class _Pattern:
+    name: str
     n: int
-    ptr: np.ndarray
-    row: np.ndarray
     akeep: ssids.AKeep
-    values: np.ndarray = None
-    fkeep: ssids.FKeep = None
-    footprint: int = 0
+    version: int = 0              # versions submitted (refactors)
+    values: np.ndarray = None     # values of the latest version (host)
+    ready: dict = field(default_factory=dict)     # version -> resident FKeep
+    failed: dict = field(default_factory=dict)    # version -> why its refactor failed
+    spare: list = field(default_factory=list)     # idle handles kept for reuse
+    pending: Counter = field(default_factory=Counter)  # version -> solves not done
+    queued_refactors: int = 0
+    footprint: int = 0            # bytes per resident handle
     last_use: float = 0.0
 
 
 @dataclass
 class _Request:
-    kind: str            # "refactor" or "solve"
+    kind: str            # "refactor", "solve" or "restore" (internal)
     pattern: str
+    version: int
     data: np.ndarray
     future: Future
     t_submit: float = field(default_factory=time.perf_counter)
@@ -55,45 +70,68 @@ class SolverService:
         self.refine, self.tol = refine, tol
         self.cache = ssids.AnalyseCache()
         self.patterns = {}
-        self.q = queue.Queue()
         self.lat = {"refactor": [], "solve": []}
         self.rounds = 0
-        self._stop = threading.Event()
-        self._thread = None
+        self._cv = threading.Condition()
+        self._fq = deque()       # factor lane queue
+        self._sq = deque()       # solve lane queue (arrived, not yet taken)
+        self._waiting = []       # solves whose version is not resident yet
+        self._stop = False
+        self._threads = []
 
     # ---- client side
-    def register(self, name, n, ptr, row, grid=None):
-        ak, inf = self.cache.analyse(n, ptr, row, self.options, grid=grid)
+    def register(self, name, n, ptr, row, grid=None, val=None):
+        ak, inf = self.cache.analyse(n, ptr, row, self.options, grid=grid, val=val)
         if inf.flag < 0:
             raise RuntimeError(f"analyse of {name} failed ({inf.flag})")
-        self.patterns[name] = _Pattern(n=n, ptr=ptr, row=row, akeep=ak)
+        self.patterns[name] = _Pattern(name=name, n=n, akeep=ak)
 
     def submit_refactor(self, name, values):
-        return self._submit("refactor", name, np.array(values, np.float64, copy=True))
+        values = np.array(values, np.float64, copy=True)
+        with self._cv:
+            p = self._pattern(name)
+            p.version += 1
+            p.values = values
+            p.queued_refactors += 1
+            r = _Request("refactor", name, p.version, values, Future())
+            self._fq.append(r)
+            self._cv.notify_all()
+        return r.future
 
     def submit_solve(self, name, rhs):
-        return self._submit("solve", name, np.array(rhs, np.float64, copy=True))
+        rhs = np.array(rhs, np.float64, copy=True)
+        with self._cv:
+            p = self._pattern(name)
+            if p.version == 0:
+                raise RuntimeError(f"{name}: solve before any refactor")
+            p.pending[p.version] += 1
+            r = _Request("solve", name, p.version, rhs, Future())
+            self._sq.append(r)
+            self._cv.notify_all()
+        return r.future
 
-    def _submit(self, kind, name, data):
+    def _pattern(self, name):
         if name not in self.patterns:
             raise KeyError(name)
-        f = Future()
-        self.q.put(_Request(kind, name, data, f))
-        return f
+        return self.patterns[name]
 
     def start(self):
-        self._thread = threading.Thread(target=self._run, daemon=True)
-        self._thread.start()
+        self._threads = [threading.Thread(target=self._factor_lane, daemon=True),
+                         threading.Thread(target=self._solve_lane, daemon=True)]
+        for t in self._threads:
+            t.start()
 
     def stop(self):
-        self._stop.set()
-        self.q.put(None)
-        if self._thread:
-            self._thread.join()
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join()
         for p in self.patterns.values():
-            if p.fkeep is not None:
-                ssids.ssids_free(None, p.fkeep)
-                p.fkeep = None
+            for fk in list(p.ready.values()) + p.spare:
+                ssids.ssids_free(None, fk)
+            p.ready.clear()
+            p.spare.clear()
         self.cache.free()
 
     def latency_ms(self, kind):
@@ -103,109 +141,166 @@ class SolverService:
         return {"p50": float(np.percentile(a, 50)), "p99": float(np.percentile(a, 99)),
                 "count": int(a.size)}
 
-    # ---- worker
-    def _run(self):
-        while not self._stop.is_set():
-            first = self.q.get()
-            if first is None:
-                break
-            batch = [first]
-            while True:  # everything that has arrived joins this round
-                try:
-                    r = self.q.get_nowait()
-                except queue.Empty:
-                    break
-                if r is None:
-                    self._stop.set()
-                    break
-                batch.append(r)
-            self.rounds += 1
-            by_pattern = {}
-            for r in batch:
-                by_pattern.setdefault(r.pattern, []).append(r)
-            for name, reqs in by_pattern.items():
-                self._serve_pattern(name, reqs)
-
-    def _ensure_resident(self, p):
-        if p.fkeep is not None and p.fkeep.numeric:
-            return
-        # admit: evict least recently used patterns until the footprint fits
-        resident = sorted((q for q in self.patterns.values()
-                           if q is not p and q.fkeep is not None), key=lambda q: q.last_use)
-        used = sum(q.footprint for q in resident)
-        while resident and used + p.footprint > self.budget:
-            q = resident.pop(0)
-            ssids.ssids_free(None, q.fkeep)
-            q.fkeep = None
-            used -= q.footprint
-        self._factor(p)
-
-    def _factor(self, p):
-        fk, inf = ssids.ssids_factor(False, p.values, p.akeep, self.options, fkeep=p.fkeep)
-        p.fkeep = fk
-        return inf
-
-    def _serve_pattern(self, name, reqs):
-        p = self.patterns[name]
-        i = 0
-        while i < len(reqs):
-            r = reqs[i]
-            if r.kind == "refactor":
-                p.values = r.data
-                try:
-                    if p.fkeep is None:
-                        self._ensure_resident(p)
-                        inf = p.fkeep.inform
-                    else:
-                        inf = self._factor(p)
-                    p.footprint = max(p.footprint, self._footprint(p))
-                    p.last_use = time.perf_counter()
-                    r.future.set_result(inf)
-                except Exception as e:  # the request fails, the service goes on
-                    r.future.set_exception(e)
-                self.lat["refactor"].append(time.perf_counter() - r.t_submit)
-                i += 1
-                continue
-            # a run of solves: one multi-RHS call
-            j = i
-            while j < len(reqs) and reqs[j].kind == "solve":
-                j += 1
-            run = reqs[i:j]
+    # ---- factor lane
+    def _factor_lane(self):
+        while True:
+            with self._cv:
+                while not self._fq and not self._stop:
+                    self._cv.wait()
+                if self._stop and not self._fq:
+                    return
+                r = self._fq.popleft()
+                p = self.patterns[r.pattern]
+                # a handle to build into: a spare one, else the oldest
+                # resident version once none of its solves is outstanding
+                # (at most two handles per pattern)
+                while True:
+                    fk = self._take_handle(p)
+                    if fk is not False:
+                        break
+                    self._cv.wait()
+                if fk is None:
+                    self._admit(p)
             try:
-                if p.values is None:
-                    raise RuntimeError(f"{name}: solve before any refactor")
-                self._ensure_resident(p)
-                cols = [r.data.reshape(p.n, -1) for r in run]
-                x = np.concatenate(cols, axis=1)
-                if self.refine:
-                    inf, bwd, its = ssids.ssids_solve_refine(x, p.akeep, p.fkeep, self.options,
-                                                             tol=self.tol)
+                fk, inf = ssids.ssids_factor(False, r.data, p.akeep, self.options, fkeep=fk)
+                ok = inf.flag >= 0 and fk.numeric
+            except Exception as e:  # the request fails, the service goes on
+                inf, ok = e, False
+            with self._cv:
+                if ok:
+                    p.ready[r.version] = fk
+                    p.footprint = max(p.footprint, self._footprint(p, fk))
+                    # older versions nobody will solve with again become spare
+                    for v in [v for v in p.ready if v < r.version and p.pending[v] == 0]:
+                        p.spare.append(p.ready.pop(v))
                 else:
-                    inf = ssids.ssids_solve(x, p.akeep, p.fkeep, self.options)
-                if inf.flag < 0:
-                    raise RuntimeError(f"{name}: solve failed ({inf.flag})")
+                    p.failed[r.version] = (inf if isinstance(inf, Exception) else
+                                           RuntimeError(f"{p.name}: refactor failed "
+                                                        f"(flag {inf.flag}, HIP error "
+                                                        f"{inf.cuda_error})"))
+                    if fk is not None and getattr(fk, "numeric", None):
+                        p.spare.append(fk)
                 p.last_use = time.perf_counter()
-                c = 0
-                for r in run:
-                    k = r.data.reshape(p.n, -1).shape[1]
-                    out = x[:, c:c + k]
-                    r.future.set_result(out.reshape(r.data.shape).copy())
-                    c += k
-            except Exception as e:
-                for r in run:
-                    r.future.set_exception(e)
-            now = time.perf_counter()
+                if r.kind == "refactor":
+                    p.queued_refactors -= 1
+                self._cv.notify_all()
+            if r.kind == "refactor":
+                if isinstance(inf, Exception):
+                    r.future.set_exception(inf)
+                else:
+                    r.future.set_result(inf)
+                self.lat["refactor"].append(time.perf_counter() - r.t_submit)
+
+    def _take_handle(self, p):
+        """A handle for the next factorization of p (None: a new one; False:
+        wait for a resident version's solves to finish)."""
+        if p.spare:
+            return p.spare.pop()
+        if len(p.ready) < 2:
+            return None
+        oldest = min(p.ready)
+        if p.pending[oldest] == 0:
+            return p.ready.pop(oldest)
+        return False
+
+    def _admit(self, p):
+        """Evict least recently used idle patterns until one more handle of p
+        fits the budget."""
+        def used():
+            return sum(q.footprint * (len(q.ready) + len(q.spare))
+                       for q in self.patterns.values())
+        idle = sorted((q for q in self.patterns.values()
+                       if q is not p and not q.queued_refactors and not sum(q.pending.values())
+                       and (q.ready or q.spare)), key=lambda q: q.last_use)
+        while idle and used() + p.footprint > self.budget:
+            q = idle.pop(0)
+            for fk in list(q.ready.values()) + q.spare:
+                ssids.ssids_free(None, fk)
+            q.ready.clear()
+            q.spare.clear()
+
+    # ---- solve lane
+    def _solve_lane(self):
+        while True:
+            with self._cv:
+                while not self._sq and not self._stop and not self._servable():
+                    self._cv.wait()
+                if self._stop and not self._sq and not self._waiting:
+                    return
+                self._waiting.extend(self._sq)
+                self._sq.clear()
+                groups, keep = {}, []
+                for r in self._waiting:
+                    p = self.patterns[r.pattern]
+                    if r.version in p.failed:
+                        p.pending[r.version] -= 1
+                        r.future.set_exception(p.failed[r.version])
+                        self.lat["solve"].append(time.perf_counter() - r.t_submit)
+                        continue
+                    if r.version in p.ready:
+                        groups.setdefault((r.pattern, r.version), []).append(r)
+                        continue
+                    keep.append(r)
+                    # an evicted version (the latest one) is factorized again
+                    if (r.version == p.version and not p.queued_refactors
+                            and not any(q.kind == "restore" and q.pattern == r.pattern
+                                        for q in self._fq)):
+                        self._fq.append(_Request("restore", r.pattern, r.version,
+                                                 p.values, Future()))
+                        self._cv.notify_all()
+                self._waiting = keep
+                if not groups:
+                    if self._stop and not self._fq:
+                        for r in keep:
+                            r.future.set_exception(RuntimeError("service stopped"))
+                        return
+                    self._cv.wait(0.05)
+                    continue
+                self.rounds += 1
+                work = [(self.patterns[k[0]], k[1], rs) for k, rs in groups.items()]
+            for p, v, rs in work:
+                self._solve_group(p, v, rs)
+
+    def _servable(self):
+        return any(r.version in self.patterns[r.pattern].ready or
+                   r.version in self.patterns[r.pattern].failed for r in self._waiting)
+
+    def _solve_group(self, p, v, run):
+        fk = p.ready[v]  # stays resident: its pending count is above zero
+        try:
+            cols = [r.data.reshape(p.n, -1) for r in run]
+            x = np.concatenate(cols, axis=1)
+            if self.refine:
+                inf, bwd, its = ssids.ssids_solve_refine(x, p.akeep, fk, self.options,
+                                                         tol=self.tol)
+            else:
+                inf = ssids.ssids_solve(x, p.akeep, fk, self.options)
+            if inf.flag < 0:
+                raise RuntimeError(f"{p.name}: solve failed ({inf.flag})")
+            c = 0
             for r in run:
-                self.lat["solve"].append(now - r.t_submit)
-            i = j
+                k = r.data.reshape(p.n, -1).shape[1]
+                r.future.set_result(x[:, c:c + k].reshape(r.data.shape).copy())
+                c += k
+        except Exception as e:
+            for r in run:
+                r.future.set_exception(e)
+        now = time.perf_counter()
+        for r in run:
+            self.lat["solve"].append(now - r.t_submit)
+        with self._cv:
+            p.pending[v] -= len(run)
+            p.last_use = now
+            if p.pending[v] == 0 and v < p.version and v in p.ready and \
+                    any(w > v for w in p.ready):
+                p.spare.append(p.ready.pop(v))  # superseded and idle
+            self._cv.notify_all()
 
     @staticmethod
-    def _footprint(p):
-        """HBM held by a resident pattern: the factorization pool's peak
+    def _footprint(p, fk):
+        """HBM held by one resident handle: the factorization pool's peak
         (factors, contribution blocks, workspaces) plus the solve workspace
         for the largest right-hand-side count served (64)."""
-        peak = int(p.fkeep.inform.pool_peak_bytes)
-        # per column: every front's rows (the row lists, twice for the
-        # backward partial sums) and four vectors of n
+        peak = int(fk.inform.pool_peak_bytes)
         solve_ws = 8 * 64 * (2 * len(p.akeep.rlist) + 4 * int(p.akeep.n))
         return max(peak, 8 * int(p.akeep.num_factor)) + solve_ws

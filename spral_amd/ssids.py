@@ -16,8 +16,9 @@ not available; 2 is the reference's matching-based order (maximum-product
 matching, matched pairs ordered together; needs val) with approximate minimum
 degree on the compressed graph; 3 is geometric nested dissection for an
 nx*ny*nz grid (pass grid=(nx,ny,nz)); 4 is the approximate minimum degree at
-any size.  Scalings: options.scaling 1 is the matching (Hungarian) scaling, 3
-the scaling of ordering 2, >= 4 norm equilibration on the device.
+any size.  Scalings: options.scaling 1 is the matching (Hungarian) scaling, 2
+the auction scaling, 3 the scaling of ordering 2, >= 4 norm equilibration on
+the device.
 """
 import ctypes as C
 from dataclasses import dataclass, field
@@ -335,9 +336,24 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
             scale[:] = akeep.scaling
         scale = akeep.scaling
     elif options.scaling == 2:
-        raise NotImplementedError(
-            "options.scaling=2 (auction) is not built; use 1 (matching), 3, 0 with a user "
-            "scaling or >= 4 (equilibration)")
+        # auction scaling (src/ssids/ssids.f90:952-977) with the default
+        # auction_options (src/scaling.f90:33-38)
+        s = np.zeros(akeep.n)
+        hp, hr = _as_i64(akeep.ptr), _as_i32(akeep.row)
+        mu = np.array([10, 100, 100], np.int32)
+        mp = np.array([0.90, 0.0, 0.0], np.float32)
+        nm, it, un = C.c_int(0), C.c_int(0), C.c_int(0)
+        af = L.spral_auction_scale_sym(akeep.n, hp.ctypes.data, hr.ctypes.data,
+                                       val.ctypes.data, s.ctypes.data, None, 30000,
+                                       mu.ctypes.data, mp.ctypes.data,
+                                       float(np.float32(0.01)), C.byref(nm), C.byref(it),
+                                       C.byref(un))
+        if af < 0:
+            inf = SSIDSInform(flag=SSIDS_ERROR_ALLOCATION)
+            return FKeep(inform=inf), inf
+        if scale is not None:
+            scale[:] = s
+        scale = s
     if scale is not None:
         # scaling is applied in elimination order inside the subtree
         sc = np.ascontiguousarray(np.asarray(scale, np.float64)[akeep.invp])
@@ -644,11 +660,13 @@ class AnalyseCache:
         h.update(repr((options.ordering, options.nemin, options.device, grid)).encode())
         return h.hexdigest()
 
-    def analyse(self, n, ptr, row, options=None, grid=None):
+    def analyse(self, n, ptr, row, options=None, grid=None, val=None):
+        """val: the values the matching-based order (options.ordering = 2)
+        is computed from, for the first analyse of the pattern."""
         options = options or SSIDSOptions()
         k = self.key(n, ptr, row, options, grid)
         if k not in self._store:
-            self._store[k] = ssids_analyse(False, n, ptr, row, options, grid=grid)
+            self._store[k] = ssids_analyse(False, n, ptr, row, options, grid=grid, val=val)
         return self._store[k]
 
     def __len__(self):

@@ -4,8 +4,8 @@ a saddle-point system's constraint rows are eliminated next to their
 matched partners: the device factorization delays nothing, keeps the
 closed-form inertia (num_neg = constraint rows, H positive definite and A of
 full row rank) and solves to the 1e-14 backward error.  The Hungarian
-scaling (options.scaling = 1) gives the same inertia and accuracy on any
-order."""
+(options.scaling = 1) and auction (2) scalings give the same inertia and
+accuracy on any order."""
 import numpy as np
 import pytest
 
@@ -49,9 +49,12 @@ def test_matching_order_no_delays(lib, case):
     assert delays4 > 0
 
 
-def test_hungarian_scaling_factor(lib):
+@pytest.mark.parametrize("scaling", [1, 2])
+def test_matching_scalings_factor(lib, scaling):
+    """Hungarian (1) and auction (2) scalings: same inertia and rank as the
+    unscaled factorization, 1e-14 after refinement."""
     m = matrices.random_indef(500, density=0.02, seed=17)
-    print(f"hungarian scaling n={m.n}", flush=True)
-    neg1, _, rank1, _ = _run(m, 4, 1)
+    print(f"matching scaling {scaling} n={m.n}", flush=True)
+    neg1, _, rank1, _ = _run(m, 4, scaling)
     neg0, _, rank0, _ = _run(m, 4, 0)
     assert neg1 == neg0 and rank1 == rank0

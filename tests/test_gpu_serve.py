@@ -92,3 +92,55 @@ def test_service_results_equal_alone(lib):
                 assert matrices.bwd_error(full, x, cols[:, j]).max() <= BWD_TOL
         ssids.ssids_free(ak, fk)
     assert svc.rounds < nreq
+
+
+def test_refactor_and_solves_overlap(lib):
+    """A long refactor of one pattern and solves of another are in flight
+    together: solves submitted after the refactor complete before it does,
+    and a solve of the resident version of the refactoring pattern itself is
+    served while its next version is built.  Results equal replays alone."""
+    import time
+    big = matrices.laplace3d(40, diag=5.7)          # ~30 ms factorizations
+    small = matrices.random_indef(300, density=0.03, seed=5)
+    svc = serve.SolverService(options=ssids.SSIDSOptions())
+    svc.register("big", big.n, big.ptr, big.row)
+    svc.register("small", small.n, small.ptr, small.row)
+    svc.start()
+    svc.submit_refactor("big", big.val).result(timeout=120)
+    svc.submit_refactor("small", small.val).result(timeout=120)
+    done = {}
+
+    def stamp(key):
+        return lambda f: done.setdefault(key, time.perf_counter())
+
+    rng = np.random.default_rng(3)
+    b_big = rng.uniform(-1, 1, big.n)
+    f_old = svc.submit_solve("big", b_big)          # version 1 of big
+    f_ref = svc.submit_refactor("big", big.val * 1.25)
+    f_ref.add_done_callback(stamp("refactor"))
+    f_new = svc.submit_solve("big", b_big)          # version 2: waits for the refactor
+    small_f = []
+    for i in range(6):
+        f = svc.submit_solve("small", rng.uniform(-1, 1, small.n))
+        f.add_done_callback(stamp(f"small{i}"))
+        small_f.append(f)
+    x_old = f_old.result(timeout=120)
+    x_new = f_new.result(timeout=120)
+    for f in small_f:
+        f.result(timeout=120)
+    f_ref.result(timeout=120)
+    svc.stop()
+    early = [k for k in done if k.startswith("small") and done[k] < done["refactor"]]
+    print(f"  small solves done before the big refactor: {len(early)} of 6", flush=True)
+    assert early, done
+    # version semantics: the solve submitted before the refactor used the old
+    # values, the one after it the new ones
+    o = ssids.SSIDSOptions()
+    ak, _ = ssids.ssids_analyse(False, big.n, big.ptr, big.row, o)
+    for vals, got in ((big.val, x_old), (big.val * 1.25, x_new)):
+        fk, _ = ssids.ssids_factor(False, vals, ak, o)
+        x = b_big.copy()
+        ssids.ssids_solve_refine(x, ak, fk, o, tol=BWD_TOL)
+        assert np.array_equal(got, x)
+        ssids.ssids_free(None, fk)
+    ssids.ssids_free(ak)

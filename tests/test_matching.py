@@ -169,3 +169,44 @@ def test_match_order_needs_values(lib):
     o = ssids.SSIDSOptions(ordering=2, use_gpu=False)
     ak, inf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, o)
     assert inf.flag == ssids.SSIDS_ERROR_VAL
+
+
+# ---- auction scaling (options.scaling = 2) -----------------------------------
+
+def auction(lib, n, ptr, row, val, max_iterations=30000, max_unchanged=(10, 100, 100),
+            min_proportion=(0.90, 0.0, 0.0), eps_initial=0.01):
+    ptr = np.ascontiguousarray(ptr, np.int64)
+    row = np.ascontiguousarray(row, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    s = np.zeros(max(n, 1))
+    mt = np.zeros(max(n, 1), np.int32)
+    mu = np.ascontiguousarray(max_unchanged, np.int32)
+    mp = np.ascontiguousarray(min_proportion, np.float32)
+    nm, it, un = C.c_int(0), C.c_int(0), C.c_int(0)
+    fl = lib.spral_auction_scale_sym(n, ptr.ctypes.data, row.ctypes.data, val.ctypes.data,
+                                     s.ctypes.data, mt.ctypes.data, max_iterations,
+                                     mu.ctypes.data, mp.ctypes.data, float(np.float32(eps_initial)),
+                                     C.byref(nm), C.byref(it), C.byref(un))
+    return s[:n], mt[:n], nm.value, it.value, fl
+
+
+@pytest.mark.skipif(not osc.ref_available(), reason="reference scaling (oracle/_ref) not built")
+@pytest.mark.parametrize("name,n,mat", CASES, ids=[c[0] for c in CASES])
+def test_auction_matches_reference_bitwise(lib, name, n, mat):
+    ptr, row, val = mat
+    s, mt, nm, it, fl = auction(lib, n, ptr, row, val)
+    rs, rmt, rnm, rit, rfl = osc.ref_auction_scale_sym(n, ptr, row, val)
+    assert (nm, it, fl) == (rnm, rit, rfl), (name, nm, rnm, it, rit)
+    assert np.array_equal(mt, rmt), name
+    assert np.array_equal(s, rs), (name, np.max(np.abs(s - rs) / rs))
+
+
+def test_auction_scaled_entries_bounded(lib):
+    """The auction's approximate duals keep every scaled entry of S A S near
+    or below 1 (within the eps the last bids paid) on a KKT system."""
+    m = matrices.kkt_laplace(grid=6, m_con=60)
+    s, mt, nm, it, fl = auction(lib, m.n, m.ptr, m.row, m.val)
+    assert fl == 0 and nm > 0.9 * m.n
+    col = np.repeat(np.arange(m.n), np.diff(m.ptr))
+    sv = np.abs(s[col] * m.val * s[np.asarray(m.row) - 1])
+    assert sv.max() < 10.0
