@@ -124,16 +124,28 @@ def batch_bench(args):
     count = args.batch
     rng = np.random.Generator(np.random.PCG64(20261202 + 4))
     sizes = rng.integers(200, 5001, count)
-    mats = [matrices.kkt_random(int(n), seed=20261202 + 4 + i) for i, n in enumerate(sizes)]
+
+    def note(msg):
+        print(f"  batch: {msg}", file=sys.stderr, flush=True)
+
+    mats = []
+    for i, n in enumerate(sizes):
+        mats.append(matrices.kkt_random(int(n), seed=20261202 + 4 + i))
+        if (i + 1) % 512 == 0:
+            note(f"generated {i + 1} of {count}")
     kord = args.kkt_ordering
+    note("analyse")
     b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=kord,
                                                           scaling=3 if kord == 2 else 0))
     assert inf.flag >= 0, inf
+    note(f"analysed: {int(b.akeep.nnodes)} fronts")
     rhs = [m.full @ rng.uniform(-1, 1, m.n) for m in mats]
     val = np.concatenate([m.val for m in mats])
     for _ in range(args.warmup):
-        batch.batch_factor(b, val)
+        finf, _, _ = batch.batch_factor(b, val)
+        assert finf.flag >= 0, finf
         batch.batch_solve(b, rhs)
+        note("warm-up step done")
     devmem.synchronize()
     ts = []
     for _ in range(args.steps):
@@ -142,6 +154,8 @@ def batch_bench(args):
         xs = batch.batch_solve(b, rhs)
         devmem.synchronize()
         ts.append(time.perf_counter() - t0)
+        assert finf.flag >= 0, finf
+        note(f"step {ts[-1] * 1e3:.1f} ms")
     tb = float(np.mean(ts))
     bwd = max(float(matrices.bwd_error(m.full, x, r).max()) for m, x, r in zip(mats, xs, rhs))
     ok = all(int(k) == int(0.2 * m.n) for k, m in zip(neg, mats))
@@ -151,14 +165,19 @@ def batch_bench(args):
         o = ssids.SSIDSOptions(ordering=0)
         ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, o, order=b.orders[i])
         singles.append((ak, o))
+        if (i + 1) % 512 == 0:
+            note(f"single analyses: {i + 1} of {count}")
     devmem.synchronize()
     t0 = time.perf_counter()
     for i, ((ak, o), m, r) in enumerate(zip(singles, mats, rhs)):
         sc = None if b.scale is None else b.scale[b.offsets[i]:b.offsets[i + 1]].copy()
         fk, fi = ssids.ssids_factor(False, m.val, ak, o, scale=sc)
+        assert fi.flag >= 0, fi
         x = r.copy()
         ssids.ssids_solve(x, ak, fk, o)
         ssids.ssids_free(None, fk)
+        if (i + 1) % 512 == 0:
+            note(f"loop of single calls: {i + 1} of {count}")
     devmem.synchronize()
     tl = time.perf_counter() - t0
     for ak, _ in singles:
@@ -209,34 +228,52 @@ def kkt27_bench(args):
         opts.schur_precision = prec
         fk = None
         fac, sol = [], []
+        failed = None
         for step in range(args.warmup + args.steps):
             devmem.synchronize()
             t0 = time.perf_counter()
             fk, inf = ssids.ssids_factor(False, m.val, ak, opts, fkeep=fk)
             devmem.synchronize()
             t1 = time.perf_counter()
+            print(f"  precision {prec} step {step}: factor {(t1 - t0) * 1e3:.1f} ms, flag "
+                  f"{inf.flag}, HIP error {inf.cuda_error}, pool peak "
+                  f"{inf.pool_peak_bytes / 2 ** 30:.1f} GiB", file=sys.stderr, flush=True)
+            if inf.flag < 0:
+                failed = {"failed": True, "flag": int(inf.flag),
+                          "hip_error": int(inf.cuda_error),
+                          "pool_peak_gib": round(inf.pool_peak_bytes / 2 ** 30, 1)}
+                break
             x = b.copy()
-            _, _, its = ssids.ssids_solve_refine(x, ak, fk, opts, tol=1e-14)
+            sinf, _, its = ssids.ssids_solve_refine(x, ak, fk, opts, tol=1e-14)
             t2 = time.perf_counter()
-            print(f"  precision {prec} step {step}: factor {(t1 - t0) * 1e3:.1f} ms, "
-                  f"solve {(t2 - t1) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+            assert sinf.flag >= 0, sinf
+            print(f"  precision {prec} step {step}: solve {(t2 - t1) * 1e3:.1f} ms",
+                  file=sys.stderr, flush=True)
             if step >= args.warmup:
                 fac.append(t1 - t0)
                 sol.append(t2 - t1)
+        if failed is not None:
+            res[prec] = failed
+            if fk is not None:
+                ssids.ssids_free(None, fk)
+            continue
         res[prec] = {"factor_ms": round(float(np.mean(fac)) * 1e3, 1),
                      "gflops": round(ak.num_flops / float(np.mean(fac)) / 1e9, 1),
                      "solve_ms": round(float(np.mean(sol)) * 1e3, 2), "refine_steps": int(its),
                      "bwd_err": float(matrices.bwd_error(m.full, x, b).max()),
                      "fwd_err_max": float(np.abs(x - x_true).max()),
                      "num_neg": int(inf.num_neg), "num_delay": int(inf.num_delay),
-                     "num_two": int(inf.num_two), "inertia_ok": int(inf.num_neg) == mc}
+                     "num_two": int(inf.num_two), "inertia_ok": int(inf.num_neg) == mc,
+                     "pool_peak_gib": round(inf.pool_peak_bytes / 2 ** 30, 1)}
         ssids.ssids_free(None, fk)
     ssids.ssids_free(ak)
     f64 = res[0]
     print(json.dumps({
         "metric": "factor GFLOP/s + solve ms at bwd err <=1e-14, FP64 vs mixed",
-        "value": f64["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(f64["factor_ms"] + f64["solve_ms"], 1),
+        "value": f64.get("gflops"), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": (round(f64["factor_ms"] + f64["solve_ms"], 1)
+                        if "factor_ms" in f64 else None),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"KKT: 27-point H on {g}^3, {mc} constraint rows x 4, "

@@ -7,6 +7,8 @@ namespace ssids_hip {
 
 constexpr int kBlk = 64;        // APP pivot block width (columns per step)
 constexpr int kThreads = 256;   // threads per workgroup in most kernels
+// least workgroups of a large front's APP trailing update (two per CU)
+constexpr int kUpdMinWgs = 512;
 
 // Per-node descriptor, stored contiguously in tree-level order ("slots").
 // Front layout (rows and columns): [delays in (nd) | own columns (sn) |
@@ -120,7 +122,12 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
     const int t0 = imin(n, (stp + 1) * kBlk);
     const int TI = idiv_up(m - t0, kBlk), TJ = idiv_up(n - t0, kBlk);
     const int nt = TJ * TI - TJ * (TJ - 1) / 2;
-    return imax(1, imin(nt, 4096));
+    // the count above assumes every earlier pivot passed; failed columns
+    // leave a larger trailing panel (down to the whole front), which the
+    // workgroups stride over, so a large front keeps at least kUpdMinWgs
+    const int TM = idiv_up(m, kBlk), TN = idiv_up(n, kBlk);
+    const int nt_all = TN * TM - TN * (TN - 1) / 2;
+    return imax(1, imax(imin(nt, 4096), imin(nt_all, kUpdMinWgs)));
   }
   if (t == 2 * S + 2) {
     if (!is_root || m <= n) return 0;

@@ -231,7 +231,12 @@ struct Plan {
   size_t tabs_cap = 0;
   int *d_pack = nullptr;                   // 8 ints per front
   std::vector<std::pair<void *, size_t>> allocs;
+  // the large transient blocks (contribution blocks, L21 D, workspaces of
+  // kArenaMin bytes or more), at offsets packed by lifetime
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
   ~Plan() {
+    if (arena) (void)hipFree(arena);
     if (d_tmpl) (void)hipFree(d_tmpl);
     if (d_tabs) (void)hipFree(d_tabs);
     if (d_pack) (void)hipFree(d_pack);
@@ -550,12 +555,40 @@ static void node_static(const Symbolic &S, int sl, NodeDev &D) {
   D.acol = S.d_acol + S.acol_off[i];
 }
 
+// The contribution block and L21 D share one allocation (same lifetime)
+static void set_contrib(NodeDev &D, char *base, size_t cbytes, size_t ldbytes) {
+  D.C = (D.ldc > 0) ? (double *)base : nullptr;
+  D.LD = ldbytes ? (double *)(base + cbytes) : nullptr;
+}
+// The APP workspace: backups, the block inverse, the swap buffer
+static void set_work(NodeDev &D, char *w, int nsteps) {
+  D.bak = (double *)w;
+  D.dbak = D.bak + (size_t)D.ldbak * kBlk;
+  D.W = D.dbak + kBlk * kBlk;
+  D.swp = (nsteps > 1) ? D.W + kBlk * kBlk : nullptr;
+}
+
+// Bytes of a front's transient blocks (D.ldl, D.ldc, D.ldbak set): the
+// contribution block (FP32 under mixed precision), L21 D (not for roots) and
+// the APP workspace
+static void trans_sizes(const Numeric &N, const NodeDev &D, int n, int nsteps, int m,
+                        size_t &cbytes, size_t &ldbytes, size_t &wb) {
+  const size_t cesz = N.c32 ? sizeof(float) : sizeof(double);
+  cbytes = ((size_t)D.ldc * (D.sm - D.sn) * cesz + 255) & ~size_t(255);
+  ldbytes = D.is_root ? 0 : (size_t)D.ldl * n * sizeof(double);
+  wb = ((size_t)D.ldbak * kBlk + 2 * kBlk * kBlk) * sizeof(double);
+  if (nsteps > 1) wb += (size_t)2 * kBlk * m * sizeof(double);
+}
+
 // Buffers of a front with room for m rows / n fully-summed columns.
 // Persistent factors come from `keep`; the contribution block and the
-// workspace are returned through cb/wk for the caller to release.
-template <class Keep>
+// workspace come from `trans(bytes, kind)` (kind 0: contribution block,
+// 1: workspace) and are returned through cb/wk for the
+// caller to release.  `trans` may return null for a block placed later
+// (set_contrib / set_work then set its pointers).
+template <class Keep, class Trans>
 static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int m, int n,
-                         int nsteps, Keep keep, std::pair<void *, size_t> &cb,
+                         int nsteps, Keep keep, Trans trans, std::pair<void *, size_t> &cb,
                          std::pair<void *, size_t> &wk) {
   const int i = S.node_of[sl];
   // columns start on 128-byte lines (16 doubles), so a 64-row tile column is
@@ -571,31 +604,64 @@ static void node_buffers(Numeric &N, const Symbolic &S, int sl, NodeDev &D, int 
   D.perm = (int *)keep(std::max(n, 1) * sizeof(int));
   D.perm0 = (int *)keep(std::max(n, 1) * sizeof(int));
   D.ipos = (int *)keep(std::max(n, 1) * sizeof(int));
-  // contribution block and L21 D share one allocation (same lifetime)
-  const size_t cesz = N.c32 ? sizeof(float) : sizeof(double);
-  const size_t cbytes = ((size_t)D.ldc * (D.sm - D.sn) * cesz + 255) & ~size_t(255);
-  const size_t ldbytes = D.is_root ? 0 : (size_t)D.ldl * n * sizeof(double);
-  char *cbase = (cbytes + ldbytes) ? (char *)N.pool.alloc(cbytes + ldbytes) : nullptr;
-  D.C = (D.ldc > 0) ? (double *)cbase : nullptr;
-  D.LD = ldbytes ? (double *)(cbase + cbytes) : nullptr;
+  size_t cbytes, ldbytes, wb;
+  trans_sizes(N, D, n, nsteps, m, cbytes, ldbytes, wb);
+  char *cbase = (cbytes + ldbytes) ? (char *)trans(cbytes + ldbytes, 0) : nullptr;
+  if (cbase) set_contrib(D, cbase, cbytes, ldbytes);
   cb = {cbase, cbytes + ldbytes};
-  size_t wb = ((size_t)D.ldbak * kBlk + 2 * kBlk * kBlk) * sizeof(double);
-  if (nsteps > 1) wb += (size_t)2 * kBlk * m * sizeof(double);
   // child tile tables: built by the assembly, kept for the forward solve's gather
   const size_t tabn = (size_t)(cdiv(m, 64) + 1) * std::max<size_t>(S.children[i].size(), 1);
   D.ctab = (int *)keep(tabn * sizeof(int));
-  char *w = (char *)N.pool.alloc(wb);
+  char *w = (char *)trans(wb, 1);
+  if (w) set_work(D, w, nsteps);
   wk = {w, wb};
-  D.bak = (double *)w;
-  D.dbak = D.bak + (size_t)D.ldbak * kBlk;
-  D.W = D.dbak + kBlk * kBlk;
-  D.swp = (nsteps > 1) ? D.W + kBlk * kBlk : nullptr;
 }
 
+// Called with no launch of this handle in flight (a factorization starts, or
+// a static factorization has ended and synchronised).
 static void plan_release(Numeric &N) {
   for (auto &a : N.plan.allocs) N.pool.release(a.first, a.second);
   N.plan.allocs.clear();
+  if (N.plan.arena) (void)hipFree(N.plan.arena);
+  N.plan.arena = nullptr;
+  N.plan.arena_bytes = 0;
   N.plan.valid = false;
+}
+
+// Offsets for blocks (bytes, first level, last level) with closed lifetimes:
+// largest first, each at the lowest offset free of every placed block whose
+// lifetime meets its own.  Returns the arena size.
+struct ArenaBlock {
+  size_t bytes;
+  int t0, t1;
+  int slot, kind;  // kind 0: contribution block + L21 D, 1: workspace
+  size_t off;
+};
+static size_t arena_place(std::vector<ArenaBlock> &b) {
+  std::vector<int> order(b.size());
+  for (size_t k = 0; k < b.size(); ++k) order[k] = (int)k;
+  std::sort(order.begin(), order.end(), [&](int x, int y) {
+    if (b[x].bytes != b[y].bytes) return b[x].bytes > b[y].bytes;
+    return x < y;
+  });
+  std::vector<int> placed;
+  std::vector<std::pair<size_t, size_t>> busy;
+  size_t arena = 0;
+  for (int k : order) {
+    busy.clear();
+    for (int j : placed)
+      if (!(b[j].t1 < b[k].t0 || b[k].t1 < b[j].t0)) busy.push_back({b[j].off, b[j].bytes});
+    std::sort(busy.begin(), busy.end());
+    size_t cur = 0;
+    for (auto &o : busy) {
+      if (o.first >= cur + b[k].bytes) break;
+      cur = std::max(cur, o.first + o.second);
+    }
+    b[k].off = cur;
+    placed.push_back(k);
+    arena = std::max(arena, cur + b[k].bytes);
+  }
+  return arena;
 }
 
 // Place every front for a delay capacity of
@@ -623,9 +689,30 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
     P.allocs.push_back({p, b});
     return p;
   };
+  // Transient blocks of kArenaMin bytes or more go to one arena, placed by
+  // lifetime once every front is sized: a contribution block lives from its
+  // front's level to its parent's, a workspace for its own level.  In the
+  // pool, the space such a block frees is split by the smaller kept blocks
+  // placed after it while the blocks grow toward the root, so each level near
+  // the root would take a new chunk (a 27-point KKT tree at 56^3: 334 GiB of
+  // chunks for 97 GiB of live blocks; with the arena 145 GiB).
+  constexpr size_t kArenaMin = size_t(64) << 20;
+  std::vector<int> lev_of(nn);
+  for (int L = 0; L < nlev; ++L)
+    for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl) lev_of[sl] = L;
+  std::vector<ArenaBlock> big;
+  int cur_sl = 0, cur_lev = 0;
+  auto trans = [&](size_t b, int kind) -> void * {
+    if (b < kArenaMin) return N.pool.alloc(b);
+    const NodeDev &D = P.tmpl[cur_sl];
+    const int t1 = (kind == 1) ? cur_lev : (D.parent >= 0 ? lev_of[D.parent] : nlev);
+    big.push_back({(b + 255) & ~size_t(255), cur_lev, t1, cur_sl, kind, 0});
+    return nullptr;
+  };
   size_t tab_total = 0;
   for (int L = 0; L < nlev; ++L) {
     const int s0 = S.lvl_ptr[L], s1 = S.lvl_ptr[L + 1], nl = s1 - s0;
+    cur_lev = L;
     Plan::Level &lv = P.lv[L];
     std::vector<int> mc(nl), nc(nl), sc(nl);
     for (int sl = s0; sl < s1; ++sl) {
@@ -650,7 +737,8 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
       sc[sl - s0] = nsteps;
       lv.maxsteps = std::max(lv.maxsteps, nsteps);
       if (nsteps > 1) lv.multi = true;
-      node_buffers(N, S, sl, D, m, n, nsteps, keep, cof[sl], wof[sl]);
+      cur_sl = sl;
+      node_buffers(N, S, sl, D, m, n, nsteps, keep, trans, cof[sl], wof[sl]);
     }
     const int ntab = level_tables(lv.maxsteps);
     lv.cap.assign(ntab, 0);
@@ -672,6 +760,23 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
   }
   for (int sl = 0; sl < nn; ++sl)  // roots keep their contribution block
     if (cof[sl].first) P.allocs.push_back(cof[sl]);
+  if (!big.empty()) {
+    P.arena_bytes = arena_place(big);
+    HIPCHK(hipMalloc(&P.arena, P.arena_bytes));
+    for (const ArenaBlock &a : big) {
+      NodeDev &D = P.tmpl[a.slot];
+      const int n = D.sn + D.ndcap, m = D.sm + D.ndcap;
+      const int nsteps = pivot_tpp ? 0 : cdiv(n, kBlk);
+      char *base = (char *)P.arena + a.off;
+      if (a.kind == 0) {
+        size_t cbytes, ldbytes, wb;
+        trans_sizes(N, D, n, nsteps, m, cbytes, ldbytes, wb);
+        set_contrib(D, base, cbytes, ldbytes);
+      } else {
+        set_work(D, base, nsteps);
+      }
+    }
+  }
   if (!P.d_tmpl) HIPCHK(hipMalloc(&P.d_tmpl, std::max(nn, 1) * sizeof(NodeDev)));
   HIPCHK(hipMemcpy(P.d_tmpl, P.tmpl.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice));
   if (tab_total > P.tabs_cap) {
@@ -872,7 +977,9 @@ static void factor_dynamic(Numeric &N, const FactorParams &fp,
       D.nsteps = fp.pivot_tpp ? 0 : cdiv(D.n, kBlk);
       maxsteps = std::max(maxsteps, D.nsteps);
       if (D.nsteps > 1) multi = true;
-      node_buffers(N, S, sl, D, D.m, D.n, D.nsteps, keep, contrib_of[sl], work_of[sl]);
+      node_buffers(N, S, sl, D, D.m, D.n, D.nsteps, keep,
+                   [&](size_t b, int) { return N.pool.alloc(b); }, contrib_of[sl],
+                   work_of[sl]);
     }
     HIPCHK(hipMemcpyAsync(N.d_nodes + s0, N.nodes.data() + s0, nl * sizeof(NodeDev),
                           hipMemcpyHostToDevice, st));
@@ -1205,7 +1312,7 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
   o.maxsupernode = maxsn;
   o.num_levels = nlev;
   o.factor_ms = ms;
-  o.pool_peak_bytes = (int64_t)N.pool.peak();
+  o.pool_peak_bytes = (int64_t)(N.pool.peak() + N.plan.arena_bytes);
   o.static_plan = done ? 1 : 0;
   o.kernel_ms = 0.0;
   o.kernel_launches = 0;
