@@ -51,6 +51,12 @@ def parse():
                     help="instead of the default workload: the mixed-precision KKT system "
                          "(27-point H on this grid, grid^3 * 88000/512000 constraint rows with "
                          "4 entries) factored and solved in FP64 and mixed precision")
+    ap.add_argument("--kkt-refactor", type=int, default=0,
+                    help="instead of the default workload: refactors of a serving pattern "
+                         "(7-point H on this grid plus grid^3/8 constraint rows) whose H "
+                         "diagonal gets log-uniform [1e-8, 1e8] barrier terms, matching-based "
+                         "order from the unperturbed values and the matching scaling per "
+                         "refactor")
     ap.add_argument("--kkt-ordering", type=int, default=2,
                     help="--kkt27 / --batch ordering: 2 matching-based (with its scaling, "
                          "the reference's choice for saddle-point systems), 4 AMD")
@@ -199,6 +205,49 @@ def batch_bench(args):
         "loop_value": round(count / tl, 2), "loop_s": round(tl, 3),
         "speedup_vs_loop": round(tl / tb, 2), "max_bwd_err_raw": bwd,
         "inertia_ok": ok}), flush=True)
+
+
+def kkt_refactor_bench(args):
+    """The serving stream's refactor on its own: barrier-perturbed values on a
+    resident pattern, refactored on the warm handle (steps of fresh barrier
+    terms after warm-up ones).  One JSON line; `value` is refactors/s."""
+    from spral_amd import matrices, ssids
+    g = args.kkt_refactor
+    m = matrices.kkt_laplace(grid=g, m_con=g ** 3 // 8, seed=20261202 + 5 + g)
+    opts = ssids.SSIDSOptions(ordering=2, scaling=1)
+    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, val=m.val)
+    assert ainf.flag >= 0, ainf
+    col = np.repeat(np.arange(m.n), np.diff(m.ptr))
+    diag = np.nonzero((m.row - 1 == col) & (col < g ** 3))[0]
+    rng = np.random.Generator(np.random.PCG64(20261202 + 6))
+    fk, inf = ssids.ssids_factor(False, m.val, ak, opts)
+    assert inf.flag >= 0, inf
+    times, delays, failed = [], [], []
+    for step in range(args.warmup + args.steps):
+        v = m.val.copy()
+        v[diag] += 10.0 ** rng.uniform(-8, 8, diag.size)
+        t0 = time.perf_counter()
+        fk, inf = ssids.ssids_factor(False, v, ak, opts, fkeep=fk)
+        dt = time.perf_counter() - t0
+        assert inf.flag >= 0, inf
+        print(f"  refactor {step}: {dt * 1e3:.1f} ms, delays {inf.num_delay}, APP failed "
+              f"{inf.not_first_pass}", file=sys.stderr, flush=True)
+        if step >= args.warmup:
+            times.append(dt)
+            delays.append(int(inf.num_delay))
+            failed.append(int(inf.not_first_pass))
+    ssids.ssids_free(ak, fk)
+    mean = float(np.mean(times))
+    print(json.dumps({
+        "metric": "barrier-perturbed KKT refactors/s", "value": round(1.0 / mean, 3),
+        "unit": "refactors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mean * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"KKT: 7-point H on {g}^3 + {g ** 3 // 8} constraint rows, "
+                               "log-uniform [1e-8, 1e8] barrier terms on H's diagonal, "
+                               "matching-based order + matching scaling per refactor",
+                   "n": m.n, "num_flops": int(ak.num_flops), "parallelism": "single-gpu"},
+        "num_delay": delays, "not_first_pass": failed}), flush=True)
 
 
 def kkt27_bench(args):
@@ -361,6 +410,8 @@ def main():
         return serve_bench(args)
     if args.kkt27 > 0:
         return kkt27_bench(args)
+    if args.kkt_refactor > 0:
+        return kkt_refactor_bench(args)
     from spral_amd import _lib, matrices, ssids
     if not _lib.LIB_PATH.exists():
         from spral_amd import build

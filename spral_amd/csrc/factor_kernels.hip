@@ -1127,6 +1127,10 @@ struct TppShared {
   double red[NT / 64];
   int ired[NT / 64];
   int flag;
+  // one candidate per wave (batched scan): result code, partner, 2x2 D^-1
+  int code[NT / 64];
+  int tt[NT / 64];
+  double dd[3][NT / 64];
 };
 
 template <int NT>
@@ -1269,6 +1273,103 @@ __device__ void tpp_elim2(NodeDev &N, int k, int m, int n, double *ld, double *l
   __syncthreads();
 }
 
+// One TPP candidate p (> nelim) tested by one wave, reading the panel only
+// (the tests of a scan are read-only, so the waves of a workgroup test
+// consecutive candidates at once and the first that passes is taken, as a
+// one-at-a-time scan would).  The decisions are those of the scan's steps
+// (ldlt_tpp.cxx:192-259): a zero row/column; else t = the first largest
+// |a(p, c)|, c in [nelim, p), the 2x2 test on (t, p) against the largest
+// other entries of rows/columns t and p, else the 1x1 test on p.  Two passes:
+// row p and column p give the zero test, t, row p's largest entry other
+// than t (its second largest: t holds the first largest) and column p's
+// largest; row t and column t give t's largest other than p.  Every maximum
+// is exact, so the decision equals the one-at-a-time scan's.  Returns the
+// code (0 fail, 1 zero column, 2 2x2, 3 1x1), wave-uniform; t and the 2x2
+// D^-1 in the out arguments.
+__device__ inline int tpp_wave_test(NodeDev &N, int p, int nelim, int m, double u,
+                                    double small, int &t_out, double &d0, double &d1,
+                                    double &d3) {
+  const int lane = threadIdx.x & 63;
+  // pass A: row p over [nelim, p) and column p over [p, m)
+  int big = 0;                        // some |entry| not below small (NaN counts)
+  double m1 = -1.0, m2 = -1.0;        // row p: largest, second largest (-1: none)
+  int a1 = INT_MAX;                   // first column holding m1
+  for (int c = nelim + lane; c < p; c += 64) {
+    const double v = fabs(Lat(N, p, c));
+    if (!(v < small)) big = 1;
+    if (v > m1) {
+      m2 = m1;
+      m1 = v;
+      a1 = c;
+    } else if (v > m2) {
+      m2 = v;
+    }
+  }
+  double cp = 0.0;                    // column p below the diagonal
+  for (int r = p + lane; r < m; r += 64) {
+    const double v = fabs(Lat(N, r, p));
+    if (!(v < small)) big = 1;
+    if (r > p) cp = fmax(cp, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double om1 = __shfl_xor(m1, off), om2 = __shfl_xor(m2, off);
+    const int oa1 = __shfl_xor(a1, off);
+    if (om1 > m1 || (om1 == m1 && oa1 < a1)) {
+      m2 = fmax(om2, m1);
+      m1 = om1;
+      a1 = oa1;
+    } else {
+      m2 = fmax(m2, om1);
+    }
+    cp = fmax(cp, __shfl_xor(cp, off));
+  }
+  if (!__ballot(big)) return 1;
+  // t: the first largest of row p (nelim when every entry is NaN)
+  const int t = (a1 == INT_MAX) ? nelim : __builtin_amdgcn_readfirstlane(a1);
+  t_out = t;
+  // maxp = rc_max(p, skip t): row p without column t, column p below
+  double maxp = fmax(fmax(0.0, (a1 == INT_MAX) ? -1.0 : m2), cp);
+  // pass B: maxt = rc_max(t, skip p): row t over [nelim, t), column t over
+  // (t, m) without row p
+  double mt = 0.0;
+  for (int c = nelim + lane; c < t; c += 64) mt = fmax(mt, fabs(Lat(N, t, c)));
+  for (int r = t + 1 + lane; r < m; r += 64)
+    if (r != p) mt = fmax(mt, fabs(Lat(N, r, t)));
+  for (int off = 32; off > 0; off >>= 1) mt = fmax(mt, __shfl_xor(mt, off));
+  const double maxt = uni(mt);
+  maxp = uni(maxp);
+  // 2x2 test (t < p)
+  const double a11 = Lat(N, t, t), a21 = Lat(N, p, t), a22 = Lat(N, p, p);
+  const double maxpiv = fmax(fabs(a11), fmax(fabs(a21), fabs(a22)));
+  if (!(maxpiv < small)) {
+    const double detscale = 1 / maxpiv;
+    const double det0 = (a11 * detscale) * a22;
+    const double det1 = (a21 * detscale) * a21;
+    const double det = det0 - det1;
+    if (!(fabs(det) < fmax(small, fmax(fabs(det0 / 2), fabs(det1 / 2))))) {
+      const double e0 = (a22 * detscale) / det;
+      const double e1 = (-a21 * detscale) / det;
+      const double e3 = (a11 * detscale) / det;
+      bool ok2;
+      if (fmax(maxp, maxt) < small) {
+        ok2 = true;
+      } else {
+        const double x1 = fabs(e0) * maxt + fabs(e1) * maxp;
+        const double x2 = fabs(e1) * maxt + fabs(e3) * maxp;
+        ok2 = u * fmax(x1, x2) < 1.0;
+      }
+      if (ok2) {
+        d0 = e0;
+        d1 = e1;
+        d3 = e3;
+        return 2;
+      }
+    }
+  }
+  const double maxp2 = fmax(maxp, fabs(a21));
+  return (fabs(a22) >= u * maxp2) ? 3 : 0;
+}
+
 // TPP on the columns [nelim, n) that APP left, by the whole workgroup;
 // returns the new nelim (uniform) and stores it in N.nelim.
 template <int NT>
@@ -1292,63 +1393,51 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
       ++nelim;
       continue;
     }
-    int p;
     int done = 0;
-    for (p = nelim + 1; p < n; ++p) {
-      if (tpp_col_small(N, p, nelim, m, small, sh)) {
+    // candidates nelim+1, nelim+2, ... in batches of one per wave; the first
+    // that passes in order is taken
+    constexpr int NW = NT / 64;
+    const int wv = threadIdx.x >> 6;
+    for (int base = nelim + 1; base < n && !done; base += NW) {
+      const int pc = base + wv;
+      int code = 0, tw = 0;
+      double e0 = 0.0, e1 = 0.0, e3 = 0.0;
+      if (pc < n) code = tpp_wave_test(N, pc, nelim, m, u, small, tw, e0, e1, e3);
+      if ((tid & 63) == 0) {
+        sh.code[wv] = code;
+        sh.tt[wv] = tw;
+        sh.dd[0][wv] = e0;
+        sh.dd[1][wv] = e1;
+        sh.dd[2][wv] = e3;
+      }
+      __syncthreads();
+      int w0 = -1;
+      for (int v = 0; v < NW; ++v)
+        if (sh.code[v] != 0) {
+          w0 = v;
+          break;
+        }
+      int c = 0, t = 0;
+      double d0 = 0.0, d1 = 0.0, d3 = 0.0;
+      if (w0 >= 0) {
+        c = sh.code[w0];
+        t = sh.tt[w0];
+        d0 = sh.dd[0][w0];
+        d1 = sh.dd[1][w0];
+        d3 = sh.dd[2][w0];
+      }
+      __syncthreads();  // every thread has read the batch before it is reused
+      if (w0 < 0) continue;
+      const int p = base + w0;
+      done = 1;
+      if (c == 1) {  // a zero row/column: a zero pivot at nelim
         if (!fp.action) { if (tid == 0) N.flag = -5; return nelim; }
         tpp_swap<NT>(N, p, nelim, m);
         for (int r = nelim + tid; r < m; r += NT) Lat(N, r, nelim) = 0.0;
         if (tid == 0) { N.D[2 * nelim] = 0.0; N.D[2 * nelim + 1] = 0.0; N.nzero++; }
         __syncthreads();
         ++nelim;
-        done = 1;
-        break;
-      }
-      // t = argmax |a(p, nelim..p-1)| (first index on ties), by wave 0: each
-      // lane keeps the first maximum of its columns, and the merge prefers
-      // the larger value, then the smaller index
-      if (tid < 64) {
-        int best = INT_MAX;
-        double bv = -1.0;
-        for (int c = nelim + tid; c < p; c += 64) {
-          const double v = fabs(Lat(N, p, c));
-          if (v > bv) { bv = v; best = c; }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(bv, off);
-          const int oi = __shfl_xor(best, off);
-          if (ov > bv || (ov == bv && oi < best)) { bv = ov; best = oi; }
-        }
-        if (tid == 0) s_t = best;
-      }
-      __syncthreads();
-      const int t = s_t;
-      double maxt = tpp_rc_max(N, t, nelim, m, p, sh);
-      double maxp = tpp_rc_max(N, p, nelim, m, t, sh);
-      // 2x2 test (t < p)
-      double a11 = Lat(N, t, t), a21 = Lat(N, p, t), a22 = Lat(N, p, p);
-      double maxpiv = fmax(fabs(a11), fmax(fabs(a21), fabs(a22)));
-      bool ok2 = false;
-      double d0 = 0, d1 = 0, d3 = 0;
-      if (!(maxpiv < small)) {
-        double detscale = 1 / maxpiv;
-        double det0 = (a11 * detscale) * a22;
-        double det1 = (a21 * detscale) * a21;
-        double det = det0 - det1;
-        if (!(fabs(det) < fmax(small, fmax(fabs(det0 / 2), fabs(det1 / 2))))) {
-          d0 = (a22 * detscale) / det;
-          d1 = (-a21 * detscale) / det;
-          d3 = (a11 * detscale) / det;
-          if (fmax(maxp, maxt) < small) ok2 = true;
-          else {
-            double x1 = fabs(d0) * maxt + fabs(d1) * maxp;
-            double x2 = fabs(d1) * maxt + fabs(d3) * maxp;
-            ok2 = u * fmax(x1, x2) < 1.0;
-          }
-        }
-      }
-      if (ok2) {
+      } else if (c == 2) {
         tpp_swap<NT>(N, t, nelim, m);
         tpp_swap<NT>(N, p, nelim + 1, m);
         if (tid == 0) {
@@ -1360,11 +1449,7 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
         __syncthreads();
         tpp_elim2<NT>(N, nelim, m, n, ld, ld2);
         nelim += 2;
-        done = 1;
-        break;
-      }
-      maxp = fmax(maxp, fabs(Lat(N, p, t)));
-      if (fabs(Lat(N, p, p)) >= u * maxp) {
+      } else {
         tpp_swap<NT>(N, p, nelim, m);
         if (tid == 0) {
           N.D[2 * nelim] = 1.0 / Lat(N, nelim, nelim);
@@ -1373,12 +1458,10 @@ __device__ int tpp_front(NodeDev &N, const FactorParams &fp, int nelim, TppShare
         __syncthreads();
         tpp_elim1<NT>(N, nelim, m, n, ld);
         nelim += 1;
-        done = 1;
-        break;
       }
     }
     if (!done) {
-      p = nelim;
+      const int p = nelim;
       double maxp = tpp_rc_max(N, p, nelim, m, -1, sh);
       if (fabs(Lat(N, p, p)) >= u * maxp) {
         if (tid == 0) {
