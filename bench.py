@@ -222,7 +222,7 @@ def kkt_refactor_bench(args):
     rng = np.random.Generator(np.random.PCG64(20261202 + 6))
     fk, inf = ssids.ssids_factor(False, m.val, ak, opts)
     assert inf.flag >= 0, inf
-    times, delays, failed = [], [], []
+    times, delays, failed, static = [], [], [], []
     for step in range(args.warmup + args.steps):
         v = m.val.copy()
         v[diag] += 10.0 ** rng.uniform(-8, 8, diag.size)
@@ -231,11 +231,13 @@ def kkt_refactor_bench(args):
         dt = time.perf_counter() - t0
         assert inf.flag >= 0, inf
         print(f"  refactor {step}: {dt * 1e3:.1f} ms, delays {inf.num_delay}, APP failed "
-              f"{inf.not_first_pass}", file=sys.stderr, flush=True)
+              f"{inf.not_first_pass}, static plan {inf.static_plan}", file=sys.stderr,
+              flush=True)
         if step >= args.warmup:
             times.append(dt)
             delays.append(int(inf.num_delay))
             failed.append(int(inf.not_first_pass))
+            static.append(int(inf.static_plan))
     ssids.ssids_free(ak, fk)
     mean = float(np.mean(times))
     print(json.dumps({
@@ -247,7 +249,8 @@ def kkt_refactor_bench(args):
                                "log-uniform [1e-8, 1e8] barrier terms on H's diagonal, "
                                "matching-based order + matching scaling per refactor",
                    "n": m.n, "num_flops": int(ak.num_flops), "parallelism": "single-gpu"},
-        "num_delay": delays, "not_first_pass": failed}), flush=True)
+        "num_delay": delays, "not_first_pass": failed,
+        "static_plan": static}), flush=True)
 
 
 def kkt27_bench(args):

@@ -726,9 +726,17 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
       }
       const int seen = N.nd_seen[sl];
       int cap = seen + (int)std::ceil(slack * (D.sn + seen));
-      // no slack beyond the last APP step the seen delays already need: an
-      // extra step would cost every factorization a round of empty launches
-      if (!pivot_tpp) cap = std::min(cap, cdiv(D.sn + seen, kBlk) * kBlk - D.sn);
+      if (seen > 0) {
+        // a front that has received delays receives a varying number when
+        // the values change (refactors of one pattern): room for half as
+        // many again as the most seen, past the last APP step's padding (a
+        // factorization that exceeds the plan repeats itself level by level)
+        cap = std::max(cap, seen + (seen + 1) / 2);
+      } else if (!pivot_tpp) {
+        // no slack beyond the last APP step the front needs anyway: an extra
+        // step would cost every factorization a round of empty launches
+        cap = std::min(cap, cdiv(D.sn, kBlk) * kBlk - D.sn);
+      }
       D.ndcap = std::max(seen, std::min(cap, room));
       const int m = D.sm + D.ndcap, n = D.sn + D.ndcap;
       const int nsteps = pivot_tpp ? 0 : cdiv(n, kBlk);
@@ -1265,7 +1273,11 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     if (!N.plan.valid || N.plan.pivot_tpp != fp.pivot_tpp || N.plan.c32 != N.c32)
       plan_build(N, fp.pivot_tpp, opt.multiplier);
     done = factor_static(N, fp, opt, tev);
-    if (!done) {
+    if (done) {
+      // the delays of every factorization inform the next plan
+      for (int sl = 0; sl < nn; ++sl)
+        N.nd_seen[sl] = std::max(N.nd_seen[sl], N.nodes[sl].nd);
+    } else {
       for (auto e : tev) (void)hipEventDestroy(e);
       tev.clear();
       plan_release(N);  // rebuilt with the delays seen, on the next factorization

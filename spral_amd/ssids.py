@@ -106,6 +106,7 @@ class SSIDSInform:
     factor_ms: float = 0.0
     num_levels: int = 0
     pool_peak_bytes: int = 0  # peak device pool use of the factorization
+    static_plan: int = 0      # 1: the factorization ran on the static plan
 
 
 @dataclass
@@ -268,6 +269,7 @@ def _inform_from_stats(st, akeep):
     inf.factor_ms = st.factor_ms
     inf.num_levels = st.num_levels
     inf.pool_peak_bytes = st.pool_peak_bytes
+    inf.static_plan = st.static_plan
     return inf
 
 
