@@ -57,6 +57,8 @@ def parse():
                          "diagonal gets log-uniform [1e-8, 1e8] barrier terms, matching-based "
                          "order from the unperturbed values and the matching scaling per "
                          "refactor")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="--batch: skip the loop of single calls (loop_value null)")
     ap.add_argument("--kkt-ordering", type=int, default=2,
                     help="--kkt27 / --batch ordering: 2 matching-based (with its scaling, "
                          "the reference's choice for saddle-point systems), 4 AMD")
@@ -167,7 +169,7 @@ def batch_bench(args):
     ok = all(int(k) == int(0.2 * m.n) for k, m in zip(neg, mats))
     # the loop: a fresh handle per system, as a caller of ssids_factor would
     singles = []
-    for i, m in enumerate(mats):
+    for i, m in enumerate(mats if not args.no_loop else []):
         o = ssids.SSIDSOptions(ordering=0)
         ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, o, order=b.orders[i])
         singles.append((ak, o))
@@ -185,7 +187,7 @@ def batch_bench(args):
         if (i + 1) % 512 == 0:
             note(f"loop of single calls: {i + 1} of {count}")
     devmem.synchronize()
-    tl = time.perf_counter() - t0
+    tl = (time.perf_counter() - t0) if not args.no_loop else None
     for ak, _ in singles:
         ssids.ssids_free(ak)
     n_total, flops, fronts = int(b.offsets[-1]), int(b.akeep.num_flops), int(b.akeep.nnodes)
@@ -202,8 +204,9 @@ def batch_bench(args):
                    "n_total": n_total, "num_flops": flops, "fronts": fronts,
                    "parallelism": "single-gpu"},
         "factor_ms_device": round(finf.factor_ms, 3),
-        "loop_value": round(count / tl, 2), "loop_s": round(tl, 3),
-        "speedup_vs_loop": round(tl / tb, 2), "max_bwd_err_raw": bwd,
+        "loop_value": round(count / tl, 2) if tl else None,
+        "loop_s": round(tl, 3) if tl else None,
+        "speedup_vs_loop": round(tl / tb, 2) if tl else None, "max_bwd_err_raw": bwd,
         "inertia_ok": ok}), flush=True)
 
 

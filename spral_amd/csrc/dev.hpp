@@ -102,9 +102,11 @@ __host__ __device__ inline int idiv_up(long long a, long long b) { return (int)(
 //   S+1        failed-column swap
 //   S+2..2S+1  trailing panel update, step t-S-2
 //   2S+2       contribution update of a root's TPP pivots
-//   2S+3       gather, first tile column only
-//   2S+4       gather, the remaining tiles
-__host__ __device__ inline int level_tables(int S) { return 2 * S + 5; }
+//   2S+3       gather of the diagonal tile (0, 0) fused with step 0's pivot
+//              block: one workgroup per front
+//   2S+4       gather, the tiles outside the first tile column
+//   2S+5       gather, the first tile column's off-diagonal tiles
+__host__ __device__ inline int level_tables(int S) { return 2 * S + 6; }
 __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int nsteps,
                                                  int is_root) {
   if (t == 0) {
@@ -134,10 +136,13 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
     const int T = idiv_up(m - n, kBlk);
     return imax(1, imin(T * (T + 1) / 2, 4096));
   }
-  // gather split in two launches: the first 64-column tile column (what
-  // the first pivot step needs), then the remaining tiles
+  // gather split in three launches: the diagonal tile (with step 0's pivot
+  // block), the rest of the first tile column (what step 0's apply needs),
+  // then the remaining tiles
   const int T = idiv_up(m, kBlk);
-  return (t == 2 * S + 3) ? T : T * (T + 1) / 2 - T;
+  if (t == 2 * S + 3) return T > 0 ? 1 : 0;
+  if (t == 2 * S + 4) return T * (T + 1) / 2 - T;
+  return imax(T - 1, 0);
 }
 
 // Index li of the table segment holding workgroup b: off[li] <= b < off[li+1]

@@ -104,11 +104,12 @@ struct GatherArgs {  // the level's gather inputs (a pivot launch that assembles
   const double *scaling;
 };
 
-// FIRST: one workgroup per tile of the first tile column of every front (the
-// gather's part 1); the workgroup that built a front's diagonal tile (0, 0)
-// then runs step 0's pivot block on it, as soon as its block is assembled
-// and in a workgroup that already holds its CU, instead of queueing behind
-// the rest of the level's gather.
+// FIRST: one workgroup per front gathers the front's diagonal tile (0, 0)
+// (the gather's part 1) and runs step 0's pivot block on it, as soon as its
+// block is assembled and in a workgroup that already holds its CU, instead of
+// queueing behind the rest of the level's gather.  The first tile column's
+// other tiles (part 3) are gathered by k_gather, whose smaller LDS fits
+// twice as many workgroups on a CU.
 // C32 (FIRST only): the children's contribution blocks are FP32
 template <int NW, bool FIRST, bool C32>
 __global__ __launch_bounds__(64 * NW) void k_diag(NodeDev *nodes, const int *lvl_slots,
@@ -814,8 +815,10 @@ __device__ __forceinline__ int gather_tile(
   gcint *prows = (gcint *)P.rows;
   const int64_t *pacol = P.acol;
   const int T = (m + kGT - 1) / kGT;
-  const int local = blockIdx.x - wg_off[li] + (PART == 2 ? T : 0);
-  if (local >= (PART == 1 ? T : T * (T + 1) / 2)) return -1;
+  // PART 1: the diagonal tile (0, 0); 3: the first tile column's other
+  // tiles; 2: the tiles outside the first column; 0: every tile
+  const int local = blockIdx.x - wg_off[li] + (PART == 2 ? T : PART == 3 ? 1 : 0);
+  if (local >= (PART == 1 ? 1 : PART == 3 ? T : T * (T + 1) / 2)) return -1;
   int I, J;
   tile_of(local, T, T, I, J);
   const int r0 = I * kGT, c0 = J * kGT;
@@ -1752,10 +1755,10 @@ void launch_gather(hipStream_t st, int part, int nwg, NodeDev *nodes, const int 
                    const int64_t *asrc, const int *arow, const int *acix,
                    const double *scaling, int c32) {
   if (nwg <= 0) return;
-  auto kern = c32 ? ((part == 1)   ? k_gather<1, float>
+  auto kern = c32 ? ((part == 3)   ? k_gather<3, float>
                      : (part == 2) ? k_gather<2, float>
                                    : k_gather<0, float>)
-                  : ((part == 1)   ? k_gather<1, double>
+                  : ((part == 3)   ? k_gather<3, double>
                      : (part == 2) ? k_gather<2, double>
                                    : k_gather<0, double>);
   hipLaunchKernelGGL(kern, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off, nlvl,

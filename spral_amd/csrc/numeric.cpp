@@ -84,6 +84,31 @@ struct HipError : std::runtime_error {
     if (e_ != hipSuccess) throw HipError(e_); \
   } while (0)
 
+// Copies the library makes are ordered on a stream and have completed on
+// return.  None goes through the legacy null stream: a copy there waits for
+// every blocking stream of the process, so a solve on one handle would wait
+// for a refactor running on another (a service runs both at once).
+static void copy_now(void *dst, const void *src, size_t bytes, hipMemcpyKind kind,
+                     hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+static void copy2d_now(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                       size_t height, hipMemcpyKind kind, hipStream_t st) {
+  HIPCHK(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, kind, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+// a non-blocking stream of this host thread, for copies outside any handle
+// (kept until the process ends)
+static hipStream_t thread_stream() {
+  thread_local std::map<int, hipStream_t> streams;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipStream_t &st = streams[dev];
+  if (!st) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return st;
+}
+
 // ---------------------------------------------------------------------------
 // Device memory pool: large chunks, best-fit free blocks (indexed by size, so
 // an allocation is O(log blocks) however many fronts a batch places) with
@@ -172,7 +197,7 @@ T *dev_upload(const std::vector<T> &v, std::vector<void *> &owned) {
   if (v.empty()) return nullptr;
   void *p;
   HIPCHK(hipMalloc(&p, v.size() * sizeof(T)));
-  HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  copy_now(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, thread_stream());
   owned.push_back(p);
   return (T *)p;
 }
@@ -273,7 +298,7 @@ struct Numeric {
   // thread (a synchronous copy) would otherwise depend on the capturing
   // stream, which invalidates the capture
   hipStream_t stream_cap = nullptr;
-  hipEvent_t ev_level = nullptr, ev_rest = nullptr;
+  hipEvent_t ev_level = nullptr, ev_rest = nullptr, ev_first = nullptr;
   // the static-plan factorization captured as a graph, replayed by later
   // refactors with the same plan, parameters and buffers
   hipGraph_t graph = nullptr;
@@ -347,6 +372,7 @@ struct Numeric {
     if (graph) (void)hipGraphDestroy(graph);
     if (ev_level) (void)hipEventDestroy(ev_level);
     if (ev_rest) (void)hipEventDestroy(ev_rest);
+    if (ev_first) (void)hipEventDestroy(ev_first);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream_cap) (void)hipStreamDestroy(stream_cap);
     if (stream) (void)hipStreamDestroy(stream);
@@ -786,7 +812,7 @@ static void plan_build(Numeric &N, int pivot_tpp, double multiplier) {
     }
   }
   if (!P.d_tmpl) HIPCHK(hipMalloc(&P.d_tmpl, std::max(nn, 1) * sizeof(NodeDev)));
-  HIPCHK(hipMemcpy(P.d_tmpl, P.tmpl.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice));
+  copy_now(P.d_tmpl, P.tmpl.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice, N.stream);
   if (tab_total > P.tabs_cap) {
     if (P.d_tabs) HIPCHK(hipFree(P.d_tabs));
     HIPCHK(hipMalloc(&P.d_tabs, std::max<size_t>(tab_total, 1) * sizeof(int)));
@@ -820,6 +846,7 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
   if (!N.stream2) HIPCHK(hipStreamCreateWithFlags(&N.stream2, hipStreamNonBlocking));
   if (!N.ev_level) HIPCHK(hipEventCreateWithFlags(&N.ev_level, hipEventDisableTiming));
   if (!N.ev_rest) HIPCHK(hipEventCreateWithFlags(&N.ev_rest, hipEventDisableTiming));
+  if (!N.ev_first) HIPCHK(hipEventCreateWithFlags(&N.ev_first, hipEventDisableTiming));
   hipStream_t st2 = N.stream2;
   // the whole launch sequence (no host round trip, no memset: the counters
   // are cleared before it), captured once as a graph when not timing
@@ -837,7 +864,11 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     launch_maptab(st, nl, N.d_nodes, d_slots, S.d_child);
     HIPCHK(hipEventRecord(N.ev_level, st));
     HIPCHK(hipStreamWaitEvent(st2, N.ev_level, 0));
-    // the first tile column, and step 0's pivot blocks in its diagonal tiles
+    // the first tile column's other tiles on st2, ahead of the rest
+    launch_gather(st2, 3, lv.cap[2 * ms + 5], N.d_nodes, d_slots, toff(2 * ms + 5), nl,
+                  S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp.c32);
+    HIPCHK(hipEventRecord(N.ev_first, st2));
+    // the diagonal tiles, each with step 0's pivot block
     launch_gather_first(st, lv.cap[2 * ms + 3], N.d_nodes, d_slots, toff(2 * ms + 3), nl,
                         S.d_child, N.d_aval, S.d_asrc, S.d_arow, S.d_acix, N.d_scal, fp);
     if (opt.timing) tev.emplace_back(timed_event(st2));
@@ -852,6 +883,8 @@ static bool factor_static(Numeric &N, const FactorParams &fp, const spral_ssids_
     };
     for (int stp = 0; stp < ms; ++stp) {
       if (stp > 0) launch_diag(st, nl, N.d_nodes, d_slots, stp, fp);
+      // step 0's apply reads the whole first tile column
+      if (stp == 0) HIPCHK(hipStreamWaitEvent(st, N.ev_first, 0));
       launch_apply(st, lv.cap[1 + stp], N.d_nodes, d_slots, toff(1 + stp), nl, stp, fp);
       // one step (every n <= 64): the update stays inside the first tile
       // column too; with more steps it reaches the later columns
@@ -1114,8 +1147,8 @@ static void build_solve_tables(Numeric &N) {
     std::vector<int> iota(std::max(nn, 1));
     for (int i = 0; i < nn; ++i) iota[i] = i;
     HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
+    copy_now(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
+                     hipMemcpyHostToDevice, N.stream);
   }
   // solve row-workgroup tables (depend on nelim): every front gets at least
   // one, which in the backward solve also reduces and solves the triangle
@@ -1160,8 +1193,8 @@ static void build_solve_tables(Numeric &N) {
     if (N.d_gemv_tab) HIPCHK(hipFree(N.d_gemv_tab));
     N.d_gemv_tab = nullptr;
     HIPCHK(hipMalloc(&N.d_gemv_tab, std::max<size_t>(tab.size(), 1) * sizeof(int)));
-    HIPCHK(hipMemcpy(N.d_gemv_tab, tab.data(), tab.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
+    copy_now(N.d_gemv_tab, tab.data(), tab.size() * sizeof(int),
+                     hipMemcpyHostToDevice, N.stream);
   }
   // per front, per right-hand side: X (m) and the backward partial sums
   // (row workgroups x nelim, at least nelim: big fronts keep their
@@ -1175,7 +1208,7 @@ static void build_solve_tables(Numeric &N) {
     }
     N.solve_ws_per_rhs = std::max<int64_t>(off, 1);
   }
-  HIPCHK(hipMemcpy(N.d_sn, N.sn_h.data(), nn * sizeof(SolveNode), hipMemcpyHostToDevice));
+  copy_now(N.d_sn, N.sn_h.data(), nn * sizeof(SolveNode), hipMemcpyHostToDevice, N.stream);
   N.big_of_level.assign(S.lvl_ptr.size() - 1, {});
   for (int L = 0; L + 1 < (int)S.lvl_ptr.size(); ++L)
     for (int sl = S.lvl_ptr[L]; sl < S.lvl_ptr[L + 1]; ++sl)
@@ -1209,8 +1242,8 @@ static void build_solve_tables(Numeric &N) {
     N.d_bigtab = nullptr;
     if (!tab.empty()) {
       HIPCHK(hipMalloc(&N.d_bigtab, tab.size() * sizeof(int)));
-      HIPCHK(hipMemcpy(N.d_bigtab, tab.data(), tab.size() * sizeof(int),
-                       hipMemcpyHostToDevice));
+      copy_now(N.d_bigtab, tab.data(), tab.size() * sizeof(int),
+                       hipMemcpyHostToDevice, N.stream);
     }
   }
 }
@@ -1259,8 +1292,8 @@ static void factor_impl(Numeric &N, const double *d_aval, const double *d_scalin
     std::vector<int> iota(std::max(nn, 1));
     for (int i = 0; i < nn; ++i) iota[i] = i;
     HIPCHK(hipMalloc(&N.d_slots_all, iota.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
+    copy_now(N.d_slots_all, iota.data(), iota.size() * sizeof(int),
+                     hipMemcpyHostToDevice, N.stream);
   }
   HIPCHK(hipMemsetAsync(N.d_counters, 0, 16 * sizeof(int), st));
   std::vector<hipEvent_t> tev;  // pairs around the gather launches
@@ -1664,11 +1697,11 @@ static int save_numeric(const Numeric &N, const char *path) {
   w.put1(N.stats);
   std::vector<double> hv(std::max<int64_t>(naval, S.n));
   if (naval > 0) {
-    HIPCHK(hipMemcpy(hv.data(), N.d_aval, naval * sizeof(double), hipMemcpyDeviceToHost));
+    copy_now(hv.data(), N.d_aval, naval * sizeof(double), hipMemcpyDeviceToHost, N.stream);
     w.put(hv.data(), naval);
   }
   if (N.d_scal) {
-    HIPCHK(hipMemcpy(hv.data(), N.d_scal, S.n * sizeof(double), hipMemcpyDeviceToHost));
+    copy_now(hv.data(), N.d_scal, S.n * sizeof(double), hipMemcpyDeviceToHost, N.stream);
     w.put(hv.data(), S.n);
   }
   std::vector<double> dbuf;
@@ -1681,19 +1714,19 @@ static int save_numeric(const Numeric &N, const char *path) {
     w.put(st, 10);
     dbuf.resize((size_t)D.ldl * D.n + 2 * (size_t)D.n);
     if (D.n > 0) {
-      HIPCHK(hipMemcpy(dbuf.data(), D.L, (size_t)D.ldl * D.n * sizeof(double),
-                       hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(dbuf.data() + (size_t)D.ldl * D.n, D.D, 2 * (size_t)D.n * sizeof(double),
-                       hipMemcpyDeviceToHost));
+      copy_now(dbuf.data(), D.L, (size_t)D.ldl * D.n * sizeof(double),
+                       hipMemcpyDeviceToHost, N.stream);
+      copy_now(dbuf.data() + (size_t)D.ldl * D.n, D.D, 2 * (size_t)D.n * sizeof(double),
+                       hipMemcpyDeviceToHost, N.stream);
       w.put(dbuf.data(), dbuf.size());
       ibuf.resize(2 * (size_t)D.n);
-      HIPCHK(hipMemcpy(ibuf.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(ibuf.data() + D.n, D.ipos, D.n * sizeof(int), hipMemcpyDeviceToHost));
+      copy_now(ibuf.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost, N.stream);
+      copy_now(ibuf.data() + D.n, D.ipos, D.n * sizeof(int), hipMemcpyDeviceToHost, N.stream);
       w.put(ibuf.data(), ibuf.size());
     }
     const size_t tn = ctab_ints(S, sl, D.m);
     ibuf.resize(tn);
-    HIPCHK(hipMemcpy(ibuf.data(), D.ctab, tn * sizeof(int), hipMemcpyDeviceToHost));
+    copy_now(ibuf.data(), D.ctab, tn * sizeof(int), hipMemcpyDeviceToHost, N.stream);
     w.put(ibuf.data(), tn);
     const int cm = D.m - D.n;
     const size_t cb = (D.is_root && cm > 0 && D.C)
@@ -1702,7 +1735,7 @@ static int save_numeric(const Numeric &N, const char *path) {
     w.put1((int64_t)cb);
     if (cb) {
       cbuf.resize(cb);
-      HIPCHK(hipMemcpy(cbuf.data(), D.C, cb, hipMemcpyDeviceToHost));
+      copy_now(cbuf.data(), D.C, cb, hipMemcpyDeviceToHost, N.stream);
       w.put(cbuf.data(), cb);
     }
   }
@@ -1749,12 +1782,12 @@ static Numeric *load_numeric(Symbolic *Sp, const char *path, int *flag) {
     N->d_aval_cap = naval;
     if (naval > 0) {
       r.get(hv.data(), naval);
-      HIPCHK(hipMemcpy(N->d_aval, hv.data(), naval * sizeof(double), hipMemcpyHostToDevice));
+      copy_now(N->d_aval, hv.data(), naval * sizeof(double), hipMemcpyHostToDevice, N->stream);
     }
     if (hdr[4]) {
       r.get(hv.data(), S.n);
       HIPCHK(hipMalloc(&N->d_scal, std::max(S.n, 1) * sizeof(double)));
-      HIPCHK(hipMemcpy(N->d_scal, hv.data(), S.n * sizeof(double), hipMemcpyHostToDevice));
+      copy_now(N->d_scal, hv.data(), S.n * sizeof(double), hipMemcpyHostToDevice, N->stream);
     }
     const int nn = S.nnodes;
     N->nodes.assign(nn, NodeDev{});
@@ -1798,19 +1831,19 @@ static Numeric *load_numeric(Symbolic *Sp, const char *path, int *flag) {
         ibuf.resize(2 * (size_t)D.n);
         r.get(ibuf.data(), ibuf.size());
         if (!r.ok) break;
-        HIPCHK(hipMemcpy(D.L, dbuf.data(), (size_t)D.ldl * D.n * sizeof(double),
-                         hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(D.D, dbuf.data() + (size_t)D.ldl * D.n,
-                         2 * (size_t)D.n * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(D.perm, ibuf.data(), D.n * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(D.ipos, ibuf.data() + D.n, D.n * sizeof(int), hipMemcpyHostToDevice));
+        copy_now(D.L, dbuf.data(), (size_t)D.ldl * D.n * sizeof(double),
+                         hipMemcpyHostToDevice, N->stream);
+        copy_now(D.D, dbuf.data() + (size_t)D.ldl * D.n,
+                         2 * (size_t)D.n * sizeof(double), hipMemcpyHostToDevice, N->stream);
+        copy_now(D.perm, ibuf.data(), D.n * sizeof(int), hipMemcpyHostToDevice, N->stream);
+        copy_now(D.ipos, ibuf.data() + D.n, D.n * sizeof(int), hipMemcpyHostToDevice, N->stream);
       }
       const size_t tn = ctab_ints(S, sl, D.m);
       D.ctab = (int *)keep(tn * sizeof(int));
       ibuf.resize(tn);
       r.get(ibuf.data(), tn);
       if (!r.ok) break;
-      HIPCHK(hipMemcpy(D.ctab, ibuf.data(), tn * sizeof(int), hipMemcpyHostToDevice));
+      copy_now(D.ctab, ibuf.data(), tn * sizeof(int), hipMemcpyHostToDevice, N->stream);
       const int64_t cb = r.get1<int64_t>();
       const int cm = D.m - D.n;
       const size_t cbmax =
@@ -1824,7 +1857,7 @@ static Numeric *load_numeric(Symbolic *Sp, const char *path, int *flag) {
         cbuf.resize(cb);
         r.get(cbuf.data(), cb);
         if (!r.ok) break;
-        HIPCHK(hipMemcpy(D.C, cbuf.data(), cb, hipMemcpyHostToDevice));
+        copy_now(D.C, cbuf.data(), cb, hipMemcpyHostToDevice, N->stream);
       }
     }
     std::fclose(f);
@@ -1835,7 +1868,7 @@ static Numeric *load_numeric(Symbolic *Sp, const char *path, int *flag) {
       return nullptr;
     }
     if (!N->d_nodes) HIPCHK(hipMalloc(&N->d_nodes, std::max(nn, 1) * sizeof(NodeDev)));
-    HIPCHK(hipMemcpy(N->d_nodes, N->nodes.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice));
+    copy_now(N->d_nodes, N->nodes.data(), nn * sizeof(NodeDev), hipMemcpyHostToDevice, N->stream);
     if (!N->d_counters) HIPCHK(hipMalloc(&N->d_counters, 16 * sizeof(int)));
     if (!N->d_scratch_n) HIPCHK(hipMalloc(&N->d_scratch_n, std::max(S.n, 1) * sizeof(int)));
     build_solve_tables(*N);
@@ -1945,10 +1978,10 @@ void *spral_ssids_hip_factor(bool posdef, void *symbolic, const double *aval, in
   try {
     HIPCHK(hipSetDevice(S->device));
     HIPCHK(hipMalloc(&d_a, std::max<int64_t>(naval, 1) * sizeof(double)));
-    HIPCHK(hipMemcpy(d_a, aval, naval * sizeof(double), hipMemcpyHostToDevice));
+    copy_now(d_a, aval, naval * sizeof(double), hipMemcpyHostToDevice, thread_stream());
     if (scaling) {
       HIPCHK(hipMalloc(&d_s, std::max(S->n, 1) * sizeof(double)));
-      HIPCHK(hipMemcpy(d_s, scaling, S->n * sizeof(double), hipMemcpyHostToDevice));
+      copy_now(d_s, scaling, S->n * sizeof(double), hipMemcpyHostToDevice, thread_stream());
     }
     res = spral_ssids_hip_factor_device(posdef, symbolic, d_a, naval, d_s, options, stats);
   } catch (HipError &e) {
@@ -2084,8 +2117,8 @@ int spral_ssids_hip_solve_refine(void *numeric, int nrhs, const double *b, doubl
     HIPCHK(hipSetDevice(N->symb->device));
     HIPCHK(hipMalloc(&db, (size_t)n * nrhs * sizeof(double)));
     HIPCHK(hipMalloc(&dx, (size_t)n * nrhs * sizeof(double)));
-    HIPCHK(hipMemcpy2D(db, n * sizeof(double), b, ldx * sizeof(double), n * sizeof(double),
-                       nrhs, hipMemcpyHostToDevice));
+    copy2d_now(db, n * sizeof(double), b, ldx * sizeof(double), n * sizeof(double),
+                       nrhs, hipMemcpyHostToDevice, N->stream);
     rc = refine_device(*N, nrhs, db, dx, n, maxit, tol, bwd, iters, N->stream);
     HIPCHK(hipMemcpy2DAsync(x, ldx * sizeof(double), dx, n * sizeof(double),
                             n * sizeof(double), nrhs, hipMemcpyDeviceToHost, N->stream));
@@ -2113,7 +2146,7 @@ int spral_ssids_hip_enquire_posdef(const void *numeric, double *d) {
       const int ne = D.nelim;
       if (ne == 0) continue;
       std::vector<double> dh(2 * (size_t)ne);
-      HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
+      copy_now(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost, N->stream);
       for (int k = 0; k < ne; ++k) *dp++ = 1.0 / std::sqrt(dh[2 * k]);
     }
     return 0;
@@ -2155,8 +2188,8 @@ int spral_ssids_hip_enquire_indef(const void *numeric, int *piv_order, double *d
         HIPCHK(hipMalloc(&d_off, (nn + 1) * sizeof(int64_t)));
         HIPCHK(hipMalloc(&d_d, 2 * tot * sizeof(double)));
         HIPCHK(hipMalloc(&d_p, tot * sizeof(int)));
-        HIPCHK(hipMemcpy(d_slot, slot_of.data(), nn * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_off, off.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        copy_now(d_slot, slot_of.data(), nn * sizeof(int), hipMemcpyHostToDevice, N->stream);
+        copy_now(d_off, off.data(), (nn + 1) * sizeof(int64_t), hipMemcpyHostToDevice, N->stream);
         launch_pack_dperm(N->stream, N->d_nodes, d_slot, d_off, nn, d_d, d_p);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(dall.data(), d_d, 2 * tot * sizeof(double),
@@ -2223,7 +2256,7 @@ int spral_ssids_hip_alter(void *numeric, const double *d) {
       int ne = D.nelim;
       if (ne == 0) continue;
       std::vector<double> dh(2 * (size_t)ne);
-      HIPCHK(hipMemcpy(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost));
+      copy_now(dh.data(), D.D, 2 * ne * sizeof(double), hipMemcpyDeviceToHost, N->stream);
       for (int k = 0; k < ne;) {
         if (k + 1 == ne || std::isfinite(dh[2 * k + 2])) {
           dh[2 * k] = dp[0];
@@ -2238,7 +2271,7 @@ int spral_ssids_hip_alter(void *numeric, const double *d) {
           k += 2;
         }
       }
-      HIPCHK(hipMemcpy(D.D, dh.data(), 2 * ne * sizeof(double), hipMemcpyHostToDevice));
+      copy_now(D.D, dh.data(), 2 * ne * sizeof(double), hipMemcpyHostToDevice, N->stream);
     }
     return 0;
   } catch (HipError &) {
@@ -2274,7 +2307,7 @@ int spral_ssids_hip_numeric_scaling(const void *numeric, double *s) {
   if (!N->d_scal) return 0;
   try {
     HIPCHK(hipSetDevice(N->symb->device));
-    HIPCHK(hipMemcpy(s, N->d_scal, N->symb->n * sizeof(double), hipMemcpyDeviceToHost));
+    copy_now(s, N->d_scal, N->symb->n * sizeof(double), hipMemcpyDeviceToHost, N->stream);
   } catch (HipError &) {
     return -51;
   }
@@ -2306,13 +2339,13 @@ int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *
     if (cm > 0) {
       if (N->c32) {  // mixed precision: the block is held in FP32
         std::vector<float> f((size_t)cm * cm);
-        HIPCHK(hipMemcpy2D(f.data(), cm * sizeof(float), D.C, D.ldc * sizeof(float),
-                           cm * sizeof(float), cm, hipMemcpyDeviceToHost));
+        copy2d_now(f.data(), cm * sizeof(float), D.C, D.ldc * sizeof(float),
+                           cm * sizeof(float), cm, hipMemcpyDeviceToHost, N->stream);
         for (size_t k = 0; k < f.size(); ++k) N->contrib_val[k] = f[k];
       } else {
-        HIPCHK(hipMemcpy2D(N->contrib_val.data(), cm * sizeof(double), D.C,
+        copy2d_now(N->contrib_val.data(), cm * sizeof(double), D.C,
                            D.ldc * sizeof(double), cm * sizeof(double), cm,
-                           hipMemcpyDeviceToHost));
+                           hipMemcpyDeviceToHost, N->stream);
       }
       // symmetrise the lower triangle into the full square the reference passes
       for (int j = 0; j < cm; ++j)
@@ -2328,11 +2361,11 @@ int spral_ssids_hip_get_contrib(void *numeric, int *n, const double **val, int *
     N->contrib_delay_val.assign((size_t)nf * D.m, 0.0);
     if (nf > 0) {
       std::vector<int> ph(D.n);
-      HIPCHK(hipMemcpy(ph.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost));
+      copy_now(ph.data(), D.perm, D.n * sizeof(int), hipMemcpyDeviceToHost, N->stream);
       for (int k = 0; k < nf; ++k) N->contrib_delay_perm[k] = ph[D.nelim + k] + 1;
-      HIPCHK(hipMemcpy2D(N->contrib_delay_val.data(), D.m * sizeof(double),
+      copy2d_now(N->contrib_delay_val.data(), D.m * sizeof(double),
                          D.L + (size_t)D.nelim * D.ldl, D.ldl * sizeof(double),
-                         D.m * sizeof(double), nf, hipMemcpyDeviceToHost));
+                         D.m * sizeof(double), nf, hipMemcpyDeviceToHost, N->stream);
     }
     *n = cm;
     *val = N->contrib_val.data();
@@ -2367,11 +2400,11 @@ int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, doubl
   info[5] = Dn.sn;
   try {
     if (L && Dn.n > 0)
-      HIPCHK(hipMemcpy(L, Dn.L, (size_t)Dn.ldl * Dn.n * sizeof(double), hipMemcpyDeviceToHost));
+      copy_now(L, Dn.L, (size_t)Dn.ldl * Dn.n * sizeof(double), hipMemcpyDeviceToHost, N->stream);
     if (D && Dn.n > 0)
-      HIPCHK(hipMemcpy(D, Dn.D, 2 * (size_t)Dn.n * sizeof(double), hipMemcpyDeviceToHost));
+      copy_now(D, Dn.D, 2 * (size_t)Dn.n * sizeof(double), hipMemcpyDeviceToHost, N->stream);
     if (perm && Dn.n > 0)
-      HIPCHK(hipMemcpy(perm, Dn.perm, Dn.n * sizeof(int), hipMemcpyDeviceToHost));
+      copy_now(perm, Dn.perm, Dn.n * sizeof(int), hipMemcpyDeviceToHost, N->stream);
   } catch (HipError &) {
     return -51;
   }
