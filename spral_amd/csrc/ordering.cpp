@@ -7,7 +7,7 @@
 //  * order_grid_nd: geometric nested dissection of an nx*ny*nz grid whose
 //    variable (i,j,k) is i + nx*(j + ny*k) (0-based), separators are middle
 //    planes of the longest box dimension, ordered after both halves, each
-//    with the vertices next to an enclosing separator first;
+//    with the vertices next to an enclosing separator last;
 //  * order_min_degree: exact minimum degree on the explicit elimination graph
 //    (ties to the lowest index), for moderate n;
 //  * order_amd: approximate minimum degree on the quotient graph, for large
@@ -28,10 +28,14 @@ struct Box {
 };
 
 // Separator vertices that touch the box's outside (an ancestor separator,
-// never the domain boundary) come first, the rest after.  Each such vertex
-// brings one ancestor row of its own into the fill; eliminating them first
-// lets every later separator column share one structure, so the rest of the
-// separator is a single supernode instead of one per grid line.
+// never the domain boundary) come last, after the rest of the separator.
+// Each such vertex brings ancestor rows of its own into the fill; with them
+// last, the separator's interior shares one structure and is a single
+// supernode (instead of one per grid line, as in plain row-major order), and
+// the boundary vertices follow as small fronts of their own.  Ordered first
+// instead, they fill into the enclosing separators and amalgamate into the
+// root: on 64^3 a 6,112-column root against 4,128, and 241 pivot steps in
+// sequence over the tree levels against 215.
 void nd_box(const Box& b, const int dims[3], std::vector<int>& seq) {
   const int nx = dims[0], ny = dims[1];
   int ext[3] = {b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]};
@@ -66,7 +70,7 @@ void nd_box(const Box& b, const int dims[3], std::vector<int>& seq) {
       for (int j = sep.lo[1]; j < sep.hi[1]; ++j)
         for (int i = sep.lo[0]; i < sep.hi[0]; ++i) {
           const int p[3] = {i, j, k};
-          if (touches_outside(p) == (pass == 0)) seq.push_back(i + nx * (j + ny * k));
+          if (touches_outside(p) == (pass == 1)) seq.push_back(i + nx * (j + ny * k));
         }
 }
 
