@@ -272,6 +272,13 @@ int spral_ssids_hip_node_factors(const void *numeric, int node, int *info, doubl
  * the children.  Returns 0. */
 int spral_ssids_hip_node_pivots(const void *numeric, int *nelim_app, int *ndelay_in);
 
+/* Planning helper (no reference counterpart; the factorization runs the
+ * fronts of one tree level per launch sequence): the level of each of nnodes
+ * fronts (0-based postorder; parent[i] > i, or -1 for a root; ncols = the
+ * front's fully-summed columns), the assignment the analyse uses.  Returns
+ * the number of levels, or -1 when parent is not a postorder forest. */
+int spral_ssids_hip_tree_levels(int nnodes, const int *parent, const int *ncols, int *level);
+
 /* Number of visible HIP devices (replaces the cudaGetDeviceCount probe of
  * src/hw_topology/guess_topology.cxx:65). */
 int spral_ssids_hip_device_count(void);

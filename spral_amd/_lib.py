@@ -123,6 +123,7 @@ SIGNATURES = {
     "spral_ssids_hip_free_contrib": (None, [C.c_void_p]),
     "spral_ssids_hip_device_count": (C.c_int, []),
     "spral_ssids_hip_node_pivots": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "spral_ssids_hip_tree_levels": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
     "spral_rb_peek": (C.c_int, [C.c_char_p] + [C.c_void_p] * 9),

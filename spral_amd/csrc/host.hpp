@@ -28,6 +28,10 @@ void build_map(int n, const int64_t* ptr, const int* row, const int* order,
 void expand_pattern(int n, const int64_t* ptr, const int* row,
                     std::vector<int64_t>& aptr, std::vector<int>& arow);
 
+// Levels of the assembly tree for the numeric factorization (parent[i] > i
+// or -1; ncols = fully-summed columns per front): level[child] < level[parent].
+std::vector<int> tree_levels(const std::vector<int>& parent, const std::vector<int>& ncols);
+
 // Orderings (1-based order(i) = pivot position of variable i).
 void order_grid_nd(int nx, int ny, int nz, int* order);
 int order_min_degree(int n, const int64_t* ptr, const int* row, int* order);

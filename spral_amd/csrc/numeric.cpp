@@ -24,6 +24,7 @@
 #include "../../include/spral_ssids_hip.h"
 #include "api_lock.hpp"
 #include "dev.hpp"
+#include "host.hpp"
 
 namespace ssids_hip {
 
@@ -460,13 +461,10 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   }
   S->acol_off[nn] = (int64_t)acol.size();
   S->naval_min = amax;
-  // levels by height (leaves = 0)
-  std::vector<int> lev(nn, 0);
+  // tree levels (children precede parents: postorder)
+  const std::vector<int> lev = tree_levels(S->parent, S->sn);
   int nlev = 0;
-  for (int i = 0; i < nn; ++i) {  // children precede parents (postorder)
-    for (int c : S->children[i]) lev[i] = std::max(lev[i], lev[c] + 1);
-    nlev = std::max(nlev, lev[i] + 1);
-  }
+  for (int i = 0; i < nn; ++i) nlev = std::max(nlev, lev[i] + 1);
   S->lvl_ptr.assign(nlev + 1, 0);
   for (int i = 0; i < nn; ++i) S->lvl_ptr[lev[i] + 1]++;
   for (int L = 0; L < nlev; ++L) S->lvl_ptr[L + 1] += S->lvl_ptr[L];
