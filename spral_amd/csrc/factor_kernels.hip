@@ -742,6 +742,151 @@ __global__ __launch_bounds__(kThreads, 4) void k_update(NodeDev *nodes, const in
   }
 }
 
+// ---------------------------------------------------------------------------
+// The APP step's trailing panel update (k_update mode 0) with the MFMA
+// operands loaded straight into the 16x16x4 fragment layout: lane i + 16 kq
+// holds row i of A (= L, pivot kq of the 4-pivot step) and column i of
+// B = L D for the same pivot; two register buffers of two steps each
+// alternate, so the loads of the next 8 pivots are in flight during the
+// MFMAs.  The per-pivot coefficients of L D come from pivot_d once per
+// workgroup.  Each element gets the same products in the same order as in
+// k_update.  Three workgroups per SIMD fit (162 VGPRs): loading the tile's
+// current values before the products as well needs 198 registers and two
+// workgroups, and measured slower (64^3 factor 47.4 against 46.7 ms, the
+// LDS-staged k_update 48.1 ms).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads, 3) void k_update_panel(NodeDev *nodes,
+                                                           const int *lvl_slots,
+                                                           const int *wg_off, int nlvl,
+                                                           int step, FactorParams fp) {
+  __shared__ double ce1[kBlk], ce2[kBlk];
+  __shared__ int ctwo[kBlk];
+  const int li = find_node(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
+  NodeDev &N = nodes[lvl_slots[li]];
+  const int local = blockIdx.x - wg_off[li], count = wg_off[li + 1] - wg_off[li];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (step >= N.nsteps) return;
+  const int p = N.p, w = N.w;
+  if (w == 0) return;
+  const int np = fp.posdef ? w : block_npass(N, fp.u);
+  if (local == 0 && tid == 0) N.npass = np;
+  const int a0 = p, K = np, t0 = p + np, cend = N.n, fa = p + np, fb = p + w;
+  const int m = N.m;
+  if (m - t0 <= 0 || cend - t0 <= 0) return;
+  const int TI = (m - t0 + kTile - 1) / kTile, TJ = (cend - t0 + kTile - 1) / kTile;
+  const int ntiles = TJ * TI - TJ * (TJ - 1) / 2;
+  {
+    const double *D = N.D;
+    for (int t = tid; t < K; t += kThreads) {
+      double e1, e2;
+      int two;
+      pivot_d(D, a0 + t, a0 + K, e1, e2, two);
+      ce1[t] = e1;
+      ce2[t] = e2;
+      ctwo[t] = two;
+    }
+  }
+  __syncthreads();
+  const int ldl = N.ldl;
+  gdouble *NL = (gdouble *)N.L;
+  gcdouble *dbak = (gcdouble *)N.dbak;
+  gcdouble *bak = (gcdouble *)N.bak;
+  const int *lpv = N.lp;
+  const int ldbak = N.ldbak;
+  const int gmax = imin(m, cend) - 1;  // last row a B operand reads
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int ri = lane & 15, kq = lane >> 4;
+  constexpr int KC = 2;  // 4-pivot steps per register buffer
+  for (int tile = local; tile < ntiles; tile += count) {
+    int I, J;
+    tile_of(tile, TI, TJ, I, J);
+    const int r0 = t0 + I * kTile, c0 = t0 + J * kTile;
+    const bool has_fail = (fb > fa) && (c0 < fb) && (c0 + kTile > fa);
+    if (K == 0 && !has_fail) continue;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    const int ra0 = r0 + wr + ri, ra1 = ra0 + 16, cb0 = c0 + wc + ri, cb1 = cb0 + 16;
+    const int ra0c = imin(ra0, m - 1), ra1c = imin(ra1, m - 1);
+    const int cb0c = imin(cb0, gmax), cb1c = imin(cb1, gmax);
+    // raw operands of KC steps: A rows 0/1, L of B columns 0/1 (pivot and
+    // its 2x2 partner)
+    double xa0[KC], xa1[KC], xk0[KC], xo0[KC], xk1[KC], xo1[KC];
+    double ya0[KC], ya1[KC], yk0[KC], yo0[KC], yk1[KC], yo1[KC];
+    auto load = [&](int T0, double *A0, double *A1, double *K0, double *O0, double *K1,
+                    double *O1) {
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int kc = imin(T0 + 4 * u + kq, K - 1);
+        const int two = ctwo[kc];
+        const int ofs = (two == 1) ? 1 : (two == 2) ? -1 : 0;
+        const size_t ok = (size_t)(a0 + kc) * ldl, oo = (size_t)(a0 + kc + ofs) * ldl;
+        A0[u] = NL[ok + ra0c];
+        A1[u] = NL[ok + ra1c];
+        K0[u] = NL[ok + cb0c];
+        O0[u] = NL[oo + cb0c];
+        K1[u] = NL[ok + cb1c];
+        O1[u] = NL[oo + cb1c];
+      }
+    };
+    auto mfma = [&](int T0, const double *A0, const double *A1, const double *K0,
+                    const double *O0, const double *K1, const double *O1) {
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        if (T0 + 4 * u >= K) break;
+        const int kk = T0 + 4 * u + kq;
+        const bool kok = kk < K;
+        const int kc = imin(kk, K - 1);
+        const double e1 = ce1[kc], e2 = ce2[kc];
+        const int two = ctwo[kc];
+        const double b0 = (two == 0) ? K0[u] * e1
+                          : (two == 1) ? K0[u] * e1 + O0[u] * e2 : O0[u] * e1 + K0[u] * e2;
+        const double b1 = (two == 0) ? K1[u] * e1
+                          : (two == 1) ? K1[u] * e1 + O1[u] * e2 : O1[u] * e1 + K1[u] * e2;
+        const double av0 = masked(A0[u], kok && ra0 < m), av1 = masked(A1[u], kok && ra1 < m);
+        const double bv0 = masked(b0, kok && cb0 <= gmax), bv1 = masked(b1, kok && cb1 <= gmax);
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, acc[1][1], 0, 0, 0);
+      }
+    };
+    if (K > 0) {
+      load(0, xa0, xa1, xk0, xo0, xk1, xo1);
+      load(4 * KC, ya0, ya1, yk0, yo0, yk1, yo1);
+      for (int T = 0; T < K; T += 8 * KC) {
+        if (T > 0) load(T + 4 * KC, ya0, ya1, yk0, yo0, yk1, yo1);
+        mfma(T, xa0, xa1, xk0, xo0, xk1, xo1);
+        if (T + 4 * KC >= K) break;
+        load(T + 8 * KC, xa0, xa1, xk0, xo0, xk1, xo1);
+        mfma(T + 4 * KC, ya0, ya1, yk0, yo0, yk1, yo1);
+      }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg) {
+          const int gi = r0 + wr + bi * 16 + kq + 4 * rg, gj = c0 + wc + bj * 16 + ri;
+          if (gi < m && gj < cend && gi >= gj) {
+            double old;
+            if (gj >= fa && gj < fb) {
+              const int jj = gj - p;
+              old = (gi < p + w) ? dbak[lpv[gi - p] + lpv[jj] * kBlk]
+                                 : bak[(size_t)jj * ldbak + (gi - p - w)];
+            } else {
+              old = NL[(size_t)gj * ldl + gi];
+            }
+            NL[(size_t)gj * ldl + gi] = old - acc[bi][bj][rg];
+          }
+        }
+  }
+}
+
 // D (not its inverse) per pivot for the lazy contribution update:
 // Dd[3k] = D_kk, Dd[3k+1] = D_k,partner, Dd[3k+2] = partner offset.
 // ---------------------------------------------------------------------------
@@ -1795,7 +1940,11 @@ void launch_apply(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
 }
 void launch_update(hipStream_t st, int nwg, NodeDev *nodes, const int *slots,
                    const int *wg_off, int nlvl, int step, int mode, FactorParams fp) {
-  if (nwg > 0)
+  if (nwg <= 0) return;
+  if (mode == 0)
+    hipLaunchKernelGGL(k_update_panel, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
+                       nlvl, step, fp);
+  else
     hipLaunchKernelGGL(k_update, dim3(nwg), dim3(kThreads), 0, st, nodes, slots, wg_off,
                        nlvl, step, mode, fp);
 }
