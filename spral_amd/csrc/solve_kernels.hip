@@ -50,6 +50,14 @@ __device__ inline int find_node_s(const int *wg_off, int nlvl, int b) {
   return table_find(wg_off, nlvl, b);
 }
 
+// FP64 MFMA accumulators of the multi-right-hand-side products
+typedef double mr_acc_t __attribute__((ext_vector_type(4)));
+constexpr int kMrChunks = 4;  // 16-column MFMA outputs per wave: 64 right-hand sides a pass
+// independent accumulators per output (4-position step u goes to u % kMrSplit,
+// the partial sums added in order at the end): 16 dependent MFMAs become
+// 4 chains of 4
+constexpr int kMrSplit = 4;
+
 // value of v in lane j (j uniform)
 __device__ inline double bcast_lane(double v, int j) {
   const long long b = __double_as_longlong(v);
@@ -519,6 +527,91 @@ __global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, cons
   }
 }
 
+// k_bwd_rows on FP64 MFMA for any number of right-hand sides (the launch
+// used for every nrhs, so each column is the same whatever nrhs): wave w
+// takes columns 16w..16w+15 of the workgroup's 64-column chunk, the
+// right-hand sides are the 16-wide output columns (64 a pass), and the
+// workgroup's 256 rows the K dimension in steps of 4, in row order, dealt to
+// kMrSplit accumulators whose sums are added in order.
+// NCH: 16-column output chunks per pass (1 for up to 16 right-hand sides,
+// which keeps the registers for several workgroups per CU; a column's
+// arithmetic is the same in both instantiations)
+template <int NCH>
+__global__ __launch_bounds__(kThreads) void k_bwd_rows_mr(const SolveNode *sn, const int *slots,
+                                                          const int *wg_off, int nlvl,
+                                                          const double *x, int ldx, int nrhs,
+                                                          double *ws) {
+  constexpr int kSteps = 8;  // 4-row steps loaded per batch
+  __shared__ int svar[kThreads];
+  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
+  if (li < 0) return;
+  const SolveNode &P = sn[slots[li]];
+  gcdouble *PL = (gcdouble *)P.L;
+  gcint *Pperm = (gcint *)P.perm;
+  gcint *Prows = (gcint *)P.rows;
+  gdouble *part = (gdouble *)solve_P(ws, P, nrhs);  // count x nrhs x ne
+  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
+  const int nchunk = idiv_up(imax(ne, 1), kBwdCols);
+  const int local_all = blockIdx.x - wg_off[li];
+  const int local = local_all / nchunk, chunk = local_all - local * nchunk;
+  const int jbeg = chunk * kBwdCols, jend = imin(ne, jbeg + kBwdCols);
+  const int r0 = ne + local * kThreads, r1 = imin(m, r0 + kThreads);
+  if (jend <= jbeg || r1 <= r0) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ri = lane & 15, kq = lane >> 4;
+  {
+    const int i = r0 + tid;
+    svar[tid] = (i < r1) ? ((i < n) ? Pperm[i] : Prows[i - nd]) : 0;
+  }
+  __syncthreads();
+  const int nrow = r1 - r0;
+  // A operand: column acol of L (the lane's column of the wave's 16, clamped)
+  const int acol = imin(jbeg + 16 * wave + ri, jend - 1);
+  gcdouble *La = PL + (size_t)acol * ldl + r0;
+  for (int rc = 0; rc < nrhs; rc += 16 * NCH) {
+    const int nr = imin(16 * NCH, nrhs - rc);
+    mr_acc_t acc[NCH][kMrSplit];
+#pragma unroll
+    for (int h = 0; h < NCH; ++h)
+#pragma unroll
+      for (int t = 0; t < kMrSplit; ++t) acc[h][t] = (mr_acc_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nrow; k0 += 4 * kSteps) {
+      double av[kSteps];
+      int vv[kSteps];
+#pragma unroll
+      for (int u = 0; u < kSteps; ++u) {
+        const int kk = k0 + 4 * u + kq;
+        av[u] = masked(La[imin(kk, nrow - 1)], kk < nrow);
+        vv[u] = svar[imin(kk, nrow - 1)];
+      }
+#pragma unroll
+      for (int h = 0; h < NCH; ++h) {
+        if (16 * h >= nr) break;
+        const int q = 16 * h + ri;
+        const double *xq = x + (size_t)(rc + imin(q, nr - 1)) * ldx;
+        double bv[kSteps];
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u)
+          bv[u] = masked(xq[vv[u]], q < nr && k0 + 4 * u + kq < nrow);
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u)
+          acc[h][u % kMrSplit] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              av[u], bv[u], acc[h][u % kMrSplit], 0, 0, 0);
+      }
+    }
+    // element (column kq + 4 rg of the wave's 16, right-hand side ri) of chunk h
+#pragma unroll
+    for (int h = 0; h < NCH; ++h)
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg) {
+        const int c = 16 * wave + kq + 4 * rg, q = 16 * h + ri;
+        if (jbeg + c < jend && q < nr)
+          part[((size_t)local * nrhs + rc + q) * ne + jbeg + c] =
+              ((acc[h][0][rg] + acc[h][1][rg]) + acc[h][2][rg]) + acc[h][3][rg];
+      }
+  }
+}
+
 // Value of position j after x = D^{-1} x (the expressions of k_dsolve, so
 // the result is the same whether the diagonal solve runs on its own or
 // inside the backward solve)
@@ -730,14 +823,21 @@ __global__ void k_big_sync_clear(int *sync, int n) {
   if (i < n) sync[i] = 0;
 }
 
-// forward: thread (lane = row of block b, wave = a quarter of every earlier
-// block's columns); RC right-hand sides per pass
-template <int RC>
+// The products of the other blocks go to FP64 MFMA (v_mfma_f64_16x16x4f64)
+// for up to 64 right-hand sides at once: wave w takes 16 rows (forward) or
+// 16 columns (backward) of block b, the right-hand sides are the 16-wide
+// output columns, and the 64 positions of each other block are the K
+// dimension in steps of 4, in block order.  An output column depends only
+// on its own right-hand side, so every column is the same whatever nrhs.
+// The diagonal block is then solved once per right-hand side (lane = row,
+// the right-hand sides dealt to the waves in turn).
+
+// forward: block b's rows minus L[b rows, earlier columns] Y[earlier]
 __global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, const int *boff,
                                                           const int *bslot, const int *bsync,
                                                           int nbig, double *x, int ldx,
                                                           int nrhs, double *ws, int *sync) {
-  __shared__ double red[kGroups][RC][kBlk];
+  __shared__ double red[16 * kMrChunks][kBlk];
   __shared__ int s_b;
   const int li = table_find(boff, nbig, blockIdx.x);
   if (li < 0) return;
@@ -745,6 +845,7 @@ __global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, c
   int *tick = sync + bsync[li];
   int *flag = tick + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ri = lane & 15, kq = lane >> 4;
   if (tid == 0) s_b = atomicAdd(tick, 1);
   __syncthreads();
   const int b = s_b;
@@ -753,37 +854,56 @@ __global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, c
   double *PX = solve_X(ws, P, nrhs);
   const int ldl = P.ldl, m = P.m, ne = P.nelim;
   const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
-  const int row = j0 + imin(lane, wb - 1);
-  for (int rc = 0; rc < nrhs; rc += RC) {
-    const int nr = imin(RC, nrhs - rc);
-    double acc[RC];
+  // A operand row of this lane (the wave's 16 rows of block b, clamped)
+  const int arow = j0 + imin(16 * wave + ri, wb - 1);
+  // L of the diagonal block for the substitutions (lane = row), loaded once
+  double l[kBlk];
+  {
+    const int rr = j0 + imin(lane, wb - 1);
 #pragma unroll
-    for (int q = 0; q < RC; ++q) acc[q] = 0.0;
+    for (int j = 0; j < kBlk; ++j)
+      l[j] = masked(PL[(size_t)(j0 + imin(j, wb - 1)) * ldl + rr], j < lane && lane < wb);
+  }
+  for (int rc = 0; rc < nrhs; rc += 16 * kMrChunks) {
+    const int nr = imin(16 * kMrChunks, nrhs - rc);
+    mr_acc_t acc[kMrChunks][kMrSplit];
+#pragma unroll
+    for (int h = 0; h < kMrChunks; ++h)
+#pragma unroll
+      for (int t = 0; t < kMrSplit; ++t) acc[h][t] = (mr_acc_t){0.0, 0.0, 0.0, 0.0};
     for (int j = 0; j < b; ++j) {  // every block before b is a full block
-      const int c0 = j * kBlk + 16 * wave;
-      double lv[16];
+      const int c0 = j * kBlk;
+      double av[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)(c0 + u) * ldl + row];
+      for (int u = 0; u < 16; ++u) av[u] = PL[(size_t)(c0 + 4 * u + kq) * ldl + arow];
       if (rc == 0) wait_flag(flag + j);
 #pragma unroll
-      for (int q = 0; q < RC; ++q) {
-        if (q < nr) {
-          const double *y = PX + (size_t)(rc + q) * m + c0;
+      for (int h = 0; h < kMrChunks; ++h) {
+        if (16 * h >= nr) break;
+        const int q = 16 * h + ri;
+        const double *y = PX + (size_t)(rc + imin(q, nr - 1)) * m + c0 + kq;
+        double bv[16];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) acc[q] += lv[u] * load_dev(y + u);
-        }
+        for (int u = 0; u < 16; ++u) bv[u] = masked(load_dev(y + 4 * u), q < nr);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          acc[h][u % kMrSplit] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              av[u], bv[u], acc[h][u % kMrSplit], 0, 0, 0);
       }
     }
+    // element (row kq + 4 rg of the wave's rows, right-hand side ri) of chunk h
 #pragma unroll
-    for (int q = 0; q < RC; ++q) red[wave][q][lane] = acc[q];
+    for (int h = 0; h < kMrChunks; ++h)
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg)
+        red[16 * h + ri][16 * wave + kq + 4 * rg] =
+            ((acc[h][0][rg] + acc[h][1][rg]) + acc[h][2][rg]) + acc[h][3][rg];
     __syncthreads();
     for (int q = wave; q < nr; q += kGroups) {
       double *X = PX + (size_t)(rc + q) * m;
-      double v = 0.0;
-      if (lane < wb)
-        v = X[j0 + lane] - (((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) +
-                            red[3][q][lane]);
-      v = fwd_diag_wave(PL, ldl, j0, wb, v, lane);
+      double v = (lane < wb) ? X[j0 + lane] - red[q][lane] : 0.0;
+#pragma unroll
+      for (int j = 0; j < kBlk; ++j) v = fma(-l[j], bcast_lane(v, j), v);
       if (lane < wb) {
         X[j0 + lane] = v;
         x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
@@ -794,16 +914,13 @@ __global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, c
   if (tid == 0) raise_flag(flag + b);
 }
 
-// backward: thread (lane = row of a later block, wave = a quarter of block
-// b's columns); per-lane sums over the later blocks, then one reduce over
-// the lanes
-template <int RC>
+// backward: block b's values minus L[later rows, b columns]^T X[later]
 __global__ __launch_bounds__(kThreads) void k_big_bwd_lvl(const SolveNode *sn, const int *boff,
                                                           const int *bslot, const int *bsync,
                                                           int nbig, double *x, int ldx,
                                                           int nrhs, double *ws, int *sync) {
   __shared__ double lb[kBlk][kBlk + 1];
-  __shared__ double red[RC][kBlk];
+  __shared__ double red[16 * kMrChunks][kBlk];
   __shared__ int s_b;
   const int li = table_find(boff, nbig, blockIdx.x);
   if (li < 0) return;
@@ -812,6 +929,7 @@ __global__ __launch_bounds__(kThreads) void k_big_bwd_lvl(const SolveNode *sn, c
   int *tick = sync + bsync[li] + nb + 1;
   int *flag = tick + 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ri = lane & 15, kq = lane >> 4;
   if (tid == 0) s_b = nb - 1 - atomicAdd(tick, 1);
   __syncthreads();
   const int b = s_b;
@@ -821,40 +939,54 @@ __global__ __launch_bounds__(kThreads) void k_big_bwd_lvl(const SolveNode *sn, c
   const int ldl = P.ldl, m = P.m;
   const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
   stage_lower(PL, ldl, j0, wb, lb);
-  // the wave's 16 columns of block b (clamped; the surplus sums are dropped)
-  const int c0 = j0 + 16 * wave;
-  for (int rc = 0; rc < nrhs; rc += RC) {
-    const int nr = imin(RC, nrhs - rc);
-    double acc[RC][16];
+  // A operand column of this lane (the wave's 16 columns of block b, clamped)
+  const int acol = j0 + imin(16 * wave + ri, wb - 1);
+  for (int rc = 0; rc < nrhs; rc += 16 * kMrChunks) {
+    const int nr = imin(16 * kMrChunks, nrhs - rc);
+    mr_acc_t acc[kMrChunks][kMrSplit];
 #pragma unroll
-    for (int q = 0; q < RC; ++q)
+    for (int h = 0; h < kMrChunks; ++h)
 #pragma unroll
-      for (int u = 0; u < 16; ++u) acc[q][u] = 0.0;
+      for (int t = 0; t < kMrSplit; ++t) acc[h][t] = (mr_acc_t){0.0, 0.0, 0.0, 0.0};
     for (int j = nb - 1; j > b; --j) {
       const int r0 = j * kBlk, rows = imin(kBlk, ne - r0);
-      const int i = r0 + imin(lane, rows - 1);
-      double lv[16];
+      double av[16];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(c0 + u, j1 - 1) * ldl + i];
+      for (int u = 0; u < 16; ++u)
+        av[u] = masked(PL[(size_t)acol * ldl + r0 + imin(4 * u + kq, rows - 1)],
+                       4 * u + kq < rows);
       if (rc == 0) wait_flag(flag + j);
 #pragma unroll
-      for (int q = 0; q < RC; ++q) {
-        const double xi =
-            (q < nr && lane < rows) ? load_dev(PX + (size_t)(rc + q) * m + i) : 0.0;
+      for (int h = 0; h < kMrChunks; ++h) {
+        if (16 * h >= nr) break;
+        const int q = 16 * h + ri;
+        const double *xq = PX + (size_t)(rc + imin(q, nr - 1)) * m + r0;
+        double bv[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) acc[q][u] += lv[u] * xi;
+        for (int u = 0; u < 16; ++u)
+          bv[u] = masked(load_dev(xq + imin(4 * u + kq, rows - 1)), q < nr && 4 * u + kq < rows);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+          acc[h][u % kMrSplit] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              av[u], bv[u], acc[h][u % kMrSplit], 0, 0, 0);
       }
     }
+    // element (column kq + 4 rg of the wave's columns, right-hand side ri)
 #pragma unroll
-    for (int q = 0; q < RC; ++q) {
-      const double s = wave_reduce_scatter<16>(acc[q], lane);
-      if ((lane & 3) == 0) red[q][16 * wave + (lane >> 2)] = s;
-    }
+    for (int h = 0; h < kMrChunks; ++h)
+#pragma unroll
+      for (int rg = 0; rg < 4; ++rg)
+        red[16 * h + ri][16 * wave + kq + 4 * rg] =
+            ((acc[h][0][rg] + acc[h][1][rg]) + acc[h][2][rg]) + acc[h][3][rg];
     __syncthreads();
+    double c[kBlk];
+#pragma unroll
+    for (int i = 0; i < kBlk; ++i) c[i] = lb[i][lane];
     for (int q = wave; q < nr; q += kGroups) {
       double *X = PX + (size_t)(rc + q) * m;
       double v = (lane < wb) ? X[j0 + lane] - red[q][lane] : 0.0;
-      v = bwd_diag_wave(lb, v, lane);
+#pragma unroll
+      for (int i = kBlk - 1; i >= 0; --i) v = fma(-c[i], bcast_lane(v, i), v);
       if (lane < wb) {
         X[j0 + lane] = v;
         x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
@@ -876,20 +1008,14 @@ int bwd_cols_per_wg() { return kBwdCols; }
 void launch_big_fwd_lvl(hipStream_t st, const SolveNode *sn, const int *tab, int nbig, int nwg,
                         double *x, int ldx, int nrhs, double *ws, int *sync) {
   if (nbig <= 0) return;
-  if (nrhs == 1)
-    hipLaunchKernelGGL(k_big_fwd_lvl<1>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
-                       tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
-  else
-    hipLaunchKernelGGL(k_big_fwd_lvl<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
-                       tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
+  hipLaunchKernelGGL(k_big_fwd_lvl, dim3(nwg), dim3(kThreads), 0, st, sn, tab, tab + nbig + 1,
+                     tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
 }
 void launch_big_bwd_lvl(hipStream_t st, const SolveNode *sn, const int *tab, int nbig, int nwg,
                         double *x, int ldx, int nrhs, double *ws, int *sync) {
   if (nbig <= 0) return;
-  // one right-hand side per pass: the per-lane sums of 16 columns fill the
-  // registers already
-  hipLaunchKernelGGL(k_big_bwd_lvl<1>, dim3(nwg), dim3(kThreads), 0, st, sn, tab,
-                     tab + nbig + 1, tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
+  hipLaunchKernelGGL(k_big_bwd_lvl, dim3(nwg), dim3(kThreads), 0, st, sn, tab, tab + nbig + 1,
+                     tab + 2 * nbig + 1, nbig, x, ldx, nrhs, ws, sync);
 }
 void launch_big_sync_clear(hipStream_t st, int *sync, int n) {
   if (n > 0)
@@ -929,11 +1055,11 @@ void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                 double *ws, int dfuse) {
   if (nlvl <= 0) return;
   if (nwg > 0) {
-    if (nrhs == 1)
-      hipLaunchKernelGGL(k_bwd_rows<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots, chunk_off,
-                         nlvl, x, ldx, nrhs, ws);
+    if (nrhs <= 16)
+      hipLaunchKernelGGL(k_bwd_rows_mr<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
+                         chunk_off, nlvl, x, ldx, nrhs, ws);
     else
-      hipLaunchKernelGGL(k_bwd_rows<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
+      hipLaunchKernelGGL(k_bwd_rows_mr<kMrChunks>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
                          chunk_off, nlvl, x, ldx, nrhs, ws);
   }
   if (nrhs == 1)

@@ -138,7 +138,10 @@ def test_laplacian_closed_form_inertia(lib, g):
     assert finf.num_neg == int((ev < 0).sum())
     xt, b = rhs(m, 2, g)
     x = gpu_solve(ak, fk, opts, b)
-    assert matrices.bwd_error(m.full, x, b).max() <= BWD_TOL
+    assert matrices.bwd_error(m.full, x, b).max() <= RAW_TOL
+    xr = b.copy()
+    ssids.ssids_solve_refine(xr, ak, fk, opts, maxit=5, tol=BWD_TOL)
+    assert matrices.bwd_error(m.full, xr, b).max() <= BWD_TOL
     ssids.ssids_free(ak, fk)
 
 
