@@ -15,14 +15,19 @@
 //   k_fwd_rows: 64-row workgroups over [ne, m): the children's updates that
 //               land on the rows (summed in LDS in a fixed order) minus
 //               L[rows, 0:ne] X[0:ne].  X[ne:m] is what the parent gathers.
-// Backward (root -> leaves), one launch per level:
-//   k_bwd     : 64-row workgroups over [ne, m) write partial sums of
-//               L[rows, 0:ne]^T x[rows]; the last workgroup of a front to
-//               finish (device-scope counter) reduces them in workgroup order
-//               and solves the triangle with L11^T.
+// Backward (root -> leaves), two launches per level:
+//   k_bwd_rows_mr: 256-row workgroups over [ne, m) write partial sums of
+//               L[rows, 0:ne]^T x[rows] (FP64 MFMA, up to 64 right-hand sides
+//               a pass);
+//   k_bwd_tri : one workgroup per front: the partials reduced in workgroup
+//               order, then the triangle with L11^T.
 // Fronts with ne > kBigTri (separators at the top of the tree) solve their
 // triangle in one launch per level over 64-column blocks, the blocks of a
 // front chained by flags (k_big_fwd_lvl / k_big_bwd_lvl).
+// Several right-hand sides: the front kernels take one right-hand side per
+// workgroup (gridDim.y of them in parallel) and the row kernels' passes run
+// in parallel the same way, so every column is summed exactly as a
+// single-right-hand-side solve sums it.
 //
 // Loads of L are issued in batches of 16 independent addresses (column indices
 // clamped into the block instead of guarded, the surplus products multiplied
@@ -43,6 +48,7 @@ constexpr int kRows = 64;                  // rows per workgroup of the row kern
 constexpr int kGroups = kThreads / kRows;  // column groups (one wave each)
 constexpr int kYChunk = 256;               // columns of X[0:ne] staged per pass
 constexpr int kRhsChunk = 8;
+constexpr int kTriRhsPar = 64;  // right-hand sides solved in parallel by the front kernels
 constexpr int kBigTri = 256;
 constexpr int kTriCap = 1024;  // fully-summed positions gathered in LDS
 
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
   const int cbeg = P.cbeg, cend = P.cend;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int T = idiv_up(m, 64), Tn = idiv_up(n, 64);
-  for (int r = 0; r < nrhs; ++r) {
+  for (int r = blockIdx.y; r < nrhs; r += gridDim.y) {
     gdouble *X = PX + (size_t)r * m;
     double *xr = x + (size_t)r * ldx;
     double *acc = (n <= kTriCap) ? xs : (double *)X;
@@ -235,96 +241,6 @@ __global__ __launch_bounds__(kThreads) void k_fwd_tri(const SolveNode *sn, const
   }
 }
 
-// k_fwd_tri for nrhs >= 2: each wave takes its own right-hand sides
-// (r = wave, wave + 4, ...), so no workgroup barrier falls inside a
-// right-hand side.  Every sum keeps k_fwd_tri's order (the children's
-// values arrive on k_fwd_tri's schedule; a row's triangle products are
-// summed in column order), so each column is bitwise what k_fwd_tri gives.
-// Wave-scope fences order one lane's LDS / global writes before another
-// lane's reads.
-__device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
-
-__global__ __launch_bounds__(kThreads) void k_fwd_tri_mr(const SolveNode *sn, const int *slots,
-                                                         const int *child_list, double *x,
-                                                         int ldx, int nrhs, double *ws) {
-  __shared__ double xs[kGroups][kTriCap];
-  __shared__ double ybs[kGroups][kBlk];
-  const SolveNode &P = sn[slots[blockIdx.x]];
-  gcdouble *PL = (gcdouble *)P.L;
-  gcint *Pperm = (gcint *)P.perm;
-  gcint *Pipos = (gcint *)P.ipos;
-  gcint *Prows = (gcint *)P.rows;
-  gcint *Pctab = (gcint *)P.ctab;
-  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
-  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
-  const int cbeg = P.cbeg, cend = P.cend;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int T = idiv_up(m, 64), Tn = idiv_up(n, 64);
-  double *yb = ybs[wave];
-  for (int r = wave; r < nrhs; r += kGroups) {
-    gdouble *X = PX + (size_t)r * m;
-    double *xr = x + (size_t)r * ldx;
-    double *acc = (n <= kTriCap) ? xs[wave] : (double *)X;
-    for (int p = lane; p < n; p += 64) {
-      const int vrow = (p >= nd) ? Prows[p - nd] : -1;
-      acc[Pipos[p]] = (vrow >= 0) ? xr[vrow] : 0.0;
-    }
-    wave_fence();
-    // k_fwd_tri's schedule: children four at a time, 64-entry chunk by
-    // chunk, the four children's chunks in turn (a position fed by several
-    // children receives their values in that order)
-    for (int c0 = cbeg; c0 < cend; c0 += kGroups) {
-      int nch = 0;
-      for (int g = 0; g < kGroups && c0 + g < cend; ++g) {
-        gcint *tab = Pctab + (size_t)(c0 + g - cbeg) * (T + 1);
-        nch = imax(nch, idiv_up(tab[Tn] - tab[0], 64));
-      }
-      for (int it = 0; it < nch; ++it) {
-        for (int g = 0; g < kGroups && c0 + g < cend; ++g) {
-          const int ci = c0 + g;
-          const SolveNode &c = sn[child_list[ci]];
-          gcint *tab = Pctab + (size_t)(ci - cbeg) * (T + 1);
-          const int kb = tab[0], ke = tab[Tn];
-          const int k = kb + it * 64 + lane;
-          if (k < ke) {
-            const int pre = (k < c.n) ? c.doff + (k - c.nelim) : nd + c.rmap[k - c.n];
-            if (pre < n) acc[Pipos[pre]] += solve_X(ws, c, nrhs)[(size_t)r * c.m + k];
-          }
-          wave_fence();
-        }
-      }
-    }
-    if (ne <= kBigTri) {
-      for (int j0 = 0; j0 < ne; j0 += kBlk) {
-        const int j1 = imin(j0 + kBlk, ne), wb = j1 - j0;
-        const double v =
-            fwd_diag_wave(PL, ldl, j0, wb, (lane < wb) ? acc[j0 + lane] : 0.0, lane);
-        if (lane < wb) acc[j0 + lane] = v;
-        yb[lane] = v;
-        wave_fence();
-        for (int i = j1 + lane; i < ne; i += 64) {
-          double sum = 0.0;
-#pragma unroll
-          for (int u0 = 0; u0 < kBlk; u0 += 16) {
-            double lv[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-              lv[u] = PL[(size_t)(j0 + imin(u0 + u, wb - 1)) * ldl + i];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) sum += lv[u] * yb[u0 + u];
-          }
-          acc[i] -= sum;
-        }
-        wave_fence();
-      }
-      for (int p = lane; p < ne; p += 64) xr[Pperm[p]] = acc[p];
-    }
-    if (acc != (double *)X)
-      for (int p = lane; p < n; p += 64) X[p] = acc[p];
-    wave_fence();
-  }
-}
-
 // 64-row workgroups over [ne, m) of every front of the level (RC right-hand
 // sides per pass):
 //   X[i] = (gathered fully-summed value, i < n) or (sum of the children's
@@ -352,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_rows(const SolveNode *sn, cons
   const int row = imin(r0 + rr, r1 - 1);
   const int T = idiv_up(m, 64);
   const int lo = imax(r0, n), I0 = lo / 64, I1 = idiv_up(r1, 64);
-  for (int rc = 0; rc < nrhs; rc += RC) {
+  for (int rc = blockIdx.y * RC; rc < nrhs; rc += gridDim.y * RC) {
     const int nr = imin(RC, nrhs - rc);
     for (int idx = tid; idx < RC * kRows; idx += kThreads) {
       const int q = idx / kRows, i = idx % kRows, rw = r0 + i;
@@ -475,57 +391,6 @@ __global__ void k_dsolve(const SolveNode *sn, int nslots, double *x, int ldx, in
 // reduce-scatter, then the four waves' sums in wave order.  Each (rb, j) is
 // written by exactly one workgroup, so the sums do not depend on the split.
 constexpr int kBwdCols = 64;
-template <int RC>
-__global__ __launch_bounds__(kThreads) void k_bwd_rows(const SolveNode *sn, const int *slots,
-                                                       const int *wg_off, int nlvl,
-                                                       const double *x, int ldx, int nrhs,
-                                                       double *ws) {
-  __shared__ double red[kGroups][RC][kBlk];
-  const int li = find_node_s(wg_off, nlvl, blockIdx.x);
-  const SolveNode &P = sn[slots[li]];
-  gcdouble *PL = (gcdouble *)P.L;
-  gcint *Pperm = (gcint *)P.perm;
-  gcint *Prows = (gcint *)P.rows;
-  gdouble *part = (gdouble *)solve_P(ws, P, nrhs);  // count x nrhs x ne
-  const int ldl = P.ldl, m = P.m, n = P.n, nd = P.nd, ne = P.nelim;
-  const int nchunk = idiv_up(imax(ne, 1), kBwdCols);
-  const int local_all = blockIdx.x - wg_off[li];
-  const int local = local_all / nchunk, chunk = local_all - local * nchunk;
-  const int jbeg = chunk * kBwdCols, jend = imin(ne, jbeg + kBwdCols);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r0 = ne + local * kThreads, r1 = imin(m, r0 + kThreads);
-  const int i = r0 + tid, row = imin(i, r1 - 1);
-  const int var = (i < r1) ? ((i < n) ? Pperm[i] : Prows[i - nd]) : -1;
-  for (int rc = 0; rc < nrhs; rc += RC) {
-    const int nr = imin(RC, nrhs - rc);
-    double xv[RC];
-#pragma unroll
-    for (int q = 0; q < RC; ++q) xv[q] = (q < nr && var >= 0) ? x[(size_t)(rc + q) * ldx + var] : 0.0;
-    for (int j0 = jbeg; j0 < jend; j0 += kBlk) {
-      for (int u0 = 0; u0 < kBlk && j0 + u0 < jend; u0 += 16) {
-        double lv[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) lv[u] = PL[(size_t)imin(j0 + u0 + u, jend - 1) * ldl + row];
-#pragma unroll
-        for (int q = 0; q < RC; ++q) {
-          double a[16];
-#pragma unroll
-          for (int u = 0; u < 16; ++u) a[u] = lv[u] * xv[q];
-          const double s = wave_reduce_scatter<16>(a, lane);
-          if ((lane & 3) == 0) red[wave][q][u0 + (lane >> 2)] = s;
-        }
-      }
-      __syncthreads();
-      for (int idx = tid; idx < RC * kBlk; idx += kThreads) {
-        const int q = idx / kBlk, c = idx % kBlk;
-        if (q < nr && j0 + c < jend)
-          part[((size_t)local * nrhs + rc + q) * ne + j0 + c] =
-              ((red[0][q][c] + red[1][q][c]) + red[2][q][c]) + red[3][q][c];
-      }
-      __syncthreads();
-    }
-  }
-}
 
 // k_bwd_rows on FP64 MFMA for any number of right-hand sides (the launch
 // used for every nrhs, so each column is the same whatever nrhs): wave w
@@ -644,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
   const int ldl = P.ldl, m = P.m, ne = P.nelim;
   const int count = wg_off[li + 1] - wg_off[li];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int r = 0; r < nrhs; ++r) {
+  for (int r = blockIdx.y; r < nrhs; r += gridDim.y) {
     gdouble *X = PX + (size_t)r * m;
     double *xr = x + (size_t)r * ldx;
     double *acc = (ne <= kBigTri) ? xt : (double *)X;
@@ -697,92 +562,6 @@ __global__ __launch_bounds__(kThreads) void k_bwd_tri(const SolveNode *sn, const
       __syncthreads();
     }
     for (int j = tid; j < ne; j += kThreads) xr[Pperm[j]] = xt[j];
-    __syncthreads();
-  }
-}
-
-// k_bwd_tri for nrhs >= 2: the right-hand sides four at a time, wave w on
-// right-hand side r0 + w; the staged diagonal block is shared by the four.
-// Each wave reproduces the four row groups of k_bwd_tri (its partial sums
-// over workgroups g, g+4, ..., and its triangle rows [j1 + 64g, ...)) and
-// combines them in k_bwd_tri's order, so each column is bitwise what
-// k_bwd_tri gives.
-__global__ __launch_bounds__(kThreads) void k_bwd_tri_mr(const SolveNode *sn, const int *slots,
-                                                         const int *wg_off, double *x,
-                                                         int ldx, int nrhs, double *ws,
-                                                         int dfuse) {
-  __shared__ double xts[kGroups][kBigTri];
-  __shared__ double lb[kBlk][kBlk + 1];
-  const int li = blockIdx.x;
-  const SolveNode &P = sn[slots[li]];
-  gcdouble *PL = (gcdouble *)P.L;
-  gcdouble *Pd = (gcdouble *)P.D;
-  gcint *Pperm = (gcint *)P.perm;
-  gdouble *PX = (gdouble *)solve_X(ws, P, nrhs);
-  gcdouble *part = (gcdouble *)solve_P(ws, P, nrhs);
-  const int ldl = P.ldl, m = P.m, ne = P.nelim;
-  const int count = wg_off[li + 1] - wg_off[li];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r0 = 0; r0 < nrhs; r0 += kGroups) {
-    const int r = r0 + wave;
-    const bool mine = r < nrhs;
-    double *xt = xts[wave];
-    if (mine) {
-      gdouble *X = PX + (size_t)r * m;
-      double *xr = x + (size_t)r * ldx;
-      double *acc = (ne <= kBigTri) ? xt : (double *)X;
-      for (int jc = 0; jc < ne; jc += kYChunk) {
-#pragma unroll
-        for (int k = 0; k < kYChunk / 64; ++k) {
-          const int j = jc + lane + 64 * k;
-          double sg[kGroups];
-#pragma unroll
-          for (int g = 0; g < kGroups; ++g) {
-            double sum = 0.0;
-            for (int w = g; w < count; w += kGroups)
-              if (j < ne) sum += part[((size_t)w * nrhs + r) * ne + j];
-            sg[g] = sum;
-          }
-          if (j < ne)
-            acc[j] = (dfuse ? dinv_at(Pd, Pperm, xr, j, ne) : xr[Pperm[j]]) -
-                     (((sg[0] + sg[1]) + sg[2]) + sg[3]);
-        }
-      }
-    }
-    if (ne > kBigTri) continue;  // the triangle is solved by the big-front launches
-    __syncthreads();
-    for (int b = idiv_up(ne, kBlk) - 1; b >= 0; --b) {
-      const int j0 = b * kBlk, j1 = imin(j0 + kBlk, ne), wb = j1 - j0, rows = ne - j1;
-      stage_lower(PL, ldl, j0, wb, lb);
-      __syncthreads();  // the staged block is read by every wave
-      if (mine) {
-        double sg[kGroups];
-#pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-          double sw = 0.0;
-          if (g * 64 < rows) {
-            const int t = 64 * g + lane;
-            const int i = j1 + imin(t, rows - 1);
-            const double xi = (t < rows) ? xt[i] : 0.0;
-            double a[kBlk];
-#pragma unroll
-            for (int u = 0; u < kBlk; ++u)
-              a[u] = PL[(size_t)(j0 + imin(u, wb - 1)) * ldl + i] * xi;
-            sw = wave_reduce_scatter<kBlk>(a, lane);
-          }
-          sg[g] = sw;
-        }
-        double v = 0.0;
-        if (lane < wb) v = xt[j0 + lane] - (((sg[0] + sg[1]) + sg[2]) + sg[3]);
-        v = bwd_diag_wave(lb, v, lane);
-        if (lane < wb) xt[j0 + lane] = v;
-      }
-      __syncthreads();
-    }
-    if (mine) {
-      double *xr = x + (size_t)r * ldx;
-      for (int j = lane; j < ne; j += 64) xr[Pperm[j]] = xt[j];
-    }
     __syncthreads();
   }
 }
@@ -1025,12 +804,10 @@ void launch_big_sync_clear(hipStream_t st, int *sync, int n) {
 void launch_fwd_tri(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                     const int *child_list, double *x, int ldx, int nrhs, double *ws) {
   if (nlvl <= 0) return;
-  if (nrhs == 1)
-    hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list, x,
-                       ldx, nrhs, ws);
-  else  // a wave per right-hand side: the same sums, no barrier per right-hand side
-    hipLaunchKernelGGL(k_fwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, child_list,
-                       x, ldx, nrhs, ws);
+  // a workgroup per front and right-hand side: each right-hand side gets the
+  // single-right-hand-side sums, in parallel
+  hipLaunchKernelGGL(k_fwd_tri, dim3(nlvl, imin(nrhs, kTriRhsPar)), dim3(kThreads), 0, st, sn,
+                     slots, child_list, x, ldx, nrhs, ws);
 }
 void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
                      const int *child_list, int nrhs, const int *wg_off, int nwg, double *ws) {
@@ -1038,9 +815,9 @@ void launch_fwd_rows(hipStream_t st, int nlvl, const SolveNode *sn, const int *s
   if (nrhs == 1)
     hipLaunchKernelGGL(k_fwd_rows<1>, dim3(nwg), dim3(kThreads), 0, st, sn, slots, child_list,
                        wg_off, nlvl, nrhs, ws);
-  else
-    hipLaunchKernelGGL(k_fwd_rows<kRhsChunk>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
-                       child_list, wg_off, nlvl, nrhs, ws);
+  else  // the passes of kRhsChunk right-hand sides in parallel
+    hipLaunchKernelGGL(k_fwd_rows<kRhsChunk>, dim3(nwg, idiv_up(nrhs, kRhsChunk)),
+                       dim3(kThreads), 0, st, sn, slots, child_list, wg_off, nlvl, nrhs, ws);
 }
 void launch_dsolve(hipStream_t st, int nslots, const SolveNode *sn, double *x, int ldx,
                    int nrhs, int half) {
@@ -1062,12 +839,8 @@ void launch_bwd(hipStream_t st, int nlvl, const SolveNode *sn, const int *slots,
       hipLaunchKernelGGL(k_bwd_rows_mr<kMrChunks>, dim3(nwg), dim3(kThreads), 0, st, sn, slots,
                          chunk_off, nlvl, x, ldx, nrhs, ws);
   }
-  if (nrhs == 1)
-    hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs, ws, dfuse);
-  else  // four right-hand sides at a time, one a wave
-    hipLaunchKernelGGL(k_bwd_tri_mr, dim3(nlvl), dim3(kThreads), 0, st, sn, slots, rows_off, x,
-                       ldx, nrhs, ws, dfuse);
+  hipLaunchKernelGGL(k_bwd_tri, dim3(nlvl, imin(nrhs, kTriRhsPar)), dim3(kThreads), 0, st, sn,
+                     slots, rows_off, x, ldx, nrhs, ws, dfuse);
 }
 
 }  // namespace ssids_hip
