@@ -80,3 +80,29 @@ def test_batch_refined_accuracy(lib):
     for m, x, r in zip(mats, xs, rhs):
         assert matrices.bwd_error(m.full, x, r).max() <= BWD_TOL
     batch.batch_free(b)
+
+
+@pytest.mark.timeout(900)
+def test_batch_config4_full_size(lib):
+    """4,096 KKT systems, n uniform in [200, 5000], each in the matching-based
+    order with its scaling (the batched workload at its full size): every
+    system's inertia is the generator's floor(0.2 n) and every solution
+    refines to the 1e-14 bar."""
+    rng = np.random.Generator(np.random.PCG64(20261202 + 4))
+    sizes = rng.integers(200, 5001, 4096)
+    mats = [matrices.kkt_random(int(n), seed=20261202 + 4 + i) for i, n in enumerate(sizes)]
+    b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=2, scaling=3))
+    assert inf.flag >= 0, inf
+    val = np.concatenate([m.val for m in mats])
+    finf, neg, rank = batch.batch_factor(b, val)
+    assert finf.flag >= 0, finf
+    print(f"4096 systems: n={b.akeep.n}, {b.akeep.nnodes} fronts, levels {finf.num_levels}, "
+          f"delays {finf.num_delay}", flush=True)
+    assert list(neg) == [int(0.2 * m.n) for m in mats]
+    assert list(rank) == [m.n for m in mats]
+    rhs = [m.full @ rng.uniform(-1, 1, m.n) for m in mats]
+    xs = batch.batch_solve(b, rhs, refine=True)
+    worst = max(float(matrices.bwd_error(m.full, x, r).max()) for m, x, r in zip(mats, xs, rhs))
+    print(f"  worst refined bwd err {worst:.2e}", flush=True)
+    assert worst <= BWD_TOL
+    batch.batch_free(b)
