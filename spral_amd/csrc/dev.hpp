@@ -117,7 +117,9 @@ __host__ __device__ inline int level_table_count(int t, int S, int m, int n, int
     const int stp = t - 1;
     return stp < nsteps ? stp + imax(1, idiv_up(m, kBlk)) : 0;
   }
-  if (t == S + 1) return nsteps > 1 ? imax(1, idiv_up(2LL * kBlk * m, kThreads * 8)) : 0;
+  // failed-column swap: up to 32 elements a thread (most steps swap nothing,
+  // and a small grid dispatches faster)
+  if (t == S + 1) return nsteps > 1 ? imax(1, idiv_up(2LL * kBlk * m, kThreads * 32)) : 0;
   if (t <= 2 * S + 1) {
     const int stp = t - S - 2;
     if (stp >= nsteps) return 0;
