@@ -292,6 +292,15 @@ def equilib_scale_sym(n, ptr, row, val, max_iterations=10, tol=1e-8):
     return s[:n], its.value
 
 
+def _match_order_warning(inf, akeep, options):
+    """Every factorization of a matching-based analyse that does not use the
+    matching's scaling (options.scaling != 3) warns, refactors included
+    (src/ssids/ssids.f90:915-918); a factorization warning of its own takes
+    precedence."""
+    if inf.flag == SSIDS_SUCCESS and akeep.scaling is not None and options.scaling != 3:
+        inf.flag = SSIDS_WARNING_MATCH_ORD_NO_SCALE
+
+
 def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
     """Numeric factorization on the GPU; returns (fkeep, inform)."""
     options = options or SSIDSOptions()
@@ -372,6 +381,7 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
                                           d_sc.ptr if d_sc is not None else None,
                                           C.byref(opts), C.byref(st))
         inf = _inform_from_stats(st, akeep)
+        _match_order_warning(inf, akeep, options)
         fkeep.scaling, fkeep.inform = sc, inf
         if inf.flag < 0:
             L.spral_ssids_hip_destroy_numeric(fkeep.numeric)
@@ -385,8 +395,8 @@ def ssids_factor(posdef, val, akeep, options=None, scale=None, fkeep=None):
     num = L.spral_ssids_hip_factor(bool(posdef), akeep.symbolic, val, val.size, scp,
                                    C.byref(opts), C.byref(st))
     inf = _inform_from_stats(st, akeep)
-    if num and inf.flag == SSIDS_SUCCESS and akeep.scaling is not None and options.scaling != 3:
-        inf.flag = SSIDS_WARNING_MATCH_ORD_NO_SCALE  # src/ssids/ssids.f90:915-918
+    if num:
+        _match_order_warning(inf, akeep, options)
     fk = FKeep(numeric=num, pos_def=bool(posdef), scaling=sc, inform=inf,
                symbolic=akeep.symbolic)
     if num and inf.flag == SSIDS_WARNING_FACT_SINGULAR and not options.action:

@@ -144,3 +144,75 @@ def test_refactor_and_solves_overlap(lib):
         assert np.array_equal(got, x)
         ssids.ssids_free(None, fk)
     ssids.ssids_free(ak)
+
+
+@pytest.mark.timeout(600)
+def test_service_config5_patterns(lib):
+    """The serving workload's patterns at their sizes (KKT: 7-point H on 16^3
+    and 28^3 plus grid^3/8 constraint rows, n = 4,608 and 24,696), matching-
+    based orders from the registered values and the matching scaling per
+    refactor, refactors with log-uniform [1e-8, 1e8] barrier terms on H's
+    diagonal (thousands of delayed pivots at 28^3) mixed with solves of 1 to
+    64 right-hand sides: every result is bitwise the one of replaying the
+    pattern alone, and every solve refines to 1e-14."""
+    rng = np.random.default_rng(20261203)
+    grids = (16, 28)
+    pats = {f"kkt{g}": matrices.kkt_laplace(grid=g, m_con=g ** 3 // 8, seed=20261202 + 5 + g)
+            for g in grids}
+    opts = ssids.SSIDSOptions(ordering=2, scaling=1)
+    plan = {}
+    for g, (name, m) in zip(grids, pats.items()):
+        col = np.repeat(np.arange(m.n), np.diff(m.ptr))
+        dpos = np.nonzero((m.row - 1 == col) & (col < g ** 3))[0]
+        steps = [("refactor", m.val.copy())]
+        for i in range(8):
+            if i % 3 == 1:
+                v = m.val.copy()
+                v[dpos] += 10.0 ** rng.uniform(-8, 8, dpos.size)
+                steps.append(("refactor", v))
+            else:
+                k = int(rng.choice([1, 4, 16, 64]))
+                steps.append(("solve", rng.uniform(-1, 1, (m.n, k))))
+        plan[name] = steps
+    svc = serve.SolverService(hbm_budget_bytes=64 << 30, options=opts)
+    for name, m in pats.items():
+        svc.register(name, m.n, m.ptr, m.row, val=m.val)
+    svc.start()
+    futures = {name: [] for name in plan}
+
+    def produce(name):
+        for kind, data in plan[name]:
+            f = svc.submit_refactor(name, data) if kind == "refactor" else \
+                svc.submit_solve(name, data)
+            futures[name].append(f)
+
+    th = [threading.Thread(target=produce, args=(n,)) for n in plan]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    results = {n: [f.result(timeout=300) for f in fs] for n, fs in futures.items()}
+    print(f"  refactor latency {svc.latency_ms('refactor')}, "
+          f"solve latency {svc.latency_ms('solve')}", flush=True)
+    svc.stop()
+    for name, m in pats.items():
+        ak, _ = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, val=m.val)
+        fk = None
+        vals = None
+        for (kind, data), got in zip(plan[name], results[name]):
+            if kind == "refactor":
+                vals = data
+                if fk is not None:
+                    ssids.ssids_free(None, fk)
+                fk, inf = ssids.ssids_factor(False, vals, ak, opts)
+                assert got.flag == inf.flag and got.num_neg == inf.num_neg, name
+                print(f"  {name}: refactor delays {inf.num_delay}", flush=True)
+                continue
+            low = sp.csc_matrix((vals, m.row - 1, m.ptr - 1), shape=(m.n, m.n))
+            full = low + sp.tril(low, k=-1).T
+            for j in range(data.shape[1]):
+                x = data[:, j].copy()
+                ssids.ssids_solve_refine(x, ak, fk, opts, tol=BWD_TOL)
+                assert np.array_equal(got.reshape(m.n, -1)[:, j], x), (name, j)
+                assert matrices.bwd_error(full, x, data[:, j]).max() <= BWD_TOL
+        ssids.ssids_free(ak, fk)
