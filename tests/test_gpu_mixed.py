@@ -156,3 +156,29 @@ def test_precision_switch_on_one_handle(lib):
             assert not np.array_equal(x, x1)
     L.spral_ssids_hip_destroy_numeric(num)
     ssids.ssids_free(ak)
+
+
+@pytest.mark.timeout(900)
+def test_kkt_stencil27_largest_grid(lib):
+    """The 27-point KKT workload at 56^3 (n = 205,800; 1.77e14 flops; a
+    71,616-row root front), the largest grid whose factors fit one GPU (80^3
+    needs 287 GB of factor entries with any ordering tried), in the matching-
+    based order with its scaling, FP64: exactly m_con negative pivots, full
+    rank, and a refined solution at the 1e-14 bar."""
+    g = 56
+    mc = int(round(g ** 3 * 88000 / 512000))
+    m = matrices.kkt_stencil27(g, mc)
+    opts = ssids.SSIDSOptions(ordering=2, scaling=3)
+    ak, ainf = ssids.ssids_analyse(False, m.n, m.ptr, m.row, opts, val=m.val)
+    assert ainf.flag >= 0, ainf
+    fk, inf = ssids.ssids_factor(False, m.val, ak, opts)
+    print(f"  56^3: flag {inf.flag}, delays {inf.num_delay}, "
+          f"pool peak {inf.pool_peak_bytes / 2 ** 30:.1f} GiB", flush=True)
+    assert inf.flag >= 0, inf
+    assert inf.num_neg == mc
+    assert inf.matrix_rank == m.n
+    xt, b = rhs(m, 1, 56)
+    x = np.array(b, copy=True)
+    ssids.ssids_solve_refine(x, ak, fk, opts, tol=BWD_TOL)
+    assert matrices.bwd_error(m.full, x, b).max() <= BWD_TOL
+    ssids.ssids_free(ak, fk)
