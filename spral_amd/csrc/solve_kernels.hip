@@ -38,6 +38,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "dev.hpp"
@@ -63,6 +64,10 @@ constexpr int kMrChunks = 4;  // 16-column MFMA outputs per wave: 64 right-hand 
 // the partial sums added in order at the end): 16 dependent MFMAs become
 // 4 chains of 4
 constexpr int kMrSplit = 4;
+// right-hand sides a wave substitutes through a diagonal block together when
+// there are more than four (independent chains of lane reads and FMAs in
+// one instruction stream; each right-hand side keeps its own sequence)
+constexpr int kDiagGroup = 4;
 
 // value of v in lane j (j uniform)
 __device__ inline double bcast_lane(double v, int j) {
@@ -678,16 +683,31 @@ __global__ __launch_bounds__(kThreads) void k_big_fwd_lvl(const SolveNode *sn, c
         red[16 * h + ri][16 * wave + kq + 4 * rg] =
             ((acc[h][0][rg] + acc[h][1][rg]) + acc[h][2][rg]) + acc[h][3][rg];
     __syncthreads();
-    for (int q = wave; q < nr; q += kGroups) {
-      double *X = PX + (size_t)(rc + q) * m;
-      double v = (lane < wb) ? X[j0 + lane] - red[q][lane] : 0.0;
+    auto diag = [&](auto grp) {
+      constexpr int G = decltype(grp)::value;
+      for (int q0 = wave * G; q0 < nr; q0 += kGroups * G) {
+        double v[G];
 #pragma unroll
-      for (int j = 0; j < kBlk; ++j) v = fma(-l[j], bcast_lane(v, j), v);
-      if (lane < wb) {
-        X[j0 + lane] = v;
-        x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
+        for (int r = 0; r < G; ++r) {
+          const int q = q0 + r;
+          v[r] = (q < nr && lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] - red[q][lane] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kBlk; ++j)
+#pragma unroll
+          for (int r = 0; r < G; ++r) v[r] = fma(-l[j], bcast_lane(v[r], j), v[r]);
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+          const int q = q0 + r;
+          if (q < nr && lane < wb) {
+            PX[(size_t)(rc + q) * m + j0 + lane] = v[r];
+            x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v[r];
+          }
+        }
       }
-    }
+    };
+    if (nr > kGroups) diag(std::integral_constant<int, kDiagGroup>());
+    else diag(std::integral_constant<int, 1>());
     __syncthreads();
   }
   if (tid == 0) raise_flag(flag + b);
@@ -761,16 +781,31 @@ __global__ __launch_bounds__(kThreads) void k_big_bwd_lvl(const SolveNode *sn, c
     double c[kBlk];
 #pragma unroll
     for (int i = 0; i < kBlk; ++i) c[i] = lb[i][lane];
-    for (int q = wave; q < nr; q += kGroups) {
-      double *X = PX + (size_t)(rc + q) * m;
-      double v = (lane < wb) ? X[j0 + lane] - red[q][lane] : 0.0;
+    auto diag = [&](auto grp) {
+      constexpr int G = decltype(grp)::value;
+      for (int q0 = wave * G; q0 < nr; q0 += kGroups * G) {
+        double v[G];
 #pragma unroll
-      for (int i = kBlk - 1; i >= 0; --i) v = fma(-c[i], bcast_lane(v, i), v);
-      if (lane < wb) {
-        X[j0 + lane] = v;
-        x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v;
+        for (int r = 0; r < G; ++r) {
+          const int q = q0 + r;
+          v[r] = (q < nr && lane < wb) ? PX[(size_t)(rc + q) * m + j0 + lane] - red[q][lane] : 0.0;
+        }
+#pragma unroll
+        for (int i = kBlk - 1; i >= 0; --i)
+#pragma unroll
+          for (int r = 0; r < G; ++r) v[r] = fma(-c[i], bcast_lane(v[r], i), v[r]);
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+          const int q = q0 + r;
+          if (q < nr && lane < wb) {
+            PX[(size_t)(rc + q) * m + j0 + lane] = v[r];
+            x[(size_t)(rc + q) * ldx + Pperm[j0 + lane]] = v[r];
+          }
+        }
       }
-    }
+    };
+    if (nr > kGroups) diag(std::integral_constant<int, kDiagGroup>());
+    else diag(std::integral_constant<int, 1>());
     __syncthreads();
   }
   if (tid == 0) raise_flag(flag + b);
