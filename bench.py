@@ -42,6 +42,9 @@ def parse():
                     help="instead of the default workload: factor+solve this many independent "
                          "KKT systems (n uniform in [200, 5000]) in one batched call, against a "
                          "loop of single calls")
+    ap.add_argument("--autotune", action="store_true",
+                    help="with --batch: pick the forest's tree-level schedule by timing each "
+                         "on the device (ssids_autotune) before the timed steps")
     ap.add_argument("--serve", type=float, default=0.0,
                     help="instead of the default workload: a Poisson stream of refactor (30%%) "
                          "and solve (70%%, k in 1/4/16/64) requests on resident KKT patterns "
@@ -144,9 +147,11 @@ def batch_bench(args):
     kord = args.kkt_ordering
     note("analyse")
     b, inf = batch.batch_analyse(mats, ssids.SSIDSOptions(ordering=kord,
-                                                          scaling=3 if kord == 2 else 0))
+                                                          scaling=3 if kord == 2 else 0),
+                                 autotune=args.autotune)
     assert inf.flag >= 0, inf
-    note(f"analysed: {int(b.akeep.nnodes)} fronts")
+    note(f"analysed: {int(b.akeep.nnodes)} fronts, level schedule {b.akeep.level_schedule}"
+         + (f" (autotuned: {b.akeep.schedule_ms})" if args.autotune else ""))
     rhs = [m.full @ rng.uniform(-1, 1, m.n) for m in mats]
     val = np.concatenate([m.val for m in mats])
     for _ in range(args.warmup):
@@ -191,6 +196,7 @@ def batch_bench(args):
     for ak, _ in singles:
         ssids.ssids_free(ak)
     n_total, flops, fronts = int(b.offsets[-1]), int(b.akeep.num_flops), int(b.akeep.nnodes)
+    sched, sched_ms = int(b.akeep.level_schedule), dict(b.akeep.schedule_ms)
     batch.batch_free(b)
     print(json.dumps({
         "metric": "batched factor+solve systems/s", "value": round(count / tb, 2),
@@ -204,6 +210,8 @@ def batch_bench(args):
                    "n_total": n_total, "num_flops": flops, "fronts": fronts,
                    "parallelism": "single-gpu"},
         "factor_ms_device": round(finf.factor_ms, 3),
+        "level_schedule": sched,
+        "schedule_ms": {str(k): round(v, 3) for k, v in sched_ms.items()},
         "loop_value": round(count / tl, 2) if tl else None,
         "loop_s": round(tl, 3) if tl else None,
         "speedup_vs_loop": round(tl / tb, 2) if tl else None, "max_bwd_err_raw": bwd,

@@ -45,6 +45,10 @@ struct spral_ssids_hip_options {
   int refine_max;          /* max FP64 refinement steps in solve_refine     */
   double refine_tol;       /* target scaled backward error                  */
   int timing;              /* 1: time the gather/Schur launches with events */
+  int level_schedule;      /* tree levels of the factorization (no reference *
+                            * counterpart): 0 the cheapest of 1-3 by steps   *
+                            * in sequence, 1 height from the leaves, 2       *
+                            * greedy by urgency, 3 layered heavy fronts      */
 };
 
 /* Interoperable subset of ssids_inform, as cpu_factor_stats
@@ -275,9 +279,12 @@ int spral_ssids_hip_node_pivots(const void *numeric, int *nelim_app, int *ndelay
 /* Planning helper (no reference counterpart; the factorization runs the
  * fronts of one tree level per launch sequence): the level of each of nnodes
  * fronts (0-based postorder; parent[i] > i, or -1 for a root; ncols = the
- * front's fully-summed columns), the assignment the analyse uses.  Returns
- * the number of levels, or -1 when parent is not a postorder forest. */
-int spral_ssids_hip_tree_levels(int nnodes, const int *parent, const int *ncols, int *level);
+ * front's fully-summed columns) under schedule mode 0-3, as the
+ * level_schedule option (0: the assignment the analyse uses by default).
+ * Returns the number of levels, or -1 when parent is not a postorder forest
+ * or mode is out of range. */
+int spral_ssids_hip_tree_levels(int nnodes, const int *parent, const int *ncols, int mode,
+                                int *level);
 
 /* Number of visible HIP devices (replaces the cudaGetDeviceCount probe of
  * src/hw_topology/guess_topology.cxx:65). */

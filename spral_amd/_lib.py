@@ -29,6 +29,7 @@ class Options(C.Structure):
         ("refine_max", C.c_int),
         ("refine_tol", C.c_double),
         ("timing", C.c_int),
+        ("level_schedule", C.c_int),
     ]
 
 
@@ -123,7 +124,8 @@ SIGNATURES = {
     "spral_ssids_hip_free_contrib": (None, [C.c_void_p]),
     "spral_ssids_hip_device_count": (C.c_int, []),
     "spral_ssids_hip_node_pivots": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
-    "spral_ssids_hip_tree_levels": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "spral_ssids_hip_tree_levels": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                              C.c_void_p]),
     "spral_ssids_hip_node_factors": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
     "spral_rb_peek": (C.c_int, [C.c_char_p] + [C.c_void_p] * 9),

@@ -70,8 +70,11 @@ def system_order(m, options):
     return o, None
 
 
-def batch_analyse(mats, options=None):
-    """One analyse of the block-diagonal pattern; returns (Batch, inform)."""
+def batch_analyse(mats, options=None, autotune=False):
+    """One analyse of the block-diagonal pattern; returns (Batch, inform).
+    autotune: pick the forest's tree-level schedule by timing each on the
+    device with the systems' values (ssids.ssids_autotune); the choice and
+    the times are in batch.akeep.level_schedule and .schedule_ms."""
     options = options or ssids.SSIDSOptions()
     n, ptr, row, val, sizes, offsets = block_diagonal(mats)
     res = [system_order(m, options) for m in mats]
@@ -86,6 +89,8 @@ def batch_analyse(mats, options=None):
         over["scaling"] = 0
     opts = ssids.SSIDSOptions(**{**options.__dict__, **over})
     ak, inf = ssids.ssids_analyse(False, n, ptr, row, opts, order=order)
+    if autotune and inf.flag >= 0:
+        ssids.ssids_autotune(ak, val, opts, scale=scale)
     return Batch(sizes=sizes, offsets=offsets, akeep=ak, options=opts,
                  orders=orders, scale=scale), inf
 

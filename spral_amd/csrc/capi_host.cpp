@@ -26,6 +26,7 @@ void spral_ssids_hip_default_options(struct spral_ssids_hip_options *opt) {
   opt->refine_max = 0;
   opt->refine_tol = 1e-14;
   opt->timing = 0;
+  opt->level_schedule = 0;
 }
 
 void *spral_ssids_hip_analyse(int n, const int64_t *ptr, const int *row,

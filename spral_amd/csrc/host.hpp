@@ -30,7 +30,9 @@ void expand_pattern(int n, const int64_t* ptr, const int* row,
 
 // Levels of the assembly tree for the numeric factorization (parent[i] > i
 // or -1; ncols = fully-summed columns per front): level[child] < level[parent].
-std::vector<int> tree_levels(const std::vector<int>& parent, const std::vector<int>& ncols);
+// mode: 0 the cheapest by the cost model, 1 height, 2 greedy, 3 layered.
+std::vector<int> tree_levels(const std::vector<int>& parent, const std::vector<int>& ncols,
+                             int mode = 0);
 
 // Orderings (1-based order(i) = pivot position of variable i).
 void order_grid_nd(int nx, int ny, int nz, int* order);

@@ -19,9 +19,11 @@
 //    such ancestors share a level, the gap between two such layers fitting
 //    the longest chain of lighter fronts between them; the lighter fronts
 //    sit as late as their parents allow.
-// The cheapest of the three by the cost model is kept; a new assignment must
-// beat height from the leaves by kMinGain, since the placement of the small
-// fronts also moves the lifetimes of their contribution blocks.
+// The cheapest of the three by the cost model is kept (mode 0); a new
+// assignment must beat height from the leaves by kMinGain, since the placement
+// of the small fronts also moves the lifetimes of their contribution blocks.
+// Modes 1-3 force height, greedy or layered (the schedule autotuner times
+// them on the device instead of trusting the model).
 #include <algorithm>
 #include <cstdint>
 #include <set>
@@ -134,11 +136,15 @@ std::vector<int> levels_layered(const std::vector<int> &parent, const std::vecto
 
 }  // namespace
 
-std::vector<int> tree_levels(const std::vector<int> &parent, const std::vector<int> &ncols) {
+std::vector<int> tree_levels(const std::vector<int> &parent, const std::vector<int> &ncols,
+                             int mode) {
   const int nn = (int)parent.size();
   std::vector<int> w(nn);
   for (int i = 0; i < nn; ++i) w[i] = std::max(1, (ncols[i] + kStepCols - 1) / kStepCols);
+  if (mode == 2) return levels_greedy(parent, w);
+  if (mode == 3) return levels_layered(parent, w);
   std::vector<int> best = levels_height(parent);
+  if (mode == 1) return best;
   const int64_t base = level_cost(best, w);
   int64_t bc = base;
   for (auto cand : {levels_greedy(parent, w), levels_layered(parent, w)}) {
@@ -154,12 +160,13 @@ std::vector<int> tree_levels(const std::vector<int> &parent, const std::vector<i
 }  // namespace ssids_hip
 
 extern "C" int spral_ssids_hip_tree_levels(int nnodes, const int *parent, const int *ncols,
-                                           int *level) {
-  if (nnodes < 0 || (nnodes > 0 && (!parent || !ncols || !level))) return -1;
+                                           int mode, int *level) {
+  if (nnodes < 0 || (nnodes > 0 && (!parent || !ncols || !level)) || mode < 0 || mode > 3)
+    return -1;
   std::vector<int> par(parent, parent + nnodes), nc(ncols, ncols + nnodes);
   for (int i = 0; i < nnodes; ++i)
     if (par[i] != -1 && (par[i] <= i || par[i] >= nnodes)) return -1;
-  const std::vector<int> lev = ssids_hip::tree_levels(par, nc);
+  const std::vector<int> lev = ssids_hip::tree_levels(par, nc, mode);
   int nl = 0;
   for (int i = 0; i < nnodes; ++i) {
     level[i] = lev[i];

@@ -382,7 +382,7 @@ struct Numeric {
 
 static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sptr,
                                 const int *sparent, const int64_t *rptr, const int *rlist,
-                                const int64_t *nptr, const int64_t *nlist) {
+                                const int64_t *nptr, const int64_t *nlist, int schedule) {
   auto S = std::make_unique<Symbolic>();
   S->device = device;
   S->n = n;
@@ -462,7 +462,7 @@ static Symbolic *build_symbolic(int device, int n, int sa, int en, const int *sp
   S->acol_off[nn] = (int64_t)acol.size();
   S->naval_min = amax;
   // tree levels (children precede parents: postorder)
-  const std::vector<int> lev = tree_levels(S->parent, S->sn);
+  const std::vector<int> lev = tree_levels(S->parent, S->sn, schedule);
   int nlev = 0;
   for (int i = 0; i < nn; ++i) nlev = std::max(nlev, lev[i] + 1);
   S->lvl_ptr.assign(nlev + 1, 0);
@@ -1912,10 +1912,15 @@ void *spral_ssids_hip_create_symbolic(int device, int n, int sa, int en, const i
                                       const struct spral_ssids_hip_options *options,
                                       int *flag) {
   ApiShared api_guard;
-  (void)options;
   *flag = 0;
+  const int schedule = options ? options->level_schedule : 0;
+  if (schedule < 0 || schedule > 3) {
+    *flag = -1;  // no such level schedule
+    return nullptr;
+  }
   try {
-    return build_symbolic(device, n, sa, en, sptr, sparent, rptr, rlist, nptr, nlist);
+    return build_symbolic(device, n, sa, en, sptr, sparent, rptr, rlist, nptr, nlist,
+                          schedule);
   } catch (HipError &e) {
     *flag = -51;
   } catch (std::bad_alloc &) {

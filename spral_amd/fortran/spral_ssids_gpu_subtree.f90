@@ -26,7 +26,7 @@ module spral_ssids_gpu_subtree
      integer(C_INT)    :: pivot_method, failed_pivot_method, schur_precision
      integer(C_INT)    :: refine_max
      real(C_DOUBLE)    :: refine_tol
-     integer(C_INT)    :: timing
+     integer(C_INT)    :: timing, level_schedule
   end type hip_options
 
   !> struct spral_ssids_hip_stats

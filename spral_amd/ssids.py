@@ -21,7 +21,7 @@ the auction scaling, 3 the scaling of ordering 2, >= 4 norm equilibration on
 the device.
 """
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -73,6 +73,10 @@ class SSIDSOptions:
     use_gpu: bool = True
     schur_precision: int = 0
     device: int = 0
+    # tree levels of the factorization (no reference counterpart): 0 the
+    # cheapest by the cost model, 1 height from the leaves, 2 greedy, 3
+    # layered; ssids_autotune picks one by timing them on the device
+    level_schedule: int = 0
 
     def to_c(self):
         o = _lib.default_options()
@@ -84,6 +88,7 @@ class SSIDSOptions:
         o.pivot_method = min(3, max(1, self.pivot_method))
         o.failed_pivot_method = min(2, max(1, self.failed_pivot_method))
         o.schur_precision = self.schur_precision
+        o.level_schedule = self.level_schedule
         return o
 
 
@@ -129,6 +134,8 @@ class AKeep:
     inform: SSIDSInform = field(default_factory=SSIDSInform)
     nz: int = 0
     scaling: np.ndarray = None  # matching scaling of ordering = 2 (user order)
+    level_schedule: int = 0    # schedule of the symbolic handle (ssids_autotune)
+    schedule_ms: dict = field(default_factory=dict)  # timed schedules: ms each
 
 
 @dataclass
@@ -247,6 +254,7 @@ def ssids_analyse(check, n, ptr, row, options=None, order=None, grid=None, val=N
         if not sym:
             inform.flag = f.value
         ak.symbolic = sym
+        ak.level_schedule = options.level_schedule
     ak.inform = inform
     return ak, inform
 
@@ -572,6 +580,68 @@ def ssids_free(akeep=None, fkeep=None):
     return 0
 
 
+def ssids_autotune(akeep, val, options=None, schedules=(1, 2, 3), reps=2, posdef=False,
+                   scale=None):
+    """Schedule autotuner (no reference counterpart).  Builds a device
+    symbolic handle per tree-level schedule (options.level_schedule: 1 height
+    from the leaves, 2 greedy by urgency, 3 layered heavy fronts), factors val
+    on each (one factorization, then reps refactors on the same handle, as a
+    resident pattern is refactored), and keeps in akeep the handle whose best
+    refactor took the least device time, replacing the analyse's handle.
+    Every schedule factors each front from the same inputs in the same order,
+    so the factors do not depend on the choice.  Call after ssids_analyse and
+    before the first ssids_factor of akeep; scale is passed to each
+    ssids_factor as given.  Returns {schedule: ms}; akeep records the choice
+    (level_schedule) and the times (schedule_ms)."""
+    options = options or SSIDSOptions()
+    if akeep.inform.flag < 0 or akeep.symbolic is None:
+        raise ValueError("ssids_autotune needs a successful ssids_analyse on the GPU")
+    if not schedules or reps < 1:
+        raise ValueError("ssids_autotune needs schedules and reps >= 1")
+    L = _lib.lib()
+    original = akeep.symbolic
+    handles, times = {}, {}
+    try:
+        for s in schedules:
+            o = replace(options, level_schedule=int(s))
+            f = C.c_int(0)
+            h = L.spral_ssids_hip_create_symbolic(
+                o.device, akeep.n, 1, akeep.nnodes + 1, _as_i32(akeep.sptr),
+                _as_i32(akeep.sparent), _as_i64(akeep.rptr), _as_i32(akeep.rlist),
+                _as_i64(akeep.nptr), _as_i64(akeep.nlist), C.byref(o.to_c()), C.byref(f))
+            if not h:
+                raise RuntimeError(f"symbolic handle for schedule {s}: flag {f.value}")
+            handles[int(s)] = h
+            akeep.symbolic = h
+            fk, best = None, float("inf")
+            try:
+                for r in range(reps + 1):
+                    fk, inf = ssids_factor(posdef, val, akeep, o, fkeep=fk,
+                                           scale=None if scale is None else scale.copy())
+                    if inf.flag < 0:
+                        raise RuntimeError(f"factorization under schedule {s}: flag {inf.flag}")
+                    if r > 0:
+                        best = min(best, float(inf.factor_ms))
+            finally:
+                if fk is not None:
+                    ssids_free(fkeep=fk)
+            times[int(s)] = best
+    except Exception:
+        for h in handles.values():
+            L.spral_ssids_hip_destroy_symbolic(h)
+        akeep.symbolic = original
+        raise
+    pick = min(times, key=lambda s: (times[s], s))
+    for s, h in handles.items():
+        if s != pick:
+            L.spral_ssids_hip_destroy_symbolic(h)
+    L.spral_ssids_hip_destroy_symbolic(original)
+    akeep.symbolic = handles[pick]
+    akeep.level_schedule = pick
+    akeep.schedule_ms = times
+    return times
+
+
 # ---------------------------------------------------------------------------
 # Analyse residency: an analysed pattern saved and reloaded, and a cache of
 # analyses keyed by pattern for a resident service.  Arrays only (numpy .npz,
@@ -587,16 +657,22 @@ def ssids_save_akeep(akeep, path):
     np.savez(path, n=akeep.n, nnodes=akeep.nnodes, num_factor=akeep.num_factor,
              num_flops=akeep.num_flops, nz=akeep.nz, flag=akeep.inform.flag,
              maxfront=akeep.inform.maxfront, maxsupernode=akeep.inform.maxsupernode,
-             matrix_rank=akeep.inform.matrix_rank,
+             matrix_rank=akeep.inform.matrix_rank, level_schedule=akeep.level_schedule,
              **{k: getattr(akeep, k) for k in _AKEEP_ARRAYS}, **extra)
 
 
 def ssids_load_akeep(path, options=None):
     """Reads a saved analyse; with options.use_gpu the device symbolic handle
-    is rebuilt from the arrays (no ordering or analyse is rerun)."""
+    is rebuilt from the arrays (no ordering or analyse is rerun), under the
+    saved level schedule unless options.level_schedule names another."""
     options = options or SSIDSOptions()
     z = np.load(path, allow_pickle=False)
     ak = AKeep(n=int(z["n"]))
+    if "level_schedule" in z.files:
+        ak.level_schedule = int(z["level_schedule"])
+    if options.level_schedule == 0 and ak.level_schedule != 0:
+        options = replace(options, level_schedule=ak.level_schedule)
+    ak.level_schedule = options.level_schedule
     for k in _AKEEP_ARRAYS:
         setattr(ak, k, z[k])
     ak.nnodes, ak.nz = int(z["nnodes"]), int(z["nz"])

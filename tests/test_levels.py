@@ -6,12 +6,12 @@ import numpy as np
 from spral_amd import _lib, matrices, ssids
 
 
-def tree_levels(parent, ncols):
+def tree_levels(parent, ncols, mode=0):
     parent = np.ascontiguousarray(parent, np.int32)
     ncols = np.ascontiguousarray(ncols, np.int32)
     lev = np.zeros(parent.size, np.int32)
     nl = _lib.lib().spral_ssids_hip_tree_levels(parent.size, parent.ctypes.data,
-                                                ncols.ctypes.data, lev.ctypes.data)
+                                                ncols.ctypes.data, mode, lev.ctypes.data)
     return nl, lev
 
 
@@ -107,3 +107,23 @@ def test_grid64_tree(lib):
         if parent[i] >= 0:
             path[parent[i]] = max(path[parent[i]], path[i])
     assert path.max() == 161
+
+
+def test_forced_schedules(lib):
+    """Modes 1-3 force height from the leaves, greedy and layered; mode 0
+    keeps the cheapest of them (4 per step in sequence, 1 per level); other
+    modes are rejected."""
+    parent = [1, 2, 6, 4, 5, 6, -1]
+    ncols = [32, 640, 32, 640, 32, 32, 640]
+    w = steps(ncols)
+    levs = {}
+    for mode in (1, 2, 3):
+        nl, lev = tree_levels(parent, ncols, mode)
+        check_valid(parent, lev, nl)
+        levs[mode] = lev
+    assert (levs[1] == height_levels(parent)).all()
+    cost = lambda lev: 4 * level_steps(lev, w) + int(lev.max()) + 1
+    nl0, lev0 = tree_levels(parent, ncols, 0)
+    assert cost(lev0) == min(cost(lv) for lv in levs.values()) < cost(levs[1])
+    assert tree_levels(parent, ncols, 4)[0] == -1
+    assert tree_levels(parent, ncols, -1)[0] == -1
